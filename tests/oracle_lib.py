@@ -1,0 +1,388 @@
+"""ctypes bindings for the oracle (oracle/liboracle.so) and the compiled reference
+(oracle/_ref/libaleator_ref.so).  Test infrastructure only: imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the product."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libaleator_ref.so")
+
+dp = C.POINTER(C.c_double)
+u32p = C.POINTER(C.c_uint32)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+u64p = C.POINTER(C.c_uint64)
+szp = C.POINTER(C.c_size_t)
+
+KCOULOMB = 167100.9468982874
+TRUNCATED, SHIFTED, LFS = 0, 1, 2
+
+
+def ptr(a, t=dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class OrcLattice(C.Structure):
+    _fields_ = [("m", C.c_double * 9), ("orthorhombic", C.c_int), ("reduced", C.c_double * 9)]
+
+
+class OrcParticles(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("x", dp), ("y", dp), ("z", dp), ("q", dp), ("species", u32p)]
+
+
+class OrcLJ(C.Structure):
+    _fields_ = [("epsilon", dp), ("sigma", dp), ("n_species", C.c_uint32), ("cutoff", C.c_double),
+                ("truncation", C.c_int), ("mixing", C.c_int)]
+
+
+class OrcKset(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("k", dp), ("weight", dp), ("nvec", i32p)]
+
+
+class OrcEwaldState(C.Structure):
+    _fields_ = [("lat", OrcLattice), ("alpha", C.c_double), ("terms", OrcKset), ("volume", C.c_double),
+                ("s_re", dp), ("s_im", dp), ("self_energy", C.c_double),
+                ("exclusion_energy", C.c_double)]
+
+
+class OrcSite(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("charge", C.c_double)]
+
+
+class OrcMolSite(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("lj_species", C.c_uint32),
+                ("charge", C.c_double)]
+
+
+class OrcBreakdown(C.Structure):
+    _fields_ = [("real", C.c_double), ("reciprocal", C.c_double), ("self", C.c_double),
+                ("exclusion", C.c_double), ("surface", C.c_double)]
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        _oracle = C.CDLL(ORACLE_SO)
+        _oracle.orc_lj_particle_energy.restype = C.c_double
+        _oracle.orc_lj_tail_energy.restype = C.c_double
+        _oracle.orc_lj_total_energy.restype = C.c_double
+        _oracle.orc_lj_total_virial.restype = C.c_double
+        _oracle.orc_lj_pair_energy.restype = C.c_double
+        _oracle.orc_lattice_volume.restype = C.c_double
+        _oracle.orc_perpendicular_width.restype = C.c_double
+        _oracle.orc_ewald_real_particle_energy.restype = C.c_double
+        _oracle.orc_ewald_state_committed.restype = C.c_double
+        _oracle.orc_ewald_state_delta.restype = C.c_double
+        _oracle.orc_ewald_state_recompute.restype = C.c_double
+        _oracle.orc_grid_interpolate.restype = C.c_double
+        _oracle.orc_widom.restype = C.c_double
+        _oracle.orc_rng_next_uniform.restype = C.c_double
+        _oracle.orc_rng_next_u64.restype = C.c_uint64
+        _oracle.orc_pr_fugacity.restype = C.c_double
+    return _oracle
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not have_ref():
+            raise RuntimeError(f"{REF_SO} missing: run `make -C oracle ref` where the reference exists")
+        _ref = C.CDLL(REF_SO)
+        _ref.ref_ewald_state_create.restype = C.c_void_p
+        _ref.ref_ewald_state_destroy.argtypes = [C.c_void_p]
+        for f in ("ref_ewald_state_propose", "ref_ewald_state_committed", "ref_ewald_state_recompute"):
+            getattr(_ref, f).restype = C.c_double
+        _ref.ref_ewald_state_propose.argtypes = [C.c_void_p, dp, C.c_int, dp, C.c_int]
+        _ref.ref_ewald_state_commit.argtypes = [C.c_void_p, dp, C.c_int, dp, C.c_int]
+        _ref.ref_ewald_state_committed.argtypes = [C.c_void_p]
+        _ref.ref_ewald_state_raw.argtypes = [C.c_void_p, dp, C.c_uint64]
+        _ref.ref_ewald_state_raw.restype = C.c_uint64
+        _ref.ref_ewald_state_resync.argtypes = [C.c_void_p, C.c_uint64, dp, dp, dp, dp, u64p, u64p,
+                                                C.c_uint64]
+        _ref.ref_ewald_state_recompute.argtypes = _ref.ref_ewald_state_resync.argtypes
+    return _ref
+
+
+# --------------------------------------------------------------------------- model objects
+@dataclass
+class Framework:
+    m: np.ndarray          # (3,3) row vectors
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    q: np.ndarray
+    species: np.ndarray    # uint32
+
+    @property
+    def n(self):
+        return len(self.x)
+
+
+@dataclass
+class LJ:
+    eps: np.ndarray
+    sigma: np.ndarray
+    cutoff: float
+    truncation: int = TRUNCATED
+    mixing: int = 0
+
+
+def lattice(m) -> OrcLattice:
+    lat = OrcLattice()
+    mm = np.ascontiguousarray(m, dtype=np.float64)
+    oracle().orc_lattice_init(C.byref(lat), ptr(mm))
+    return lat
+
+
+def particles(fw: Framework) -> tuple[OrcParticles, tuple]:
+    keep = tuple(np.ascontiguousarray(a) for a in (fw.x, fw.y, fw.z, fw.q))
+    sp = np.ascontiguousarray(fw.species, dtype=np.uint32)
+    p = OrcParticles(fw.n, ptr(keep[0]), ptr(keep[1]), ptr(keep[2]), ptr(keep[3]), ptr(sp, u32p))
+    return p, keep + (sp,)
+
+
+def lj_struct(lj: LJ) -> tuple[OrcLJ, tuple]:
+    e = np.ascontiguousarray(lj.eps, dtype=np.float64)
+    s = np.ascontiguousarray(lj.sigma, dtype=np.float64)
+    return OrcLJ(ptr(e), ptr(s), len(e), lj.cutoff, lj.truncation, lj.mixing), (e, s)
+
+
+def synth_framework(m, n_atoms, seed=1, stream=0, min_dist=1.0,
+                    proportions=(0.45, 0.30, 0.17, 0.08), charge_sigma=0.0, charge_stream=2):
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    x, y, z, q = (np.zeros(n_atoms) for _ in range(4))
+    sp = np.zeros(n_atoms, dtype=np.uint32)
+    props = np.ascontiguousarray(proportions, dtype=np.float64)
+    rc = oracle().orc_synth_framework(ptr(m), C.c_size_t(n_atoms), C.c_uint64(seed), C.c_uint64(stream),
+                                      C.c_double(min_dist), ptr(props), C.c_uint32(len(props)),
+                                      C.c_double(charge_sigma), C.c_uint64(charge_stream),
+                                      ptr(x), ptr(y), ptr(z), ptr(q), ptr(sp, u32p))
+    if rc != 0:
+        raise RuntimeError("synthetic framework placement failed")
+    return Framework(m, x, y, z, q, sp)
+
+
+def cubic(a):
+    return np.diag([a, a, a]).astype(np.float64)
+
+
+def cell_from_params(a, b, c, alpha, beta, gamma):
+    """Row-vector lattice matrix from edges (A) and angles (degrees)."""
+    al, be, ga = np.radians([alpha, beta, gamma])
+    ca, cb, cg, sg = np.cos(al), np.cos(be), np.cos(ga), np.sin(ga)
+    cy = (ca - cb * cg) / sg
+    cz2 = 1.0 - cb * cb - cy * cy
+    return np.array([[a, 0, 0], [b * cg, b * sg, 0], [c * cb, c * cy, c * np.sqrt(cz2)]], dtype=np.float64)
+
+
+# --------------------------------------------------------------------------- oracle wrappers
+def uniforms(seed, stream, first, n):
+    out = np.zeros(n)
+    oracle().orc_uniforms(C.c_uint64(seed), C.c_uint64(stream), C.c_uint64(first), C.c_uint64(n), ptr(out))
+    return out
+
+
+def min_supercell(m, rc):
+    n = (C.c_int * 3)()
+    lat = lattice(m)
+    oracle().orc_min_supercell_replication(C.byref(lat), C.c_double(rc), n)
+    return tuple(n)
+
+
+def replicate(fw: Framework, reps):
+    lat = lattice(fw.m)
+    p, keep = particles(fw)
+    out = OrcParticles()
+    om = np.zeros((3, 3))
+    oracle().orc_replicate_supercell(C.byref(p), C.byref(lat), reps[0], reps[1], reps[2], C.byref(out), ptr(om))
+    n = out.n
+    res = Framework(om, np.ctypeslib.as_array(out.x, (n,)).copy(), np.ctypeslib.as_array(out.y, (n,)).copy(),
+                    np.ctypeslib.as_array(out.z, (n,)).copy(), np.ctypeslib.as_array(out.q, (n,)).copy(),
+                    np.ctypeslib.as_array(out.species, (n,)).copy())
+    oracle().orc_particles_free(C.byref(out))
+    return res
+
+
+def grid_counts(m, spacing):
+    n = np.zeros(3, dtype=np.int32)
+    lat = lattice(m)
+    oracle().orc_grid_point_counts(C.byref(lat), C.c_double(spacing), ptr(n, i32p))
+    return tuple(int(v) for v in n)
+
+
+def grid_build(lj: LJ, fw: Framework, guests, counts, cap=1e5):
+    lat = lattice(fw.m)
+    p, keep = particles(fw)
+    l, keep2 = lj_struct(lj)
+    g = np.ascontiguousarray(sorted(set(guests)), dtype=np.uint32)
+    n = np.ascontiguousarray(counts, dtype=np.int32)
+    out = np.zeros((len(g), n[2], n[1], n[0]))
+    oracle().orc_grid_build(C.byref(l), C.byref(p), C.byref(lat), ptr(g, u32p), len(g), ptr(n, i32p),
+                            C.c_double(cap), ptr(out))
+    return out
+
+
+def grid_nodes_lj(lj: LJ, fw: Framework, grid_m, counts, guest, nodes, cap=1e5):
+    lat = lattice(fw.m)
+    p, keep = particles(fw)
+    l, keep2 = lj_struct(lj)
+    gm = np.ascontiguousarray(grid_m, dtype=np.float64)
+    n = np.ascontiguousarray(counts, dtype=np.int32)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+    out = np.zeros(len(nodes))
+    oracle().orc_grid_nodes_lj(C.byref(l), C.byref(p), C.byref(lat), ptr(gm), ptr(n, i32p), C.c_uint32(guest),
+                               ptr(nodes, i64p), C.c_size_t(len(nodes)), C.c_double(cap), ptr(out))
+    return out
+
+
+class EwaldState:
+    """Oracle EwaldIncrementalState over a framework (restates ewald_incremental_state.cc)."""
+
+    def __init__(self, fw: Framework, alpha, kmax):
+        self.st = OrcEwaldState()
+        lat = lattice(fw.m)
+        p, self._keep = particles(fw)
+        if oracle().orc_ewald_state_init(C.byref(self.st), C.c_double(alpha), C.c_int(kmax), C.byref(p),
+                                         C.byref(lat)) != 0:
+            raise RuntimeError("ewald state init failed")
+
+    def __del__(self):
+        try:
+            oracle().orc_ewald_state_free(C.byref(self.st))
+        except Exception:
+            pass
+
+    @property
+    def nk(self):
+        return self.st.terms.n
+
+    @staticmethod
+    def _sites(s):
+        s = np.asarray(s, dtype=np.float64).reshape(-1, 4)
+        arr = (OrcSite * max(len(s), 1))()
+        for i, r in enumerate(s):
+            arr[i] = OrcSite(*r)
+        return arr, len(s)
+
+    def propose(self, removed, added):
+        r, nr = self._sites(removed)
+        a, na = self._sites(added)
+        return oracle().orc_ewald_state_delta(C.byref(self.st), r, C.c_size_t(nr), a, C.c_size_t(na),
+                                              None, None, None, None, None)
+
+    def commit(self, removed, added):
+        r, nr = self._sites(removed)
+        a, na = self._sites(added)
+        oracle().orc_ewald_state_commit(C.byref(self.st), r, C.c_size_t(nr), a, C.c_size_t(na))
+
+    def committed(self):
+        return oracle().orc_ewald_state_committed(C.byref(self.st))
+
+    def raw(self):
+        out = np.zeros(2 * self.nk + 2)
+        oracle().orc_ewald_state_raw(C.byref(self.st), ptr(out))
+        return out
+
+    def _mol(self, fw, molecules):
+        off = np.zeros(len(molecules) + 1, dtype=np.uint64)
+        flat = []
+        for i, m in enumerate(molecules):
+            flat.extend(m)
+            off[i + 1] = len(flat)
+        flat = np.ascontiguousarray(flat if flat else [0], dtype=np.uint64)
+        return off, flat
+
+    def resync(self, fw: Framework, molecules):
+        p, keep = particles(fw)
+        off, flat = self._mol(fw, molecules)
+        oracle().orc_ewald_state_resync(C.byref(self.st), C.byref(p), ptr(off, szp), ptr(flat, szp),
+                                        C.c_size_t(len(molecules)))
+
+    def recompute(self, fw: Framework, molecules):
+        p, keep = particles(fw)
+        off, flat = self._mol(fw, molecules)
+        return oracle().orc_ewald_state_recompute(C.byref(self.st), C.byref(p), ptr(off, szp),
+                                                  ptr(flat, szp), C.c_size_t(len(molecules)))
+
+    def kvectors(self):
+        n = self.nk
+        return (np.ctypeslib.as_array(self.st.terms.k, (n, 3)).copy(),
+                np.ctypeslib.as_array(self.st.terms.weight, (n,)).copy(),
+                np.ctypeslib.as_array(self.st.terms.nvec, (n, 3)).copy())
+
+    def structure_factor(self):
+        n = self.nk
+        return (np.ctypeslib.as_array(self.st.s_re, (n,)).copy(),
+                np.ctypeslib.as_array(self.st.s_im, (n,)).copy())
+
+
+def coulomb_nodes(st: EwaldState, rc_real, fw: Framework, grid_m, counts, nodes):
+    p, keep = particles(fw)
+    gm = np.ascontiguousarray(grid_m, dtype=np.float64)
+    n = np.ascontiguousarray(counts, dtype=np.int32)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+    real = np.zeros(len(nodes))
+    rec = np.zeros(len(nodes))
+    oracle().orc_grid_nodes_coulomb(C.byref(st.st), C.c_double(rc_real), C.byref(p), ptr(gm), ptr(n, i32p),
+                                    ptr(nodes, i64p), C.c_size_t(len(nodes)), ptr(real), ptr(rec))
+    return real, rec
+
+
+def widom(lj: LJ, fw: Framework, sites, seed, stream, n_trials, temperature, ewald: EwaldState | None = None,
+          rc_real=0.0, first_call=0):
+    """sites: list of (x, y, z, lj_species, charge).  Returns (K_H, dU per trial)."""
+    lat = lattice(fw.m)
+    p, keep = particles(fw)
+    l, keep2 = lj_struct(lj)
+    arr = (OrcMolSite * len(sites))()
+    for i, s in enumerate(sites):
+        arr[i] = OrcMolSite(s[0], s[1], s[2], int(s[3]), s[4])
+    du = np.zeros(n_trials)
+    kh = oracle().orc_widom(C.byref(l), C.byref(p), C.byref(lat), arr, C.c_size_t(len(sites)),
+                            C.byref(ewald.st) if ewald is not None else None, C.c_double(rc_real),
+                            C.c_uint64(seed), C.c_uint64(stream), C.c_uint64(first_call), C.c_uint64(n_trials),
+                            C.c_double(temperature), ptr(du))
+    return kh, du
+
+
+# --------------------------------------------------------------------------- standard workloads
+C0_TYPES = ((52.8, 3.43), (30.2, 3.12), (22.1, 2.57), (62.4, 2.46))
+METHANE = (148.0, 3.73)
+CO2_C = (27.0, 2.80)
+CO2_O = (79.0, 3.05)
+
+
+def c0_system(n_atoms=424, a=25.83):
+    fw = synth_framework(cubic(a), n_atoms, seed=1, stream=0)
+    eps = np.array([t[0] for t in C0_TYPES] + [METHANE[0]])
+    sig = np.array([t[1] for t in C0_TYPES] + [METHANE[1]])
+    return fw, LJ(eps, sig, 12.8, TRUNCATED)
+
+
+def co2_sites():
+    """Rigid linear CO2 along z: O(-), C, O(+); species 4 = C, 5 = O."""
+    return [(0.0, 0.0, -1.16, 5, -0.35), (0.0, 0.0, 0.0, 4, 0.70), (0.0, 0.0, 1.16, 5, -0.35)]
+
+
+def c1_system(n_atoms=600):
+    m = cell_from_params(24.0, 24.0, 24.0, 60.0, 60.0, 60.0)
+    fw = synth_framework(m, n_atoms, seed=1, stream=0, charge_sigma=0.5, charge_stream=2)
+    eps = np.array([t[0] for t in C0_TYPES] + [CO2_C[0], CO2_O[0]])
+    sig = np.array([t[1] for t in C0_TYPES] + [CO2_C[1], CO2_O[1]])
+    return fw, LJ(eps, sig, 12.0, TRUNCATED)
