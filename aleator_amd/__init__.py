@@ -1,0 +1,363 @@
+"""aleator_amd -- MI355X (gfx950) adsorption hot path behind the C ABI of include/aleator_gpu.h.
+
+Python binding (ctypes) of the in-tree HIP library aleator_amd/lib/libaleator_gpu.so.  The
+library is the only compute path: if it is missing, importing the binding raises instead of
+falling back to anything else.  Build it with `make -C aleator_amd` (or __graft_entry__.build()).
+
+Objects mirror the reference's operator interfaces:
+  Framework.grid(...)        FrameworkEnergyGrid constructor (framework_energy_grid.cc:193-249)
+  Framework.widom(...)       MonteCarloEngine::widomInsertionHenryCoefficient (monte_carlo_engine.cc:850-917)
+  EwaldState                 forcefield::EwaldIncrementalState (ewald_incremental_state.hpp:70-171)
+Errors raise the Python analogue of the reference's C++ exception (ValueError for
+std::invalid_argument, IndexError for std::out_of_range, RuntimeError otherwise).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["Context", "Framework", "EwaldState", "grid_point_counts", "synth_framework", "library_path",
+           "FP64", "MIXED", "TRUNCATED", "SHIFTED", "LINEAR_FORCE_SHIFTED", "NotImplementedInReference"]
+
+FP64, MIXED = 0, 1
+TRUNCATED, SHIFTED, LINEAR_FORCE_SHIFTED = 0, 1, 2
+LORENTZ_BERTHELOT, GEOMETRIC = 0, 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libaleator_gpu.so")
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+class NotImplementedInReference(RuntimeError):
+    """aleator::NotImplemented (src/core/exceptions.hpp:13-17)."""
+
+
+class DeviceError(RuntimeError):
+    """HIP runtime failure."""
+
+
+_dp = C.POINTER(C.c_double)
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class _Lattice(C.Structure):
+    _fields_ = [("m", C.c_double * 9)]
+
+
+class _Particles(C.Structure):
+    _fields_ = [("x", _dp), ("y", _dp), ("z", _dp), ("charge", _dp), ("species", _u32p), ("n", C.c_uint64)]
+
+
+class _LJ(C.Structure):
+    _fields_ = [("epsilon", _dp), ("sigma", _dp), ("n_species", C.c_uint32), ("cutoff", C.c_double),
+                ("truncation", C.c_int32), ("mixing", C.c_int32)]
+
+
+class _Ewald(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("real_space_cutoff", C.c_double), ("kmax", C.c_int32)]
+
+
+class _ChargedSite(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("charge", C.c_double)]
+
+
+class _MolSite(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("lj_species", C.c_uint32),
+                ("charge", C.c_double)]
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(f"{_LIB_PATH} is missing: build the HIP library with `make -C aleator_amd`")
+    lib = C.CDLL(_LIB_PATH)
+    lib.aleator_gpu_last_error.restype = C.c_char_p
+    lib.aleator_gpu_stream.restype = C.c_void_p
+    lib.aleator_gpu_stream.argtypes = [C.c_void_p]
+    _lib = lib
+    return lib
+
+
+def _check(status: int):
+    if status == 0:
+        return
+    msg = _load().aleator_gpu_last_error().decode(errors="replace")
+    raise {1: ValueError, 2: IndexError, 3: RuntimeError, 4: NotImplementedInReference,
+           5: DeviceError}.get(status, RuntimeError)(msg)
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def _lattice(m) -> _Lattice:
+    m = np.ascontiguousarray(m, dtype=np.float64).reshape(9)
+    return _Lattice((C.c_double * 9)(*m))
+
+
+def grid_point_counts(lattice, spacing: float) -> tuple[int, int, int]:
+    """max(1, lround(L_perp/spacing)) per axis (framework_energy_grid.cc:207-211)."""
+    n = np.zeros(3, dtype=np.int32)
+    lat = _lattice(lattice)
+    _check(_load().aleator_gpu_grid_point_counts(C.byref(lat), C.c_double(spacing), _p(n, _i32p)))
+    return tuple(int(v) for v in n)
+
+
+def synth_framework(lattice, n_atoms: int, seed=1, stream=0, min_dist=1.0,
+                    proportions=(0.45, 0.30, 0.17, 0.08), charge_sigma=0.0, charge_stream=2):
+    """Synthetic framework (no reference counterpart): returns (x, y, z, charge, species)."""
+    x, y, z, q = (np.zeros(n_atoms) for _ in range(4))
+    sp = np.zeros(n_atoms, dtype=np.uint32)
+    props = np.ascontiguousarray(proportions, dtype=np.float64)
+    lat = _lattice(lattice)
+    _check(_load().aleator_synth_framework(C.byref(lat), C.c_uint64(n_atoms), C.c_uint64(seed),
+                                           C.c_uint64(stream), C.c_double(min_dist), _p(props),
+                                           C.c_uint32(len(props)), C.c_double(charge_sigma),
+                                           C.c_uint64(charge_stream), _p(x), _p(y), _p(z), _p(q), _p(sp, _u32p)))
+    return x, y, z, q, sp
+
+
+class Context:
+    """One GPU and its HIP stream (no reference counterpart)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(_load().aleator_gpu_init(C.c_int32(device), C.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return _load().aleator_gpu_stream(self._h)
+
+    def timing(self, enable: bool) -> tuple[float, int]:
+        """HIP-event time (ms) and count of dominant-kernel launches since the last call;
+        then enables or disables recording."""
+        ms = C.c_double()
+        n = C.c_uint64()
+        _check(_load().aleator_gpu_timing(self._h, C.c_int32(1 if enable else 0), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if self._h:
+            _load().aleator_gpu_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Framework:
+    """Device-resident rigid framework with its LJ table and (optionally) Ewald S(k)."""
+
+    def __init__(self, ctx: Context, lattice, x, y, z, species, epsilon, sigma, cutoff,
+                 truncation=TRUNCATED, mixing=LORENTZ_BERTHELOT, charge=None, ewald=None):
+        self.ctx = ctx
+        self.lattice = np.ascontiguousarray(lattice, dtype=np.float64).reshape(3, 3)
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z)]
+        sp = np.ascontiguousarray(species, dtype=np.uint32)
+        q = np.ascontiguousarray(charge, dtype=np.float64) if charge is not None else None
+        eps = np.ascontiguousarray(epsilon, dtype=np.float64)
+        sig = np.ascontiguousarray(sigma, dtype=np.float64)
+        parts = _Particles(_p(keep[0]), _p(keep[1]), _p(keep[2]), _p(q) if q is not None else None,
+                           _p(sp, _u32p), len(keep[0]))
+        lj = _LJ(_p(eps), _p(sig), len(eps), cutoff, truncation, mixing)
+        ew = _Ewald(*ewald) if ewald is not None else None
+        lat = _lattice(self.lattice)
+        self._h = C.c_void_p()
+        _check(_load().aleator_gpu_framework_create(ctx.handle, C.byref(lat), C.byref(parts), C.byref(lj),
+                                                    C.byref(ew) if ew is not None else None, C.byref(self._h)))
+        self.n_species = len(eps)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def num_k(self) -> int:
+        n = C.c_uint64()
+        _check(_load().aleator_gpu_framework_num_k(self._h, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if self._h:
+            _load().aleator_gpu_framework_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid(self, guests, n_points, coulomb=False, precision=FP64, cap=1.0e5, grid_lattice=None):
+        """Energy tables [len(guests) (+1 Coulomb)][nz][ny][nx] in K (Coulomb: K/e)."""
+        g = np.ascontiguousarray(guests, dtype=np.uint32)
+        n = np.ascontiguousarray(n_points, dtype=np.int32)
+        out = np.zeros((len(g) + (1 if coulomb else 0), n[2], n[1], n[0]))
+        gl = _lattice(grid_lattice) if grid_lattice is not None else None
+        _check(_load().aleator_gpu_grid_build(self._h, C.byref(gl) if gl is not None else None,
+                                              _p(g, _u32p) if len(g) else None, C.c_int32(len(g)),
+                                              C.c_int32(1 if coulomb else 0), _p(n, _i32p), C.c_int32(precision),
+                                              C.c_double(cap), _p(out)))
+        return out
+
+    def grid_device(self, guests, n_points, out_ptr: int, coulomb=False, precision=FP64, cap=1.0e5,
+                    grid_lattice=None):
+        """Device-output variant: writes to device pointer `out_ptr` on the context stream."""
+        g = np.ascontiguousarray(guests, dtype=np.uint32)
+        n = np.ascontiguousarray(n_points, dtype=np.int32)
+        gl = _lattice(grid_lattice) if grid_lattice is not None else None
+        _check(_load().aleator_gpu_grid_build_device(self._h, C.byref(gl) if gl is not None else None,
+                                                     _p(g, _u32p) if len(g) else None, C.c_int32(len(g)),
+                                                     C.c_int32(1 if coulomb else 0), _p(n, _i32p),
+                                                     C.c_int32(precision), C.c_double(cap), C.c_void_p(out_ptr)))
+
+    @staticmethod
+    def _sites(sites):
+        arr = (_MolSite * len(sites))()
+        for i, s in enumerate(sites):
+            arr[i] = _MolSite(float(s[0]), float(s[1]), float(s[2]), int(s[3]), float(s[4]))
+        return arr
+
+    def widom(self, sites, n_trials, temperature, seed=1, stream=0, first_call_index=0, precision=FP64,
+              return_du=True):
+        """Returns (sum_t exp(-dU_t/T), dU per trial or None); K_H = sum / (n_trials * T)."""
+        arr = self._sites(sites)
+        du = np.zeros(n_trials) if return_du else None
+        s = C.c_double()
+        _check(_load().aleator_gpu_widom(self._h, arr, C.c_int32(len(sites)), C.c_uint64(seed), C.c_uint64(stream),
+                                         C.c_uint64(first_call_index), C.c_uint64(n_trials),
+                                         C.c_double(temperature), C.c_int32(precision),
+                                         _p(du) if du is not None else None, C.byref(s)))
+        return s.value, du
+
+    def widom_device(self, sites, n_trials, temperature, du_ptr: int, sum_ptr: int, seed=1, stream=0,
+                     first_call_index=0, precision=FP64):
+        arr = self._sites(sites)
+        _check(_load().aleator_gpu_widom_device(self._h, arr, C.c_int32(len(sites)), C.c_uint64(seed),
+                                                C.c_uint64(stream), C.c_uint64(first_call_index),
+                                                C.c_uint64(n_trials), C.c_double(temperature), C.c_int32(precision),
+                                                C.c_void_p(du_ptr) if du_ptr else None, C.c_void_p(sum_ptr)))
+
+    def henry_coefficient(self, sites, n_trials, temperature, **kw):
+        """widomInsertionHenryCoefficient: (sum exp(-dU/T) / n_trials) / T."""
+        s, _ = self.widom(sites, n_trials, temperature, return_du=False, **kw)
+        return (s / n_trials) / temperature
+
+
+def _charged_sites(sites):
+    sites = np.asarray(sites, dtype=np.float64).reshape(-1, 4)
+    arr = (_ChargedSite * max(len(sites), 1))()
+    for i, r in enumerate(sites):
+        arr[i] = _ChargedSite(*r)
+    return arr, len(sites)
+
+
+class EwaldState:
+    """forcefield::EwaldIncrementalState on the device, initialised from a charged framework."""
+
+    def __init__(self, fw: Framework):
+        self.fw = fw
+        self._h = C.c_void_p()
+        _check(_load().aleator_gpu_ewald_state_create(fw.handle, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            _load().aleator_gpu_ewald_state_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def propose(self, removed, added) -> float:
+        r, nr = _charged_sites(removed)
+        a, na = _charged_sites(added)
+        out = C.c_double()
+        _check(_load().aleator_gpu_ewald_state_propose(self._h, r, C.c_int32(nr), a, C.c_int32(na), C.byref(out)))
+        return out.value
+
+    def propose_insertion(self, sites):
+        return self.propose([], sites)
+
+    def propose_deletion(self, sites):
+        return self.propose(sites, [])
+
+    def commit(self, removed, added):
+        r, nr = _charged_sites(removed)
+        a, na = _charged_sites(added)
+        _check(_load().aleator_gpu_ewald_state_commit(self._h, r, C.c_int32(nr), a, C.c_int32(na)))
+
+    def propose_batch(self, removed, added):
+        """removed: [P][nr][4], added: [P][na][4] -> dE[P]."""
+        added = np.asarray(added, dtype=np.float64)
+        removed = np.asarray(removed, dtype=np.float64)
+        p = added.shape[0] if added.size else removed.shape[0]
+        na = added.shape[1] if added.size else 0
+        nr = removed.shape[1] if removed.size else 0
+        a, _ = _charged_sites(added.reshape(-1, 4) if added.size else [])
+        r, _ = _charged_sites(removed.reshape(-1, 4) if removed.size else [])
+        out = np.zeros(p)
+        _check(_load().aleator_gpu_ewald_state_propose_batch(self._h, C.c_uint64(p), r, C.c_int32(nr), a,
+                                                             C.c_int32(na), _p(out)))
+        return out
+
+    def committed_energy(self) -> float:
+        out = C.c_double()
+        _check(_load().aleator_gpu_ewald_state_committed_energy(self._h, C.byref(out)))
+        return out.value
+
+    @staticmethod
+    def _particles(x, y, z, q):
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, q)]
+        return _Particles(_p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), None, len(keep[0])), keep
+
+    @staticmethod
+    def _molecules(molecules):
+        off = np.zeros(len(molecules) + 1, dtype=np.uint64)
+        flat = []
+        for i, m in enumerate(molecules):
+            flat.extend(m)
+            off[i + 1] = len(flat)
+        return off, np.ascontiguousarray(flat if flat else [0], dtype=np.uint64)
+
+    def resync(self, x, y, z, q, molecules):
+        parts, keep = self._particles(x, y, z, q)
+        off, flat = self._molecules(molecules)
+        _check(_load().aleator_gpu_ewald_state_resync(self._h, C.byref(parts), _p(off, _u64p), _p(flat, _u64p),
+                                                      C.c_uint64(len(molecules))))
+
+    def recompute_from_scratch(self, x, y, z, q, molecules) -> float:
+        parts, keep = self._particles(x, y, z, q)
+        off, flat = self._molecules(molecules)
+        out = C.c_double()
+        _check(_load().aleator_gpu_ewald_state_recompute(self._h, C.byref(parts), _p(off, _u64p), _p(flat, _u64p),
+                                                         C.c_uint64(len(molecules)), C.byref(out)))
+        return out.value
+
+    def raw(self) -> np.ndarray:
+        n = self.fw.num_k
+        out = np.zeros(2 * n + 2)
+        w = C.c_uint64()
+        _check(_load().aleator_gpu_ewald_state_raw(self._h, _p(out), C.c_uint64(len(out)), C.byref(w)))
+        return out

@@ -1,0 +1,159 @@
+// internal.hpp -- shared host/device declarations of the gfx950 hot-path library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "aleator_gpu.h"
+
+namespace aleator_gpu {
+
+// ---------------------------------------------------------------- errors
+// Exceptions inside the library are mapped to aleator_status at the C boundary.
+struct Error : std::runtime_error {
+    aleator_status status;
+    Error(aleator_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+};
+
+[[noreturn]] inline void fail(aleator_status s, const std::string& msg) { throw Error(s, msg); }
+
+#define ALEATOR_HIP_CHECK(expr)                                                                 \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            ::aleator_gpu::fail(ALEATOR_DEVICE_ERROR,                                           \
+                                std::string(#expr) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+// ---------------------------------------------------------------- limits
+constexpr int kMaxTypes = 32;       // distinct framework LJ species
+constexpr int kMaxGuests = 4;       // LJ tables per grid launch
+constexpr int kMaxSites = 8;        // sites of a Widom / proposal molecule
+constexpr int kMaxK = 16;           // kmax bound for the per-axis phase tables
+constexpr double kCoulomb = 167100.9468982874;  // ewald.hpp:62
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- host lattice
+// Row-vector lattice (core/geometry/lattice.hpp:21-30) with the reciprocal-vector
+// fractional map of lattice.cc:54-60 precomputed: f = c . inv (inv columns are
+// (a_j x a_k)/V).
+struct HostLattice {
+    double m[3][3];
+    double inv[3][3];
+    double volume;   // signed
+    double lperp[3]; // perpendicular widths (lattice.cc:257-284)
+};
+HostLattice make_lattice(const double m[3][3]);
+
+// Reference k-set (ewald.cc:137-161) in the reference's order.
+struct KSet {
+    std::vector<int32_t> n;     // [nk][3]
+    std::vector<double> k;      // [nk][3]
+    std::vector<double> weight; // [nk]
+    size_t size() const { return weight.size(); }
+};
+KSet make_kset(const HostLattice& lat, double alpha, int kmax);
+
+// ---------------------------------------------------------------- objects
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // reusable device scratch
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    void* ensure_scratch(size_t bytes);
+    // optional HIP-event timing of each call's dominant kernel, on `stream`
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    void time_begin();
+    void time_end();
+};
+
+struct Framework {
+    Context* ctx = nullptr;
+    HostLattice lat{};
+    uint64_t n = 0;
+    std::vector<double> x, y, z, q; // cartesian input
+    std::vector<double> fx, fy, fz; // fractional, wrapped into [0,1)
+    std::vector<uint32_t> species;
+    std::vector<int32_t> type;          // compact type index per atom
+    std::vector<uint32_t> type_species; // species id of each compact type
+    // LJ
+    std::vector<double> eps, sigma;
+    double rc = 0.0;
+    int trunc = 0, mixing = 0;
+    // device atoms
+    double *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr, *d_q = nullptr;
+    int32_t* d_type = nullptr;
+    // Ewald
+    bool charged = false;
+    double alpha = 0.0, rc_real = 0.0;
+    int kmax = 0;
+    KSet kset;
+    int nk_half = 0;                 // first nk/2 reference terms (lexicographically n < 0)
+    int32_t* d_kn = nullptr;         // [nk_half][4] (n0,n1,n2,pad)
+    double* d_kw = nullptr;          // [nk_half]
+    double *d_sre = nullptr, *d_sim = nullptr;  // framework S(k), half set
+    std::vector<double> h_sre, h_sim;           // host copy of the same
+    double self_energy = 0.0;
+};
+
+struct EwaldState {
+    Framework* fw = nullptr;
+    double *d_sre = nullptr, *d_sim = nullptr; // committed S(k), half set
+    double self_energy = 0.0, exclusion_energy = 0.0;
+    double* d_out = nullptr;                   // propose result scratch
+};
+
+// ---------------------------------------------------------------- device helpers
+struct LjPairTable {
+    // per (slot, type): sigma^2, 4 eps, V(rc), V'(rc)
+    double sig2[kMaxGuests > kMaxSites ? kMaxGuests : kMaxSites][kMaxTypes];
+    double eps4[kMaxGuests > kMaxSites ? kMaxGuests : kMaxSites][kMaxTypes];
+    double vrc[kMaxGuests > kMaxSites ? kMaxGuests : kMaxSites][kMaxTypes];
+    double dvrc[kMaxGuests > kMaxSites ? kMaxGuests : kMaxSites][kMaxTypes];
+};
+
+// mixed parameters and cutoff constants of one (guest species, framework type) pair
+// (lennard_jones.cc:15-30, 49-68)
+void lj_pair_constants(const Framework& fw, uint32_t guest, uint32_t fw_species, double& sig2,
+                       double& eps4, double& vrc, double& dvrc);
+
+// ---------------------------------------------------------------- launchers
+struct GridRequest {
+    const double* grid_m;  // 3x3 row-major or nullptr
+    std::vector<uint32_t> guests;
+    bool coulomb = false;
+    int32_t n[3];
+    int precision = 0;
+    double cap = 1e5;
+    double* out_device = nullptr;
+};
+void launch_grid(Framework& fw, const GridRequest& req);
+
+struct WidomRequest {
+    std::vector<aleator_molecule_site> sites;
+    uint64_t seed, stream, first_call, n_trials;
+    double temperature;
+    int precision;
+    double* du_device;  // may be null
+    double* sum_device;
+};
+void launch_widom(Framework& fw, const WidomRequest& req);
+
+// structure factor of arbitrary fractional positions over the framework half k-set
+void structure_factor(Framework& fw, uint64_t n, const double* d_fx, const double* d_fy,
+                      const double* d_fz, const double* d_q, double* d_sre, double* d_sim);
+
+// sum_k w (|S+dS|^2 - |S|^2) over the half set for `n_prop` proposals, each removing
+// n_rem and adding n_add sites (fractional coordinates + charge, device buffer
+// [n_prop][n_rem+n_add][4]); result (times 2 for the mirror half) in d_out[n_prop].
+// When `commit` is set (n_prop == 1), dS is added to S.
+void reciprocal_delta(Framework& fw, double* d_sre, double* d_sim, uint64_t n_prop, int n_rem,
+                      int n_add, const double* d_sites, double* d_out, bool commit);
+
+} // namespace aleator_gpu

@@ -1,0 +1,353 @@
+// widom.hip -- batched Widom test-particle insertion on gfx950.
+//
+// Reference: MonteCarloEngine::widomInsertionHenryCoefficient
+// (src/engines/monte_carlo/monte_carlo_engine.cc:850-917).  Trial t draws, from
+// Philox4x32Rng(seed, stream) at call index first + t*d, three uniforms for the
+// reference point a0*u0 + a1*u1 + a2*u2 (:882-884) and, for a multi-site molecule,
+// three more for a Shoemake quaternion (:56-63, :886-889) that rotates the sites
+// (:68-72).  dU = sum over sites of the LJ energy against every framework atom
+// (LennardJones::computeParticleEnergy, lennard_jones.cc:158-182) plus, when the
+// framework is charged, the real-space Ewald energy (ewald.cc:223-249) and
+// EwaldIncrementalState::proposeInsertion (reciprocal + self + exclusion,
+// ewald_incremental_state.cc:95-136).  K_H = <exp(-dU/T)>/T.
+//
+// Kernel 1 (one thread per trial): the framework is staged through LDS in chunks of
+// 256 atoms as fractional coordinates; every lane reads the same atom (broadcast),
+// folds the fractional difference to the nearest image (exact for pairs within a
+// cutoff <= L_perp/2) and accumulates LJ and erfc terms for all sites.
+// Kernel 2 (charged only; one wave per trial): per-axis phase factors of the sites in
+// LDS, then the half k-set sum of recip.hpp.  Both write per-workgroup partial sums of
+// exp(-dU/T), reduced in a fixed order by kernel 3.
+#include <cmath>
+#include <vector>
+
+#include "device.hpp"
+#include "internal.hpp"
+#include "recip.hpp"
+
+namespace aleator_gpu {
+
+namespace {
+
+constexpr int kWThreads = 256;
+
+struct WidomArgs {
+    uint64_t seed, stream, first, n_trials;
+    int draws;  // 3 or 6 per trial
+    int nsites;
+    double local[kMaxSites][3];
+    double charge[kMaxSites];
+    double m[9];    // lattice rows
+    double inv[9];  // Cartesian -> fractional
+    const double* fx;
+    const double* fy;
+    const double* fz;
+    const double* q;
+    const int32_t* type;
+    int natoms;
+    double rc2_lj, rc_lj, rc2_real, alpha, kc;
+    int trunc;
+    int charged;
+    const double* ljtab;  // [4][kMaxSites][kMaxTypes]
+    double temperature;
+    double self_delta;  // -kC alpha/sqrt(pi) sum q^2 of the molecule
+    // charged reciprocal part
+    const int4* kn;
+    const double* kw;
+    const double* sre;
+    const double* sim;
+    int nk_half, kmax;
+    double recip_scale;
+    double* du;           // [n_trials] (LJ + real in kernel 1 when charged)
+    double* block_sums;   // per workgroup sum of exp(-dU/T)
+};
+
+// Reference point and rotated sites of trial t, Cartesian, in the reference's
+// arithmetic order (no contraction), and their fractional coordinates.
+__device__ inline void trial_sites(const WidomArgs& a, uint64_t t, double cart[kMaxSites][3]) {
+#pragma clang fp contract(off)
+    const uint64_t call = a.first + t * (uint64_t)a.draws;
+    const double u0 = philox_uniform(a.seed, a.stream, call);
+    const double u1 = philox_uniform(a.seed, a.stream, call + 1);
+    const double u2 = philox_uniform(a.seed, a.stream, call + 2);
+    double ref[3];
+    for (int j = 0; j < 3; ++j) ref[j] = a.m[j] * u0 + a.m[3 + j] * u1 + a.m[6 + j] * u2;
+    if (a.nsites == 1) {
+        for (int j = 0; j < 3; ++j) cart[0][j] = ref[j] + a.local[0][j];
+        return;
+    }
+    const double v0 = philox_uniform(a.seed, a.stream, call + 3);
+    const double v1 = philox_uniform(a.seed, a.stream, call + 4);
+    const double v2 = philox_uniform(a.seed, a.stream, call + 5);
+    const double r1 = sqrt(1.0 - v0), r2 = sqrt(v0);
+    const double th1 = 2.0 * kPi * v1, th2 = 2.0 * kPi * v2;
+    const double qw = r2 * cos(th2), qx = r1 * sin(th1), qy = r1 * cos(th1), qz = r2 * sin(th2);
+    for (int s = 0; s < a.nsites; ++s) {
+        const double vx = a.local[s][0], vy = a.local[s][1], vz = a.local[s][2];
+        // t = 2 (qv x v); v' = v + w t + qv x t
+        const double tx = (qy * vz - qz * vy) * 2.0;
+        const double ty = (qz * vx - qx * vz) * 2.0;
+        const double tz = (qx * vy - qy * vx) * 2.0;
+        const double rx = (vx + tx * qw) + (qy * tz - qz * ty);
+        const double ry = (vy + ty * qw) + (qz * tx - qx * tz);
+        const double rz = (vz + tz * qw) + (qx * ty - qy * tx);
+        cart[s][0] = ref[0] + rx;
+        cart[s][1] = ref[1] + ry;
+        cart[s][2] = ref[2] + rz;
+    }
+}
+
+__device__ inline void to_frac(const WidomArgs& a, const double c[3], double f[3]) {
+    for (int j = 0; j < 3; ++j) f[j] = c[0] * a.inv[j] + c[1] * a.inv[3 + j] + c[2] * a.inv[6 + j];
+}
+
+template <typename R, int NS, bool CHARGED>
+__global__ __launch_bounds__(kWThreads) void widom_pair_kernel(const WidomArgs a) {
+    __shared__ double sfx[kWThreads], sfy[kWThreads], sfz[kWThreads], sq[kWThreads];
+    __shared__ int st[kWThreads];
+    __shared__ double tab[4][NS][kMaxTypes];
+    __shared__ double red[kWThreads / 64];
+    for (int i = threadIdx.x; i < 4 * NS * kMaxTypes; i += kWThreads) {
+        const int k = i / (NS * kMaxTypes), rem = i % (NS * kMaxTypes);
+        (&tab[0][0][0])[i] = a.ljtab[(k * kMaxSites + rem / kMaxTypes) * kMaxTypes + rem % kMaxTypes];
+    }
+    const uint64_t t = (uint64_t)blockIdx.x * kWThreads + threadIdx.x;
+    const bool valid = t < a.n_trials;
+    double cart[kMaxSites][3];
+    double fs[NS][3];
+    if (valid) {
+        trial_sites(a, t, cart);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) to_frac(a, cart[s], fs[s]);
+    }
+    double lj = 0.0, real = 0.0;
+    const R rc2 = (R)a.rc2_lj, rcr2 = (R)a.rc2_real, alpha = (R)a.alpha;
+    for (int base = 0; base < a.natoms; base += kWThreads) {
+        __syncthreads();
+        const int at = base + threadIdx.x;
+        if (at < a.natoms) {
+            sfx[threadIdx.x] = a.fx[at];
+            sfy[threadIdx.x] = a.fy[at];
+            sfz[threadIdx.x] = a.fz[at];
+            sq[threadIdx.x] = a.q[at];
+            st[threadIdx.x] = a.type[at];
+        }
+        __syncthreads();
+        const int cnt = min(kWThreads, a.natoms - base);
+        if (!valid) continue;
+        for (int j = 0; j < cnt; ++j) {
+            const double ax = sfx[j], ay = sfy[j], az = sfz[j];
+            const int ty = st[j];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                double d0 = ax - fs[s][0], d1 = ay - fs[s][1], d2 = az - fs[s][2];
+                d0 -= rint(d0);
+                d1 -= rint(d1);
+                d2 -= rint(d2);
+                const R dx = (R)(d0 * a.m[0] + d1 * a.m[3] + d2 * a.m[6]);
+                const R dy = (R)(d0 * a.m[1] + d1 * a.m[4] + d2 * a.m[7]);
+                const R dz = (R)(d0 * a.m[2] + d1 * a.m[5] + d2 * a.m[8]);
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= rc2) {
+                    const R ir2 = (R)1 / r2;
+                    const R s2 = (R)tab[0][s][ty] * ir2;
+                    const R s6 = s2 * s2 * s2;
+                    R v = (R)tab[1][s][ty] * (s6 * s6 - s6);
+                    if (a.trunc == ALEATOR_LJ_SHIFTED) {
+                        v -= (R)tab[2][s][ty];
+                    } else if (a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED) {
+                        v = v - (R)tab[2][s][ty] - (sqrt(r2) - (R)a.rc_lj) * (R)tab[3][s][ty];
+                    }
+                    lj += (double)v;
+                }
+                if (CHARGED && r2 <= rcr2) {
+                    const R r = sqrt(r2);
+                    real += (double)((R)(a.charge[s] * sq[j]) * (R)erfc(alpha * r) / r);
+                }
+            }
+        }
+    }
+    double du = lj;
+    if (CHARGED) du += a.kc * real;
+    double boltz = 0.0;
+    if (valid) {
+        a.du[t] = du;
+        if (!CHARGED) boltz = exp(-du / a.temperature);
+    }
+    if (!CHARGED) {
+        const double s = block_sum(boltz, red);
+        if (threadIdx.x == 0) a.block_sums[blockIdx.x] = s;
+    }
+}
+
+// One wave per trial: reciprocal + self + exclusion, completing dU.
+__global__ __launch_bounds__(256) void widom_recip_kernel(const WidomArgs a) {
+    extern __shared__ cplx wr_phase[];
+    __shared__ double site4[4][kMaxSites * 4];
+    __shared__ double red[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ns = a.nsites;
+    const int nkk = 2 * a.kmax + 1;
+    cplx* E = wr_phase + (size_t)wave * 3 * nkk * ns;
+    const uint64_t t = (uint64_t)blockIdx.x * 4 + wave;
+    const bool valid = t < a.n_trials;
+    double cart[kMaxSites][3];
+    double excl = 0.0;
+    if (valid) {
+        trial_sites(a, t, cart);
+        if (lane == 0) {
+            for (int s = 0; s < ns; ++s) {
+                double f[3];
+                to_frac(a, cart[s], f);
+                site4[wave][4 * s] = f[0];
+                site4[wave][4 * s + 1] = f[1];
+                site4[wave][4 * s + 2] = f[2];
+                site4[wave][4 * s + 3] = a.charge[s];
+            }
+        }
+        // intramolecular exclusion: -kC sum_{i<j} q_i q_j erf(alpha r)/r
+        // (ewald_incremental_state.cc:32-43); rigid-molecule distances are far below L_perp/2
+        double sum = 0.0;
+        for (int i = 0; i < ns; ++i)
+            for (int j = i + 1; j < ns; ++j) {
+                const double dx = cart[j][0] - cart[i][0], dy = cart[j][1] - cart[i][1],
+                             dz = cart[j][2] - cart[i][2];
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                const double eor = r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r;
+                sum += a.charge[i] * a.charge[j] * eor;
+            }
+        excl = -a.kc * sum;
+    }
+    __syncthreads();
+    if (valid) stage_phases(lane, ns, a.kmax, site4[wave], E);
+    __syncthreads();
+    double boltz = 0.0;
+    if (valid) {
+        const double acc = wave_recip_delta(lane, ns, 0, a.kmax, E, site4[wave], a.kn, a.kw,
+                                            const_cast<double*>(a.sre), const_cast<double*>(a.sim),
+                                            a.nk_half, false);
+        const double du = a.du[t] + a.recip_scale * acc + a.self_delta + excl;
+        if (lane == 0) a.du[t] = du;
+        boltz = exp(-du / a.temperature);
+    }
+    if (lane == 0) red[wave] = valid ? boltz : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) a.block_sums[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(1024) void sum_kernel(const double* parts, uint64_t n, double* out) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (uint64_t i = threadIdx.x; i < n; i += 1024) s += parts[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+template <typename R, bool CH>
+void launch_pair(const WidomArgs& a, unsigned blocks, hipStream_t s) {
+    switch (a.nsites) {
+#define CASE(N) \
+    case N: hipLaunchKernelGGL((widom_pair_kernel<R, N, CH>), dim3(blocks), dim3(kWThreads), 0, s, a); return;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
+#undef CASE
+    }
+    fail(ALEATOR_INVALID_ARGUMENT, "widom: 1 to 5 sites supported");
+}
+
+}  // namespace
+
+void launch_widom(Framework& fw, const WidomRequest& req) {
+    Context& ctx = *fw.ctx;
+    const int ns = (int)req.sites.size();
+    if (ns < 1 || ns > 5) fail(ALEATOR_INVALID_ARGUMENT, "widom: 1 to 5 sites supported");
+    if (!(req.temperature > 0.0)) fail(ALEATOR_INVALID_ARGUMENT, "widom: temperature must be positive");
+    if (req.n_trials == 0) fail(ALEATOR_INVALID_ARGUMENT, "widom: n_trials must be positive");
+    WidomArgs a{};
+    a.seed = req.seed;
+    a.stream = req.stream;
+    a.first = req.first_call;
+    a.n_trials = req.n_trials;
+    a.draws = ns > 1 ? 6 : 3;
+    a.nsites = ns;
+    bool any_charge = false;
+    double q2 = 0.0;
+    for (int s = 0; s < ns; ++s) {
+        a.local[s][0] = req.sites[s].x;
+        a.local[s][1] = req.sites[s].y;
+        a.local[s][2] = req.sites[s].z;
+        a.charge[s] = req.sites[s].charge;
+        any_charge |= req.sites[s].charge != 0.0;
+        q2 += req.sites[s].charge * req.sites[s].charge;
+        if (req.sites[s].lj_species >= fw.eps.size())
+            fail(ALEATOR_OUT_OF_RANGE, "widom: site LJ species out of range");
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            a.m[3 * i + j] = fw.lat.m[i][j];
+            a.inv[3 * i + j] = fw.lat.inv[i][j];
+        }
+    a.fx = fw.d_fx;
+    a.fy = fw.d_fy;
+    a.fz = fw.d_fz;
+    a.q = fw.d_q;
+    a.type = fw.d_type;
+    a.natoms = (int)fw.n;
+    a.rc2_lj = fw.rc * fw.rc;
+    a.rc_lj = fw.rc;
+    a.rc2_real = fw.rc_real * fw.rc_real;
+    a.alpha = fw.alpha;
+    a.kc = kCoulomb;
+    a.trunc = fw.trunc;
+    const bool charged = fw.charged && any_charge;
+    a.charged = charged;
+    a.temperature = req.temperature;
+    a.self_delta = -kCoulomb * (fw.alpha / std::sqrt(kPi)) * q2;
+    a.kn = reinterpret_cast<const int4*>(fw.d_kn);
+    a.kw = fw.d_kw;
+    a.sre = fw.d_sre;
+    a.sim = fw.d_sim;
+    a.nk_half = fw.nk_half;
+    a.kmax = fw.kmax;
+    a.recip_scale = 2.0 * kCoulomb * (2.0 * kPi / std::abs(fw.lat.volume));
+
+    std::vector<double> tab(4 * kMaxSites * kMaxTypes, 0.0);
+    for (int s = 0; s < ns; ++s)
+        for (size_t t = 0; t < fw.type_species.size(); ++t) {
+            double s2, e4, v, dv;
+            lj_pair_constants(fw, req.sites[s].lj_species, fw.type_species[t], s2, e4, v, dv);
+            tab[(0 * kMaxSites + s) * kMaxTypes + t] = s2;
+            tab[(1 * kMaxSites + s) * kMaxTypes + t] = e4;
+            tab[(2 * kMaxSites + s) * kMaxTypes + t] = v;
+            tab[(3 * kMaxSites + s) * kMaxTypes + t] = dv;
+        }
+    const unsigned pair_blocks = (unsigned)((req.n_trials + kWThreads - 1) / kWThreads);
+    const unsigned rec_blocks = (unsigned)((req.n_trials + 3) / 4);
+    const size_t tab_bytes = (tab.size() * sizeof(double) + 255) & ~size_t(255);
+    const size_t parts = charged ? rec_blocks : pair_blocks;
+    size_t du_bytes = req.du_device ? 0 : ((req.n_trials * sizeof(double) + 255) & ~size_t(255));
+    char* base = static_cast<char*>(ctx.ensure_scratch(tab_bytes + parts * sizeof(double) + du_bytes));
+    ALEATOR_HIP_CHECK(hipMemcpyAsync(base, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
+                                     ctx.stream));
+    a.ljtab = reinterpret_cast<const double*>(base);
+    a.block_sums = reinterpret_cast<double*>(base + tab_bytes);
+    a.du = req.du_device ? req.du_device
+                         : reinterpret_cast<double*>(base + tab_bytes + parts * sizeof(double));
+    const bool fp64 = req.precision == ALEATOR_PRECISION_FP64;
+    if (!fp64 && req.precision != ALEATOR_PRECISION_MIXED) fail(ALEATOR_INVALID_ARGUMENT, "widom: unknown precision");
+    ctx.time_begin();
+    if (charged) {
+        if (fp64) launch_pair<double, true>(a, pair_blocks, ctx.stream);
+        else launch_pair<float, true>(a, pair_blocks, ctx.stream);
+        const size_t lds = (size_t)4 * 3 * (2 * fw.kmax + 1) * ns * sizeof(cplx);
+        hipLaunchKernelGGL(widom_recip_kernel, dim3(rec_blocks), dim3(256), lds, ctx.stream, a);
+    } else {
+        if (fp64) launch_pair<double, false>(a, pair_blocks, ctx.stream);
+        else launch_pair<float, false>(a, pair_blocks, ctx.stream);
+    }
+    ctx.time_end();
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, ctx.stream, a.block_sums, (uint64_t)parts,
+                       req.sum_device);
+    ALEATOR_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace aleator_gpu

@@ -1,0 +1,238 @@
+/*
+ * aleator_gpu.h -- C ABI of the MI355X (gfx950) adsorption hot path.
+ *
+ * The three reference operations this library replaces, each behind plain
+ * pointers and sizes so that any host (C++, Python ctypes, ...) can bind it:
+ *
+ *   framework energy grid   FrameworkEnergyGrid::FrameworkEnergyGrid
+ *                           (src/engines/monte_carlo/framework_energy_grid.cc:193-249),
+ *                           extended with a Coulomb (Ewald real + reciprocal) table;
+ *   Widom insertion         MonteCarloEngine::widomInsertionHenryCoefficient
+ *                           (src/engines/monte_carlo/monte_carlo_engine.cc:850-917);
+ *   S(k) incremental state  forcefield::EwaldIncrementalState
+ *                           (src/forcefield/electrostatics/ewald_incremental_state.hpp:70-171).
+ *
+ * Conventions follow the reference exactly: lattice rows are the lattice
+ * vectors in Angstrom (core/geometry/lattice.hpp:25-26), energies are in K,
+ * charges in e, grid node (ix,iy,iz) sits at fractional (ix/nx, iy/ny, iz/nz)
+ * and tables are flattened (iz*ny + iy)*nx + ix
+ * (framework_energy_grid.hpp:185-194).
+ *
+ * Errors: every entry point returns an aleator_status; the message of the last
+ * failure on the calling thread is available from aleator_gpu_last_error().
+ * The status codes map one-to-one onto the reference's exception types
+ * (std::invalid_argument, std::out_of_range, std::runtime_error,
+ * aleator::NotImplemented; see src/core/exceptions.hpp:13-17).
+ *
+ * Threading: a context, and every object created from it, is used by one host
+ * thread at a time (the reference is single-threaded by contract,
+ * src/io/config.cc:131-135).
+ */
+#ifndef ALEATOR_GPU_H
+#define ALEATOR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ALEATOR_OK = 0,
+    ALEATOR_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+    ALEATOR_OUT_OF_RANGE = 2,     /* std::out_of_range */
+    ALEATOR_RUNTIME_ERROR = 3,    /* std::runtime_error */
+    ALEATOR_NOT_IMPLEMENTED = 4,  /* aleator::NotImplemented */
+    ALEATOR_DEVICE_ERROR = 5      /* HIP runtime failure (no reference counterpart) */
+} aleator_status;
+
+/* LennardJonesTruncation (pairwise/lennard_jones.hpp:21-38) */
+typedef enum { ALEATOR_LJ_TRUNCATED = 0, ALEATOR_LJ_SHIFTED = 1, ALEATOR_LJ_LINEAR_FORCE_SHIFTED = 2 } aleator_lj_truncation;
+/* MixingRule (pairwise/lennard_jones.hpp:42-50) */
+typedef enum { ALEATOR_MIX_LORENTZ_BERTHELOT = 0, ALEATOR_MIX_GEOMETRIC = 1 } aleator_mixing_rule;
+/* Arithmetic of the pair terms.  FP64: every term in double.  MIXED: geometry and
+ * pair terms in FP32, accumulation in FP64, reciprocal space always FP64. */
+typedef enum { ALEATOR_PRECISION_FP64 = 0, ALEATOR_PRECISION_MIXED = 1 } aleator_precision;
+
+typedef struct aleator_ctx aleator_ctx;
+typedef struct aleator_framework aleator_framework;
+typedef struct aleator_ewald_state aleator_ewald_state;
+
+/* core::Lattice matrix: m[i] = lattice vector a_i (lattice.hpp:21-30). */
+typedef struct {
+    double m[3][3];
+} aleator_lattice;
+
+/* core::ParticleData subset the hot path reads (particle_data.hpp:20-99): host pointers. */
+typedef struct {
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* charge;    /* may be NULL: all zero */
+    const uint32_t* species; /* index into aleator_lj_params */
+    uint64_t n;
+} aleator_particles;
+
+/* LennardJones constructor arguments (lennard_jones.hpp:64-66). */
+typedef struct {
+    const double* epsilon; /* K, per species */
+    const double* sigma;   /* Angstrom, per species */
+    uint32_t n_species;
+    double cutoff;
+    int32_t truncation; /* aleator_lj_truncation */
+    int32_t mixing;     /* aleator_mixing_rule */
+} aleator_lj_params;
+
+/* Ewald constructor arguments (ewald.hpp:76-77); k-set is the reference's
+ * spherical rule n0^2+n1^2+n2^2 < kmax^2+2, n != 0 (ewald.cc:137-161). */
+typedef struct {
+    double alpha;
+    double real_space_cutoff;
+    int32_t kmax;
+} aleator_ewald_params;
+
+/* forcefield::ChargedSite (ewald_incremental_state.hpp:17-22). */
+typedef struct {
+    double x, y, z;
+    double charge;
+} aleator_charged_site;
+
+/* engines::MoleculeSite (molecule_species.hpp:14-21), mass omitted (unused on the path). */
+typedef struct {
+    double x, y, z;
+    uint32_t lj_species;
+    double charge;
+} aleator_molecule_site;
+
+/* ---- context --------------------------------------------------------- */
+const char* aleator_gpu_last_error(void);
+/* No reference counterpart: binds the calling thread to `device` and holds its stream. */
+aleator_status aleator_gpu_init(int32_t device, aleator_ctx** out);
+aleator_status aleator_gpu_destroy(aleator_ctx* ctx);
+/* The HIP stream (hipStream_t) every launch of this context goes to. */
+void* aleator_gpu_stream(aleator_ctx* ctx);
+/* No reference counterpart (profiling): synchronises the stream, returns the summed
+ * HIP-event time (ms) and count of the dominant-kernel launches recorded since the
+ * previous call (the grid tile kernel of each grid build, the pair kernels of each
+ * Widom batch), clears them, and enables (non-zero) or disables further recording. */
+aleator_status aleator_gpu_timing(aleator_ctx* ctx, int32_t enable, double* total_ms, uint64_t* launches);
+
+/* ---- device-resident framework --------------------------------------- */
+/* Uploads the rigid framework (atoms periodic in `lattice`), precomputes the per
+ * (guest, framework type) mixed LJ table (lennard_jones.cc:49-68) and, when `ewald`
+ * is non-NULL, the reference k-set and the framework structure factor S(k)
+ * (EwaldIncrementalState ctor, ewald_incremental_state.cc:55-83).  Validates what
+ * the reference validates: positive LJ cutoff (lennard_jones.cc:39-47), Ewald
+ * parameters (ewald.cc:199-212), charge neutrality when charged (ewald.cc:181-195),
+ * cutoffs <= L_perp/2 (lattice.cc:286-300). */
+aleator_status aleator_gpu_framework_create(aleator_ctx* ctx, const aleator_lattice* lattice,
+                                            const aleator_particles* atoms,
+                                            const aleator_lj_params* lj,
+                                            const aleator_ewald_params* ewald,
+                                            aleator_framework** out);
+aleator_status aleator_gpu_framework_destroy(aleator_framework* fw);
+/* Number of reciprocal terms of the framework's k-set (0 when uncharged). */
+aleator_status aleator_gpu_framework_num_k(const aleator_framework* fw, uint64_t* nk);
+
+/* ---- framework energy grid ------------------------------------------- */
+/* Grid point counts n_axis = max(1, lround(L_perp(axis)/spacing))
+ * (framework_energy_grid.cc:207-211). */
+aleator_status aleator_gpu_grid_point_counts(const aleator_lattice* grid_lattice, double spacing,
+                                             int32_t n_points[3]);
+/* Builds one table per entry of `guest_species` (FrameworkEnergyGrid ctor body,
+ * framework_energy_grid.cc:221-248): E(node) = min(sum_f pairEnergy(r, mix(guest,f)), cap).
+ * When `coulomb` is non-zero and the framework is charged, one more table follows: the
+ * Ewald potential per unit charge (K/e), real space kC*sum q erfc(alpha r)/r (ewald.cc:223-249)
+ * plus reciprocal space, the term of EwaldIncrementalState::computeDelta linear in the
+ * inserted charge (ewald_incremental_state.cc:117-121); it is not capped.
+ * `grid_lattice` NULL means the framework lattice (the reference's behaviour); otherwise its
+ * rows must be the framework lattice rows divided by positive integers (a primitive cell of
+ * a replicated framework), and the nodes are that cell's nodes.
+ * `out_host`: (n_guest + coulomb) * nz*ny*nx doubles, table-major. */
+aleator_status aleator_gpu_grid_build(aleator_framework* fw, const aleator_lattice* grid_lattice,
+                                      const uint32_t* guest_species, int32_t n_guest,
+                                      int32_t coulomb, const int32_t n_points[3],
+                                      int32_t precision, double cap, double* out_host);
+/* Same, writing to device memory on the context stream; returns without synchronising. */
+aleator_status aleator_gpu_grid_build_device(aleator_framework* fw,
+                                             const aleator_lattice* grid_lattice,
+                                             const uint32_t* guest_species, int32_t n_guest,
+                                             int32_t coulomb, const int32_t n_points[3],
+                                             int32_t precision, double cap, double* out_device);
+
+/* ---- Widom insertion -------------------------------------------------- */
+/* Batched Widom test-particle insertion of a rigid molecule, as the reference loop
+ * (monte_carlo_engine.cc:880-916): trial t takes uniform draws from
+ * Philox4x32Rng(seed, stream) at call indices first_call_index + t*d + j, with d = 3
+ * for one site and d = 6 otherwise (3 for the position, 3 for the Shoemake quaternion),
+ * in that order.  dU = LJ + [real-space + reciprocal + self + exclusion when charged].
+ * Writes per-trial dU (nullable) and sum_t exp(-dU/T); K_H = sum/(n_trials*T). */
+aleator_status aleator_gpu_widom(aleator_framework* fw, const aleator_molecule_site* sites,
+                                 int32_t n_sites, uint64_t seed, uint64_t stream,
+                                 uint64_t first_call_index, uint64_t n_trials, double temperature,
+                                 int32_t precision, double* du_host, double* sum_boltzmann);
+/* Device variant: du_device (nullable) and sum_device (1 double) in device memory. */
+aleator_status aleator_gpu_widom_device(aleator_framework* fw, const aleator_molecule_site* sites,
+                                        int32_t n_sites, uint64_t seed, uint64_t stream,
+                                        uint64_t first_call_index, uint64_t n_trials,
+                                        double temperature, int32_t precision, double* du_device,
+                                        double* sum_device);
+
+/* ---- EwaldIncrementalState -------------------------------------------- */
+/* Initialised from the framework's atoms (ctor, ewald_incremental_state.cc:55-83). */
+aleator_status aleator_gpu_ewald_state_create(aleator_framework* fw, aleator_ewald_state** out);
+aleator_status aleator_gpu_ewald_state_destroy(aleator_ewald_state* st);
+/* proposeMove / proposeInsertion / proposeDeletion (ewald_incremental_state.cc:147-159):
+ * n_removed == 0 is an insertion, n_added == 0 a deletion.  Never mutates the state. */
+aleator_status aleator_gpu_ewald_state_propose(aleator_ewald_state* st,
+                                               const aleator_charged_site* removed,
+                                               int32_t n_removed,
+                                               const aleator_charged_site* added, int32_t n_added,
+                                               double* delta_energy);
+/* commitMove / commitInsertion / commitDeletion (ewald_incremental_state.cc:161-172). */
+aleator_status aleator_gpu_ewald_state_commit(aleator_ewald_state* st,
+                                              const aleator_charged_site* removed,
+                                              int32_t n_removed, const aleator_charged_site* added,
+                                              int32_t n_added);
+/* Many independent proposals against the same committed state in one launch:
+ * proposal p removes removed[p*n_removed .. ] and adds added[p*n_added .. ].
+ * No reference counterpart as a single call; each element equals propose(). */
+aleator_status aleator_gpu_ewald_state_propose_batch(aleator_ewald_state* st, uint64_t n_proposals,
+                                                     const aleator_charged_site* removed,
+                                                     int32_t n_removed,
+                                                     const aleator_charged_site* added,
+                                                     int32_t n_added, double* delta_energy);
+/* committedEnergy (ewald_incremental_state.cc:85-93). */
+aleator_status aleator_gpu_ewald_state_committed_energy(aleator_ewald_state* st, double* energy);
+/* resync / recomputeFromScratch (ewald_incremental_state.cc:174-247); molecules as CSR
+ * (mol_offsets[n_molecules+1] into mol_sites, indices into `particles`). */
+aleator_status aleator_gpu_ewald_state_resync(aleator_ewald_state* st,
+                                              const aleator_particles* particles,
+                                              const uint64_t* mol_offsets,
+                                              const uint64_t* mol_sites, uint64_t n_molecules);
+aleator_status aleator_gpu_ewald_state_recompute(aleator_ewald_state* st,
+                                                 const aleator_particles* particles,
+                                                 const uint64_t* mol_offsets,
+                                                 const uint64_t* mol_sites, uint64_t n_molecules,
+                                                 double* energy);
+/* rawStateForTesting (ewald_incremental_state.cc:249-257): s_re[Nk], s_im[Nk], self,
+ * exclusion, in the reference's k order; `capacity` doubles available. */
+aleator_status aleator_gpu_ewald_state_raw(aleator_ewald_state* st, double* out, uint64_t capacity,
+                                           uint64_t* written);
+
+/* ---- synthetic frameworks (no reference counterpart) ------------------ */
+/* Uniform fractional placement from Philox4x32Rng(seed, stream), rejecting a candidate
+ * closer than min_dist (minimum image) to an earlier atom; species assigned in index order
+ * with count round(p_t*N) for all but the last type; charges N(0, charge_sigma) by
+ * Box-Muller on Philox4x32Rng(seed, charge_stream) minus their mean (none if sigma <= 0). */
+aleator_status aleator_synth_framework(const aleator_lattice* lattice, uint64_t n_atoms,
+                                       uint64_t seed, uint64_t stream, double min_dist,
+                                       const double* proportions, uint32_t n_types,
+                                       double charge_sigma, uint64_t charge_stream, double* x,
+                                       double* y, double* z, double* charge, uint32_t* species);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

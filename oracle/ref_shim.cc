@@ -5,6 +5,7 @@
 // into oracle/_ref/libaleator_ref.so.  It lets the tests compare the oracle's FP64
 // restatement bit-for-bit with the reference, and lets bench.py time the
 // reference's CPU path as the baseline.  Nothing in the product links it.
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -315,6 +316,68 @@ int ref_widom(uint32_t ns, const double* eps, const double* sig, double rc, int 
         engines::MonteCarloEngine eng(makeParticles(n, x, y, z, q, species), makeLattice(m), lj,
                                       std::move(rng), temperature, sp, 1.0, 1.5, 0.5, ew);
         *kh = eng.widomInsertionHenryCoefficient(n_trials);
+    });
+}
+
+// The FrameworkEnergyGrid constructor's inner loop (framework_energy_grid.cc:227-243) with
+// the reference's own Lattice/LennardJones calls, at explicit nodes of a (possibly primitive)
+// grid lattice `gm` while atoms are periodic in `m`: the CPU baseline of one LJ table.
+int ref_grid_nodes_lj(uint32_t ns, const double* eps, const double* sig, double rc, int trunc,
+                      int mixing, const double* m, uint64_t n, const double* x, const double* y,
+                      const double* z, const uint32_t* species, const double* gm,
+                      const int32_t* counts, uint32_t guest, const int64_t* nodes,
+                      uint64_t n_nodes, double cap, double* out) {
+    return guarded([&] {
+        const auto lj = makeLj(ns, eps, sig, rc, trunc, mixing);
+        const auto lat = makeLattice(m);
+        const auto glat = makeLattice(gm);
+        const auto p = makeParticles(n, x, y, z, nullptr, species);
+        for (uint64_t i = 0; i < n_nodes; ++i) {
+            const int64_t ix = nodes[i] % counts[0];
+            const int64_t iy = (nodes[i] / counts[0]) % counts[1];
+            const int64_t iz = nodes[i] / (static_cast<int64_t>(counts[0]) * counts[1]);
+            const auto cart = glat.fractionalToCartesian(
+                {static_cast<double>(ix) / counts[0], static_cast<double>(iy) / counts[1],
+                 static_cast<double>(iz) / counts[2]});
+            double energy = 0.0;
+            for (std::size_t f = 0; f < p.size(); ++f) {
+                const auto params = lj->mixedParameters(guest, p.species[f]);
+                const auto d = lat.minimumImageDisplacement(cart, {p.x[f], p.y[f], p.z[f]});
+                energy += lj->pairEnergy(std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), params);
+            }
+            out[i] = std::min(energy, cap);
+        }
+    });
+}
+
+// Ewald potential per unit charge at explicit nodes through the reference's own calls:
+// real space = Ewald::realSpaceParticleEnergy of a unit charge appended to the framework
+// (ewald.cc:223-249); reciprocal = the part of EwaldIncrementalState::proposeInsertion linear
+// in the inserted charge, (propose(+1) - propose(-1)) / 2 (ewald_incremental_state.cc:95-136).
+int ref_coulomb_nodes(double alpha, double rc_real, int kmax, const double* m, uint64_t n,
+                      const double* x, const double* y, const double* z, const double* q,
+                      const double* gm, const int32_t* counts, const int64_t* nodes,
+                      uint64_t n_nodes, double* out_real, double* out_recip) {
+    return guarded([&] {
+        const auto lat = makeLattice(m);
+        const auto glat = makeLattice(gm);
+        forcefield::Ewald ew(alpha, rc_real, kmax);
+        auto p = makeParticles(n, x, y, z, q, nullptr);
+        forcefield::EwaldIncrementalState state(ew, p, lat);
+        for (uint64_t i = 0; i < n_nodes; ++i) {
+            const int64_t ix = nodes[i] % counts[0];
+            const int64_t iy = (nodes[i] / counts[0]) % counts[1];
+            const int64_t iz = nodes[i] / (static_cast<int64_t>(counts[0]) * counts[1]);
+            const auto c = glat.fractionalToCartesian(
+                {static_cast<double>(ix) / counts[0], static_cast<double>(iy) / counts[1],
+                 static_cast<double>(iz) / counts[2]});
+            const std::size_t idx = p.push_back(c[0], c[1], c[2], 0, 0, 0, 1.0, 1.0, 0u);
+            out_real[i] = ew.realSpaceParticleEnergy(idx, p, lat);
+            p.popBack();
+            const double plus = state.proposeInsertion({{c[0], c[1], c[2], 1.0}});
+            const double minus = state.proposeInsertion({{c[0], c[1], c[2], -1.0}});
+            out_recip[i] = 0.5 * (plus - minus);
+        }
     });
 }
 

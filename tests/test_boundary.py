@@ -1,0 +1,61 @@
+"""The C-ABI library (CPU-side checks): it loads, exports every function include/aleator_gpu.h
+declares, and its host-only entry points agree with the oracle.  No GPU call is made."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import aleator_amd as aa
+import oracle_lib as ol
+
+HEADER = os.path.join(ol.ROOT, "include", "aleator_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(aleator_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(aa.library_path())
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_grid_point_counts_match_reference_rule():
+    for cell, spacing in (((25.83, 25.83, 25.83, 90, 90, 90), 0.2), ((24, 24, 24, 60, 60, 60), 0.15),
+                          ((10.0, 37.0, 22.0, 70, 110, 95), 0.2), ((3.0, 3.0, 3.0, 90, 90, 90), 10.0)):
+        m = ol.cell_from_params(*cell)
+        assert aa.grid_point_counts(m, spacing) == ol.grid_counts(m, spacing)
+    assert aa.grid_point_counts(ol.cubic(25.83), 0.2) == (129, 129, 129)
+
+
+def test_grid_point_counts_errors():
+    with pytest.raises(ValueError):
+        aa.grid_point_counts(ol.cubic(10.0), 0.0)
+    with pytest.raises(ValueError):
+        aa.grid_point_counts(np.zeros((3, 3)), 0.2)
+
+
+@pytest.mark.parametrize("cell,n,sigma", [((24, 24, 24, 60, 60, 60), 600, 0.5), ((25.83,) * 3 + (90,) * 3, 424, 0.0),
+                                          ((12.0, 30.0, 18.0, 65, 115, 80), 900, 0.5)])
+def test_synthetic_framework_equals_oracle_generator(cell, n, sigma):
+    m = ol.cell_from_params(*cell)
+    got = aa.synth_framework(m, n, seed=1, stream=0, charge_sigma=sigma)
+    ref = ol.synth_framework(m, n, seed=1, stream=0, charge_sigma=sigma)
+    for a, b in zip(got, (ref.x, ref.y, ref.z, ref.q, ref.species)):
+        assert np.array_equal(a, b)
+
+
+def test_device_calls_without_a_context_are_rejected():
+    lib = C.CDLL(aa.library_path())
+    out = C.c_void_p()
+    assert lib.aleator_gpu_framework_create(None, None, None, None, None, C.byref(out)) == 1
+    lib.aleator_gpu_last_error.restype = C.c_char_p
+    assert b"null" in lib.aleator_gpu_last_error()
+    assert lib.aleator_gpu_ewald_state_propose(None, None, 0, None, 0, None) == 1

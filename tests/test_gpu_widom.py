@@ -1,0 +1,79 @@
+"""Parity of batched Widom insertion on the GPU with the FP64 oracle, trial by trial
+(MonteCarloEngine::widomInsertionHenryCoefficient, monte_carlo_engine.cc:850-917).
+
+The per-trial dU comparison pins the device Philox draw indices (a wrong index moves the
+trial and changes dU by orders of magnitude), the rotation and every energy term.
+Tolerance: FP64 1e-10 relative (|dU| floored at 1 K)."""
+import numpy as np
+import pytest
+
+import aleator_amd as aa
+import oracle_lib as ol
+from gpu_helpers import device_framework, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CH4 = [(0.0, 0.0, 0.0, 4, 0.0)]
+
+
+def test_widom_c0_methane_fp64(ctx):
+    fw, lj = ol.c0_system()
+    dev = device_framework(ctx, fw, lj)
+    n = 3000
+    kh_ref, du_ref = ol.widom(lj, fw, CH4, seed=1, stream=1, n_trials=n, temperature=298.0)
+    s, du = dev.widom(CH4, n, 298.0, seed=1, stream=1, precision=aa.FP64)
+    err = rel_err(du, du_ref)
+    kh = s / n / 298.0
+    print(f"c0 widom: max rel dU err {err.max():.3e}; K_H gpu {kh:.12e} oracle {kh_ref:.12e}")
+    assert err.max() <= 1e-10
+    assert abs(kh - kh_ref) <= 1e-10 * abs(kh_ref)
+
+
+def test_widom_first_call_index_offsets_the_stream(ctx):
+    fw, lj = ol.c0_system()
+    dev = device_framework(ctx, fw, lj)
+    _, du_all = ol.widom(lj, fw, CH4, seed=9, stream=4, n_trials=40, temperature=298.0)
+    _, du = dev.widom(CH4, 25, 298.0, seed=9, stream=4, first_call_index=15 * 3)
+    assert rel_err(du, du_all[15:]).max() <= 1e-10
+
+
+def test_widom_charged_co2_triclinic_fp64(ctx):
+    m = ol.cell_from_params(20.0, 21.0, 22.0, 70.0, 80.0, 95.0)
+    fw = ol.synth_framework(m, 250, seed=7, stream=0, charge_sigma=0.5)
+    eps = np.array([t[0] for t in ol.C0_TYPES] + [ol.CO2_C[0], ol.CO2_O[0]])
+    sig = np.array([t[1] for t in ol.C0_TYPES] + [ol.CO2_C[1], ol.CO2_O[1]])
+    lj = ol.LJ(eps, sig, 8.0)
+    st = ol.EwaldState(fw, 0.35, 6)
+    dev = device_framework(ctx, fw, lj, ewald=(0.35, 8.0, 6))
+    n = 150
+    kh_ref, du_ref = ol.widom(lj, fw, ol.co2_sites(), seed=2, stream=3, n_trials=n, temperature=298.0,
+                              ewald=st, rc_real=8.0)
+    s, du = dev.widom(ol.co2_sites(), n, 298.0, seed=2, stream=3, precision=aa.FP64)
+    err = rel_err(du, du_ref)
+    print(f"charged CO2 widom: max rel dU err {err.max():.3e}; dU range {du_ref.min():.3g}..{du_ref.max():.3g}")
+    assert err.max() <= 1e-10
+    kh = s / n / 298.0
+    assert abs(kh - kh_ref) <= 1e-9 * abs(kh_ref)
+
+
+def test_widom_mixed_precision_c0(ctx):
+    fw, lj = ol.c0_system()
+    dev = device_framework(ctx, fw, lj)
+    n = 3000
+    _, du_ref = ol.widom(lj, fw, CH4, seed=1, stream=1, n_trials=n, temperature=298.0)
+    _, du = dev.widom(CH4, n, 298.0, seed=1, stream=1, precision=aa.MIXED)
+    sel = np.abs(du_ref) > 1.0
+    err = np.abs(du - du_ref)[sel] / np.abs(du_ref[sel])
+    print(f"c0 widom MIXED: max rel {err.max():.3e} median {np.median(err):.3e} "
+          f"within 1e-5 {np.mean(err <= 1e-5):.4f}")
+    assert np.median(err) <= 1e-6
+
+
+def test_widom_rejects_bad_arguments(ctx):
+    fw, lj = ol.c0_system(60, a=12.0)
+    lj.cutoff = 5.9
+    dev = device_framework(ctx, fw, lj)
+    with pytest.raises(ValueError):
+        dev.widom(CH4, 10, -1.0)
+    with pytest.raises(IndexError):
+        dev.widom([(0, 0, 0, 99, 0.0)], 10, 298.0)
