@@ -115,6 +115,52 @@ def grid_point_counts(lattice, spacing: float) -> tuple[int, int, int]:
     return tuple(int(v) for v in n)
 
 
+def supercell_replication(lattice, cutoff: float) -> tuple[int, int, int]:
+    """minimumSupercellReplication (supercell.cc:7-20)."""
+    n = np.zeros(3, dtype=np.int32)
+    lat = _lattice(lattice)
+    _check(_load().aleator_supercell_replication(C.byref(lat), C.c_double(cutoff), _p(n, _i32p)))
+    return tuple(int(v) for v in n)
+
+
+def replicate_supercell(lattice, x, y, z, charge, species, reps):
+    """replicateSupercell (supercell.cc:22-60): returns (lattice, x, y, z, charge, species)."""
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, charge)]
+    sp = np.ascontiguousarray(species, dtype=np.uint32)
+    n = len(keep[0]) * int(np.prod(reps))
+    out = [np.zeros(n) for _ in range(4)]
+    osp = np.zeros(n, dtype=np.uint32)
+    parts = _Particles(_p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(sp, _u32p), len(keep[0]))
+    lat = _lattice(lattice)
+    olat = _Lattice()
+    _check(_load().aleator_replicate_supercell(C.byref(parts), C.byref(lat), C.c_int32(reps[0]), C.c_int32(reps[1]),
+                                               C.c_int32(reps[2]), _p(out[0]), _p(out[1]), _p(out[2]), _p(out[3]),
+                                               _p(osp, _u32p), C.byref(olat)))
+    return np.array(olat.m[:]).reshape(3, 3), out[0], out[1], out[2], out[3], osp
+
+
+def peng_robinson_fugacity(tc, pc, omega, temperature, pressure) -> float:
+    """PengRobinson::fugacityPascal (peng_robinson.cc:10-110)."""
+    out = C.c_double()
+    _check(_load().aleator_peng_robinson_fugacity(C.c_double(tc), C.c_double(pc), C.c_double(omega),
+                                                  C.c_double(temperature), C.c_double(pressure), C.byref(out)))
+    return out.value
+
+
+def lj_tail_insertion_delta(epsilon, sigma, cutoff, lattice, framework_species, sites, mixing=0) -> float:
+    """tailEnergyCorrection(framework + molecule) - tailEnergyCorrection(framework)."""
+    eps = np.ascontiguousarray(epsilon, dtype=np.float64)
+    sig = np.ascontiguousarray(sigma, dtype=np.float64)
+    lj = _LJ(_p(eps), _p(sig), len(eps), cutoff, 0, mixing)
+    sp = np.ascontiguousarray(framework_species, dtype=np.uint32)
+    arr = Framework._sites(sites)
+    lat = _lattice(lattice)
+    out = C.c_double()
+    _check(_load().aleator_lj_tail_insertion_delta(C.byref(lj), C.byref(lat), _p(sp, _u32p), C.c_uint64(len(sp)),
+                                                   arr, C.c_int32(len(sites)), C.byref(out)))
+    return out.value
+
+
 def synth_framework(lattice, n_atoms: int, seed=1, stream=0, min_dist=1.0,
                     proportions=(0.45, 0.30, 0.17, 0.08), charge_sigma=0.0, charge_stream=2):
     """Synthetic framework (no reference counterpart): returns (x, y, z, charge, species)."""
@@ -261,6 +307,98 @@ class Framework:
         """widomInsertionHenryCoefficient: (sum exp(-dU/T) / n_trials) / T."""
         s, _ = self.widom(sites, n_trials, temperature, return_du=False, **kw)
         return (s / n_trials) / temperature
+
+
+class _ChainDesc(C.Structure):
+    _fields_ = [("fugacity_pa", C.c_double), ("temperature", C.c_double), ("seed", C.c_uint64),
+                ("stream", C.c_uint64)]
+
+
+class _ChainStats(C.Structure):
+    _fields_ = [("n_molecules", C.c_int32), ("error", C.c_int32), ("steps", C.c_uint64),
+                ("sum_molecules", C.c_double), ("attempted", C.c_uint64 * 4), ("accepted", C.c_uint64 * 4)]
+
+
+_hip = None
+
+
+def _hip_rt():
+    """HIP runtime entry points for the few device buffers the binding itself needs."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+    return _hip
+
+
+class Gcmc:
+    """Batched single-species GCMC chains (MonteCarloEngine::run, monte_carlo_engine.cc:405-671),
+    one persistent launch advancing every chain.  chains: iterable of
+    (fugacity_pa, temperature, seed, stream)."""
+
+    def __init__(self, fw: Framework, sites, chains, max_translation=1.5, max_rotation=0.5, max_molecules=512):
+        self.fw = fw
+        self.ns = len(sites)
+        chains = list(chains)
+        self.n_chains = len(chains)
+        arr = Framework._sites(sites)
+        desc = (_ChainDesc * len(chains))(*[_ChainDesc(float(f), float(t), int(s), int(st)) for f, t, s, st in chains])
+        self._h = C.c_void_p()
+        _check(_load().aleator_gpu_gcmc_create(fw.handle, arr, C.c_int32(len(sites)), desc, C.c_int32(len(chains)),
+                                               C.c_double(max_translation), C.c_double(max_rotation),
+                                               C.c_int32(max_molecules), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            _load().aleator_gpu_gcmc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, n_steps: int, trace: bool = False):
+        """Advances every chain by n_steps; with trace, returns int32 [n_chains][n_steps] molecule
+        counts after each step."""
+        if not trace:
+            _check(_load().aleator_gpu_gcmc_run(self._h, C.c_uint64(n_steps), None))
+            return None
+        hip = _hip_rt()
+        nbytes = self.n_chains * n_steps * 4
+        d = C.c_void_p()
+        if hip.hipMalloc(C.byref(d), C.c_size_t(nbytes)) != 0:
+            raise DeviceError("hipMalloc failed")
+        try:
+            _check(_load().aleator_gpu_gcmc_run(self._h, C.c_uint64(n_steps), d))
+            out = np.zeros((self.n_chains, n_steps), dtype=np.int32)
+            if hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), d, C.c_size_t(nbytes), 2) != 0:  # device to host
+                raise DeviceError("hipMemcpy failed")
+            return out
+        finally:
+            hip.hipFree(d)
+
+    def stats(self):
+        out = (_ChainStats * self.n_chains)()
+        _check(_load().aleator_gpu_gcmc_stats(self._h, out))
+        return [{"n_molecules": s.n_molecules, "error": s.error, "steps": s.steps, "sum_molecules": s.sum_molecules,
+                 "attempted": list(s.attempted), "accepted": list(s.accepted)} for s in out]
+
+    def reset_stats(self):
+        _check(_load().aleator_gpu_gcmc_reset_stats(self._h))
+
+    def positions(self, chain: int) -> np.ndarray:
+        n = C.c_int32()
+        _check(_load().aleator_gpu_gcmc_positions(self._h, C.c_int32(chain), None, C.c_uint64(0), C.byref(n)))
+        out = np.zeros((n.value, self.ns, 3))
+        _check(_load().aleator_gpu_gcmc_positions(self._h, C.c_int32(chain), _p(out), C.c_uint64(out.size),
+                                                  C.byref(n)))
+        return out
+
+    def electrostatic_energy(self, chain: int) -> float:
+        out = C.c_double()
+        _check(_load().aleator_gpu_gcmc_electrostatic_energy(self._h, C.c_int32(chain), C.byref(out)))
+        return out.value
 
 
 def _charged_sites(sites):

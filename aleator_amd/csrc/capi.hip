@@ -24,30 +24,12 @@ struct aleator_ewald_state : aleator_gpu::EwaldState {};
 
 namespace aleator_gpu {
 
+std::string& last_error_slot() {
+    static thread_local std::string slot;
+    return slot;
+}
+
 namespace {
-thread_local std::string g_last_error;
-
-template <typename F>
-aleator_status guarded(F&& f) {
-    try {
-        f();
-        g_last_error.clear();
-        return ALEATOR_OK;
-    } catch (const Error& e) {
-        g_last_error = e.what();
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return ALEATOR_RUNTIME_ERROR;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return ALEATOR_RUNTIME_ERROR;
-    }
-}
-
-void require(bool ok, aleator_status s, const char* msg) {
-    if (!ok) fail(s, msg);
-}
 
 struct Vec {
     double v[3];
@@ -316,7 +298,7 @@ using namespace aleator_gpu;
 
 extern "C" {
 
-const char* aleator_gpu_last_error(void) { return g_last_error.c_str(); }
+const char* aleator_gpu_last_error(void) { return last_error_slot().c_str(); }
 
 aleator_status aleator_gpu_init(int32_t device, aleator_ctx** out) {
     return guarded([&] {

@@ -1,0 +1,468 @@
+// gcmc.hip -- many independent grand-canonical Monte Carlo chains in one persistent launch.
+//
+// Reference: MonteCarloEngine::run and its single-species moves
+// (src/engines/monte_carlo/monte_carlo_engine.cc:405-439 run, :441-484 translation,
+// :486-554 rotation, :556-619 insertion, :621-671 deletion, :380-403 Ewald resync every 100
+// accepted moves, :814-848 molecule removal), acceptance ratios (gcmc_acceptance.hpp:28-101)
+// and the S(k) update it drives (ewald_incremental_state.cc:95-172, 214-247).
+//
+// MI355X design.  One workgroup of 256 threads owns one chain for the whole launch.  Its
+// structure factor S(k) (half set, see recip.hpp) lives in LDS; molecule positions live in
+// global memory in the reference's molecule order (erase on delete keeps that order, which is
+// what the molecule-index draws select by).  Every random draw of a step is a pure function
+// of the chain's Philox call index, so every thread evaluates the same draws and the same
+// trial positions with no broadcast.  The O(N) energy scans (framework atoms, other guests)
+// and the k-sum are split over the workgroup's threads and reduced through LDS; one thread
+// applies the acceptance rule.  Draw order per move matches the reference exactly, so a chain
+// follows the reference trajectory for the same (seed, stream) until two energies that differ
+// only by rounding straddle an acceptance threshold.
+#include <cmath>
+#include <vector>
+
+#include "device.hpp"
+#include "gcmc.hpp"
+#include "internal.hpp"
+#include "recip.hpp"
+
+namespace aleator_gpu {
+
+namespace {
+
+constexpr int kGThreads = 256;
+constexpr int kMaxKHalf = 2048;   // S(k) half set kept in LDS
+constexpr int kResyncChunk = 16;  // guest sites per resync pass
+
+__device__ inline double uni(const ChainParams& p, uint64_t& call) { return philox_uniform(p.seed, p.stream, call++); }
+
+__device__ inline void frac_of(const GcmcArgs& a, const double c[3], double f[3]) {
+    for (int j = 0; j < 3; ++j) f[j] = c[0] * a.inv[j] + c[1] * a.inv[3 + j] + c[2] * a.inv[6 + j];
+}
+
+// minimum-image distance^2 between fractional points (exact below L_perp/2)
+__device__ inline double dist2(const GcmcArgs& a, double d0, double d1, double d2) {
+    d0 -= rint(d0);
+    d1 -= rint(d1);
+    d2 -= rint(d2);
+    const double x = d0 * a.m[0] + d1 * a.m[3] + d2 * a.m[6];
+    const double y = d0 * a.m[1] + d1 * a.m[4] + d2 * a.m[7];
+    const double z = d0 * a.m[2] + d1 * a.m[5] + d2 * a.m[8];
+    return x * x + y * y + z * z;
+}
+
+__device__ inline double lj_term(const GcmcArgs& a, const double* tab4, int stride, int idx, double r2) {
+    // tab4: [4][stride] = sigma^2, 4 eps, V(rc), V'(rc) (lennard_jones.cc:70-86)
+    const double s2 = tab4[idx] / r2;
+    const double s6 = s2 * s2 * s2;
+    double v = tab4[stride + idx] * (s6 * s6 - s6);
+    if (a.trunc == ALEATOR_LJ_SHIFTED)
+        v -= tab4[2 * stride + idx];
+    else if (a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED)
+        v = v - tab4[2 * stride + idx] - (sqrt(r2) - a.rc_lj) * tab4[3 * stride + idx];
+    return v;
+}
+
+// Reduce four doubles over the workgroup; every thread receives the totals.
+__device__ inline void block_sum4(double v[4], double* lds /* [4][4] */) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = 0; i < 4; ++i) {
+        const double s = wave_sum(v[i]);
+        if (lane == 0) lds[wave * 4 + i] = s;
+    }
+    __syncthreads();
+    for (int i = 0; i < 4; ++i) v[i] = lds[i] + lds[4 + i] + lds[8 + i] + lds[12 + i];
+    __syncthreads();
+}
+
+// Trial positions of one move, in the reference's arithmetic order (no contraction) and draw
+// order: translation (:454-461), rotation about the centroid (:512-532, Rodrigues :40-44),
+// insertion (:568-590, Shoemake quaternion :56-72).  Deletion draws nothing here.
+__device__ void trial_positions(const GcmcArgs& a, const ChainParams& P, uint64_t& call, int move,
+                                const double old_c[kMaxSites][3], double new_c[kMaxSites][3]) {
+#pragma clang fp contract(off)
+    const int ns = a.ns;
+    if (move == 0) {
+        double d[3];
+        for (int j = 0; j < 3; ++j) d[j] = (2.0 * uni(P, call) - 1.0) * a.max_translation;
+        for (int s = 0; s < ns; ++s)
+            for (int j = 0; j < 3; ++j) new_c[s][j] = old_c[s][j] + d[j];
+    } else if (move == 1) {
+        // attemptRotation (:512-532), Rodrigues about the centroid (:40-44)
+        double cen[3] = {0.0, 0.0, 0.0};
+        for (int s = 0; s < ns; ++s)
+            for (int j = 0; j < 3; ++j) cen[j] = cen[j] + old_c[s][j];
+        const double inv_n = 1.0 / (double)ns;
+        for (int j = 0; j < 3; ++j) cen[j] = cen[j] * inv_n;
+        const double cos_t = 2.0 * uni(P, call) - 1.0;
+        const double om = 1.0 - cos_t * cos_t;
+        const double sin_t = sqrt((0.0 < om) ? om : 0.0);
+        const double phi = 2.0 * kPi * uni(P, call);
+        const double ax[3] = {sin_t * cos(phi), sin_t * sin(phi), cos_t};
+        const double angle = (2.0 * uni(P, call) - 1.0) * a.max_rotation;
+        const double c = cos(angle), sn = sin(angle);
+        for (int s = 0; s < ns; ++s) {
+            const double v[3] = {old_c[s][0] - cen[0], old_c[s][1] - cen[1], old_c[s][2] - cen[2]};
+            const double cr[3] = {ax[1] * v[2] - ax[2] * v[1], ax[2] * v[0] - ax[0] * v[2],
+                                  ax[0] * v[1] - ax[1] * v[0]};
+            const double dv = ax[0] * v[0] + ax[1] * v[1] + ax[2] * v[2];
+            for (int j = 0; j < 3; ++j)
+                new_c[s][j] = cen[j] + ((v[j] * c + cr[j] * sn) + ax[j] * (dv * (1.0 - c)));
+        }
+    } else if (move == 2) {
+        // attemptInsertion (:568-590)
+        const double u0 = uni(P, call), u1 = uni(P, call), u2 = uni(P, call);
+        double ref[3];
+        for (int j = 0; j < 3; ++j) ref[j] = a.m[j] * u0 + a.m[3 + j] * u1 + a.m[6 + j] * u2;
+        if (ns == 1) {
+            for (int j = 0; j < 3; ++j) new_c[0][j] = ref[j] + a.local[0][j];
+        } else {
+            const double v0 = uni(P, call), v1 = uni(P, call), v2 = uni(P, call);
+            const double r1 = sqrt(1.0 - v0), r2 = sqrt(v0);
+            const double th1 = 2.0 * kPi * v1, th2 = 2.0 * kPi * v2;
+            const double qw = r2 * cos(th2), qx = r1 * sin(th1), qy = r1 * cos(th1), qz = r2 * sin(th2);
+            for (int s = 0; s < ns; ++s) {
+                const double vx = a.local[s][0], vy = a.local[s][1], vz = a.local[s][2];
+                const double tx = (qy * vz - qz * vy) * 2.0;
+                const double ty = (qz * vx - qx * vz) * 2.0;
+                const double tz = (qx * vy - qy * vx) * 2.0;
+                new_c[s][0] = ref[0] + ((vx + tx * qw) + (qy * tz - qz * ty));
+                new_c[s][1] = ref[1] + ((vy + ty * qw) + (qz * tx - qx * tz));
+                new_c[s][2] = ref[2] + ((vz + tz * qw) + (qx * ty - qy * tx));
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a) {
+    __shared__ double s_re[kMaxKHalf], s_im[kMaxKHalf];
+    __shared__ cplx phase[3 * (2 * kMaxK + 1) * 2 * kMaxSites];
+    __shared__ cplx rphase[3 * (2 * kMaxK + 1) * kResyncChunk];
+    __shared__ double site4[2 * kMaxSites * 4];
+    __shared__ double red[16];
+    __shared__ int s_accept;
+
+    const int chain = blockIdx.x;
+    const int tid = threadIdx.x;
+    const ChainParams P = a.params[chain];
+    ChainState* st = a.state + chain;
+    double* pos = a.mol_pos + (size_t)chain * a.max_mol * a.ns * 3;
+    const int ns = a.ns;
+    const int nkk = 2 * a.kmax + 1;
+
+    if (a.charged)
+        for (int t = tid; t < a.nk_half; t += kGThreads) {
+            s_re[t] = a.sre[(size_t)chain * a.nk_half + t];
+            s_im[t] = a.sim[(size_t)chain * a.nk_half + t];
+        }
+    __syncthreads();
+
+    uint64_t call = st->call;
+    int n_mol = st->n_mol;
+    int since_resync = st->moves_since_resync;
+    double self_e = st->self_energy, excl_e = st->exclusion_energy;
+    double sum_n = st->sum_n;
+    uint64_t att[4] = {st->attempted[0], st->attempted[1], st->attempted[2], st->attempted[3]};
+    uint64_t acc[4] = {st->accepted[0], st->accepted[1], st->accepted[2], st->accepted[3]};
+    int error = st->error;
+    const double volume = fabs(a.volume);
+
+    for (uint64_t step = 0; step < a.n_steps && !error; ++step) {
+        const double pick = uni(P, call);
+        const int move = pick < 0.25 ? 0 : (pick < 0.5 ? 1 : (pick < 0.75 ? 2 : 3));
+        bool skip = (move != 2 && n_mol == 0) || (move == 1 && ns < 2);
+        double old_c[kMaxSites][3], new_c[kMaxSites][3];
+        int mi = -1;
+        if (!skip) {
+            if (move != 2) {
+                mi = (int)(uni(P, call) * (double)n_mol);
+                for (int s = 0; s < ns; ++s)
+                    for (int j = 0; j < 3; ++j) old_c[s][j] = pos[((size_t)mi * ns + s) * 3 + j];
+            }
+            trial_positions(a, P, call, move, old_c, new_c);
+        }
+        if (skip) {
+            sum_n += (double)n_mol;
+            if (a.n_trace && tid == 0) a.n_trace[(size_t)chain * a.n_steps + step] = n_mol;
+            continue;
+        }
+        const bool has_old = move != 2, has_new = move != 3;
+        // fractional coordinates of the old (rows 0..ns-1) and new (rows ns..2ns-1) sites
+        double fo[kMaxSites][3], fn[kMaxSites][3];
+        for (int s = 0; s < ns; ++s) {
+            if (has_old) frac_of(a, old_c[s], fo[s]);
+            if (has_new) frac_of(a, new_c[s], fn[s]);
+        }
+        // ---------------- pair energies: LJ and real-space Ewald, old and new
+        double e4[4] = {0.0, 0.0, 0.0, 0.0};  // lj_old, lj_new, real_old, real_new
+        for (int j = tid; j < a.natoms; j += kGThreads) {
+            const double ax = a.fx[j], ay = a.fy[j], az = a.fz[j], qj = a.q[j];
+            const int t = a.type[j];
+            for (int s = 0; s < ns; ++s) {
+                if (has_old) {
+                    const double r2 = dist2(a, ax - fo[s][0], ay - fo[s][1], az - fo[s][2]);
+                    if (r2 <= a.rc2_lj) e4[0] += lj_term(a, a.ljtab, kMaxSites * kMaxTypes, s * kMaxTypes + t, r2);
+                    if (a.charged && r2 <= a.rc2_real) {
+                        const double r = sqrt(r2);
+                        e4[2] += a.charge[s] * qj * (erfc(a.alpha * r) / r);
+                    }
+                }
+                if (has_new) {
+                    const double r2 = dist2(a, ax - fn[s][0], ay - fn[s][1], az - fn[s][2]);
+                    if (r2 <= a.rc2_lj) e4[1] += lj_term(a, a.ljtab, kMaxSites * kMaxTypes, s * kMaxTypes + t, r2);
+                    if (a.charged && r2 <= a.rc2_real) {
+                        const double r = sqrt(r2);
+                        e4[3] += a.charge[s] * qj * (erfc(a.alpha * r) / r);
+                    }
+                }
+            }
+        }
+        for (int g = tid; g < n_mol; g += kGThreads) {
+            if (g == mi) continue;
+            for (int u = 0; u < ns; ++u) {
+                const double* gp = pos + ((size_t)g * ns + u) * 3;
+                const double gc[3] = {gp[0], gp[1], gp[2]};
+                double gf[3];
+                frac_of(a, gc, gf);
+                for (int s = 0; s < ns; ++s) {
+                    const int idx = s * kMaxSites + u;
+                    if (has_old) {
+                        const double r2 = dist2(a, gf[0] - fo[s][0], gf[1] - fo[s][1], gf[2] - fo[s][2]);
+                        if (r2 <= a.rc2_lj) e4[0] += lj_term(a, a.ljgg, kMaxSites * kMaxSites, idx, r2);
+                        if (a.charged && r2 <= a.rc2_real) {
+                            const double r = sqrt(r2);
+                            e4[2] += a.charge[s] * a.charge[u] * (erfc(a.alpha * r) / r);
+                        }
+                    }
+                    if (has_new) {
+                        const double r2 = dist2(a, gf[0] - fn[s][0], gf[1] - fn[s][1], gf[2] - fn[s][2]);
+                        if (r2 <= a.rc2_lj) e4[1] += lj_term(a, a.ljgg, kMaxSites * kMaxSites, idx, r2);
+                        if (a.charged && r2 <= a.rc2_real) {
+                            const double r = sqrt(r2);
+                            e4[3] += a.charge[s] * a.charge[u] * (erfc(a.alpha * r) / r);
+                        }
+                    }
+                }
+            }
+        }
+        block_sum4(e4, red);
+        // ---------------- reciprocal space: removed = old sites, added = new sites
+        double recip = 0.0;
+        const int n_rem = has_old ? ns : 0, n_add = has_new ? ns : 0, nsite = n_rem + n_add;
+        if (a.charged) {
+            if (tid < nsite) {
+                const double* f = tid < n_rem ? fo[tid] : fn[tid - n_rem];
+                site4[4 * tid] = f[0];
+                site4[4 * tid + 1] = f[1];
+                site4[4 * tid + 2] = f[2];
+                site4[4 * tid + 3] = a.charge[tid < n_rem ? tid : tid - n_rem];
+            }
+            __syncthreads();
+            for (int i = tid; i < 3 * nkk * nsite; i += kGThreads) {
+                const int s = i % nsite, mm = (i / nsite) % nkk, axis = i / (nsite * nkk);
+                double sv, cv;
+                sincospi(2.0 * (double)(mm - a.kmax) * site4[4 * s + axis], &sv, &cv);
+                phase[i] = {cv, sv};
+            }
+            __syncthreads();
+            double accr = 0.0;
+            for (int t = tid; t < a.nk_half; t += kGThreads) {
+                const int4 n = a.kn[t];
+                double dre = 0.0, dim = 0.0;
+                for (int s = 0; s < nsite; ++s) {
+                    const cplx e = cmul(cmul(phase[(n.x + a.kmax) * nsite + s], phase[(nkk + n.y + a.kmax) * nsite + s]),
+                                        phase[(2 * nkk + n.z + a.kmax) * nsite + s]);
+                    const double qs = s < n_rem ? -site4[4 * s + 3] : site4[4 * s + 3];
+                    dre += qs * e.re;
+                    dim += qs * e.im;
+                }
+                accr += a.kw[t] * (2.0 * (s_re[t] * dre + s_im[t] * dim) + (dre * dre + dim * dim));
+            }
+            double r4[4] = {accr, 0.0, 0.0, 0.0};
+            block_sum4(r4, red);
+            recip = a.recip_scale * r4[0];
+        }
+        // ---------------- acceptance (one thread), gcmc_acceptance.hpp:28-101
+        if (tid == 0) {
+            double self_d = 0.0, excl_d = 0.0;
+            if (a.charged) {
+                double q2 = 0.0;
+                for (int s = 0; s < ns; ++s) q2 += a.charge[s] * a.charge[s];
+                double ex_new = 0.0, ex_old = 0.0;
+                for (int i = 0; i < ns; ++i)
+                    for (int j2 = i + 1; j2 < ns; ++j2) {
+                        const double qq = a.charge[i] * a.charge[j2];
+                        if (has_new) {
+                            const double r = sqrt(dist2(a, fn[j2][0] - fn[i][0], fn[j2][1] - fn[i][1], fn[j2][2] - fn[i][2]));
+                            ex_new += qq * (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
+                        }
+                        if (has_old) {
+                            const double r = sqrt(dist2(a, fo[j2][0] - fo[i][0], fo[j2][1] - fo[i][1], fo[j2][2] - fo[i][2]));
+                            ex_old += qq * (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
+                        }
+                    }
+                self_d = a.self_pref * ((has_new ? q2 : 0.0) - (has_old ? q2 : 0.0));
+                excl_d = -a.kc * (ex_new - ex_old);
+                red[0] = self_d;
+                red[1] = excl_d;
+            }
+            const double recip_total = recip + self_d + excl_d;
+            const double T = P.temperature;
+            double du, ratio;
+            if (move <= 1) {
+                du = (e4[1] - e4[0]) + a.kc * (e4[3] - e4[2]) + recip_total;
+                ratio = exp(-du / T);
+            } else if (move == 2) {
+                du = e4[1] + a.kc * e4[3] + recip_total;
+                ratio = (P.fugacity_internal * volume / T) / (double)(n_mol + 1) * exp(-du / T);
+            } else {
+                du = -(e4[0] + a.kc * e4[2]) + recip_total;
+                ratio = (double)n_mol / (P.fugacity_internal * volume / T) * exp(-du / T);
+            }
+            s_accept = uni(P, call) < (ratio < 1.0 ? ratio : 1.0) ? 1 : 0;
+        } else {
+            call += 1;  // the accept draw, taken by thread 0
+        }
+        __syncthreads();
+        const bool accept = s_accept != 0;
+        const double self_d = a.charged ? red[0] : 0.0, excl_d = a.charged ? red[1] : 0.0;
+        __syncthreads();
+        att[move] += 1;
+        if (accept) {
+            acc[move] += 1;
+            if (move == 2 && n_mol >= a.max_mol) {
+                error = 1;
+                break;
+            }
+            // commit S(k) (the same delta, recomputed, as commit* does, :161-172)
+            if (a.charged) {
+                for (int t = tid; t < a.nk_half; t += kGThreads) {
+                    const int4 n = a.kn[t];
+                    double dre = 0.0, dim = 0.0;
+                    for (int s = 0; s < nsite; ++s) {
+                        const cplx e = cmul(cmul(phase[(n.x + a.kmax) * nsite + s], phase[(nkk + n.y + a.kmax) * nsite + s]),
+                                            phase[(2 * nkk + n.z + a.kmax) * nsite + s]);
+                        const double qs = s < n_rem ? -site4[4 * s + 3] : site4[4 * s + 3];
+                        dre += qs * e.re;
+                        dim += qs * e.im;
+                    }
+                    s_re[t] += dre;
+                    s_im[t] += dim;
+                }
+                self_e += self_d;
+                excl_e += excl_d;
+            }
+            // positions
+            if (move <= 1) {
+                if (tid < ns * 3) pos[((size_t)mi * ns + tid / 3) * 3 + tid % 3] = new_c[tid / 3][tid % 3];
+            } else if (move == 2) {
+                if (tid < ns * 3) pos[((size_t)n_mol * ns + tid / 3) * 3 + tid % 3] = new_c[tid / 3][tid % 3];
+                n_mol += 1;
+            } else {
+                // erase molecule mi, keeping the order of the others (:821)
+                const size_t first = (size_t)mi * ns * 3, count = (size_t)(n_mol - 1 - mi) * ns * 3;
+                for (size_t base = 0; base < count; base += kGThreads) {
+                    double v = 0.0;
+                    if (base + tid < count) v = pos[first + ns * 3 + base + tid];
+                    __syncthreads();
+                    if (base + tid < count) pos[first + base + tid] = v;
+                    __syncthreads();
+                }
+                n_mol -= 1;
+            }
+            __threadfence_block();
+            __syncthreads();
+            // maybeResyncEwald (:380-403)
+            if (a.charged && ++since_resync >= 100) {
+                since_resync = 0;
+                for (int t = tid; t < a.nk_half; t += kGThreads) {
+                    s_re[t] = a.fw_sre[t];
+                    s_im[t] = a.fw_sim[t];
+                }
+                const int total = n_mol * ns;
+                for (int base = 0; base < total; base += kResyncChunk) {
+                    const int cnt = min(kResyncChunk, total - base);
+                    __syncthreads();
+                    for (int i = tid; i < 3 * nkk * kResyncChunk; i += kGThreads) {
+                        const int s = i % kResyncChunk, mm = (i / kResyncChunk) % nkk, axis = i / (kResyncChunk * nkk);
+                        cplx v = {0.0, 0.0};
+                        if (s < cnt) {
+                            const double* gp = pos + (size_t)(base + s) * 3;
+                            const double gc[3] = {gp[0], gp[1], gp[2]};
+                            double gf[3];
+                            frac_of(a, gc, gf);
+                            double sv, cv;
+                            sincospi(2.0 * (double)(mm - a.kmax) * gf[axis], &sv, &cv);
+                            v = {cv, sv};
+                        }
+                        rphase[i] = v;
+                    }
+                    __syncthreads();
+                    for (int t = tid; t < a.nk_half; t += kGThreads) {
+                        const int4 n = a.kn[t];
+                        double dre = 0.0, dim = 0.0;
+                        for (int s = 0; s < cnt; ++s) {
+                            const cplx e = cmul(cmul(rphase[(n.x + a.kmax) * kResyncChunk + s],
+                                                     rphase[(nkk + n.y + a.kmax) * kResyncChunk + s]),
+                                                rphase[(2 * nkk + n.z + a.kmax) * kResyncChunk + s]);
+                            const double qs = a.charge[(base + s) % ns];
+                            dre += qs * e.re;
+                            dim += qs * e.im;
+                        }
+                        s_re[t] += dre;
+                        s_im[t] += dim;
+                    }
+                }
+                // self and exclusion from scratch (ewald_incremental_state.cc:230-246)
+                double q2 = 0.0;
+                for (int s = 0; s < ns; ++s) q2 += a.charge[s] * a.charge[s];
+                self_e = a.fw_self + a.self_pref * (q2 * (double)n_mol);
+                double ex = 0.0;
+                for (int g = 0; g < n_mol; ++g)
+                    for (int i = 0; i < ns; ++i)
+                        for (int j2 = i + 1; j2 < ns; ++j2) {
+                            double fi[3], fj[3];
+                            const double* pi = pos + ((size_t)g * ns + i) * 3;
+                            const double* pj = pos + ((size_t)g * ns + j2) * 3;
+                            const double ci[3] = {pi[0], pi[1], pi[2]}, cj[3] = {pj[0], pj[1], pj[2]};
+                            frac_of(a, ci, fi);
+                            frac_of(a, cj, fj);
+                            const double r = sqrt(dist2(a, fj[0] - fi[0], fj[1] - fi[1], fj[2] - fi[2]));
+                            ex += a.charge[i] * a.charge[j2] * (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
+                        }
+                excl_e = -a.kc * ex;
+                __syncthreads();
+            }
+        }
+        sum_n += (double)n_mol;
+        if (a.n_trace && tid == 0) a.n_trace[(size_t)chain * a.n_steps + step] = n_mol;
+    }
+    __syncthreads();
+    if (a.charged)
+        for (int t = tid; t < a.nk_half; t += kGThreads) {
+            a.sre[(size_t)chain * a.nk_half + t] = s_re[t];
+            a.sim[(size_t)chain * a.nk_half + t] = s_im[t];
+        }
+    if (tid == 0) {
+        st->call = call;
+        st->n_mol = n_mol;
+        st->moves_since_resync = since_resync;
+        st->self_energy = self_e;
+        st->exclusion_energy = excl_e;
+        st->sum_n = sum_n;
+        st->steps += a.n_steps;
+        for (int i = 0; i < 4; ++i) {
+            st->attempted[i] = att[i];
+            st->accepted[i] = acc[i];
+        }
+        st->error = error;
+    }
+}
+
+}  // namespace
+
+void launch_gcmc(const GcmcArgs& a, int n_chains, hipStream_t s) {
+    if (a.nk_half > kMaxKHalf) fail(ALEATOR_NOT_IMPLEMENTED, "gcmc: more than 4096 reciprocal terms");
+    hipLaunchKernelGGL(gcmc_chain_kernel, dim3(n_chains), dim3(kGThreads), 0, s, a);
+    ALEATOR_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace aleator_gpu

@@ -1,0 +1,277 @@
+// gcmc_api.hip -- C ABI of the batched GCMC chains (include/aleator_gpu.h).
+//
+// Host side of MonteCarloEngine's GCMC constructor checks (monte_carlo_engine.cc:174-297):
+// positive temperature and fugacity, a neutral species when the framework is charged; the
+// fugacity conversion Pa -> K/A^3 (monte_carlo_engine.hpp:184); per-chain device state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "aleator_gpu.h"
+#include "gcmc.hpp"
+#include "internal.hpp"
+
+struct aleator_gcmc {
+    aleator_gpu::Framework* fw = nullptr;
+    int n_chains = 0;
+    aleator_gpu::GcmcArgs args{};
+    aleator_gpu::ChainParams* d_params = nullptr;
+    aleator_gpu::ChainState* d_state = nullptr;
+    double* d_pos = nullptr;
+    double* d_sre = nullptr;
+    double* d_sim = nullptr;
+    double* d_ljtab = nullptr;
+    double* d_ljgg = nullptr;
+};
+
+namespace aleator_gpu {
+namespace {
+std::vector<ChainState> download_states(aleator_gcmc* g) {
+    std::vector<ChainState> st(g->n_chains);
+    ALEATOR_HIP_CHECK(hipMemcpyAsync(st.data(), g->d_state, st.size() * sizeof(ChainState), hipMemcpyDeviceToHost,
+                                     g->fw->ctx->stream));
+    ALEATOR_HIP_CHECK(hipStreamSynchronize(g->fw->ctx->stream));
+    return st;
+}
+}  // namespace
+}  // namespace aleator_gpu
+
+using namespace aleator_gpu;
+
+extern "C" {
+
+aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_molecule_site* sites, int32_t n_sites,
+                                       const aleator_gcmc_chain* chains, int32_t n_chains, double max_translation,
+                                       double max_rotation, int32_t max_molecules, aleator_gcmc** out) {
+    return guarded([&] {
+        auto* fw = reinterpret_cast<Framework*>(fw_in);
+        require(fw && sites && chains && out, ALEATOR_INVALID_ARGUMENT, "null argument");
+        require(n_sites >= 1, ALEATOR_INVALID_ARGUMENT, "MonteCarloEngine: every GCMC species must have at least one site");
+        require(n_sites <= kMaxSites, ALEATOR_NOT_IMPLEMENTED, "gcmc: at most 8 sites per molecule");
+        require(n_chains >= 1, ALEATOR_INVALID_ARGUMENT, "gcmc: at least one chain");
+        require(max_molecules >= 1, ALEATOR_INVALID_ARGUMENT, "gcmc: max_molecules must be positive");
+        double qsum = 0.0;
+        for (int s = 0; s < n_sites; ++s) {
+            require(sites[s].lj_species < fw->eps.size(), ALEATOR_OUT_OF_RANGE, "site LJ species out of range");
+            qsum += sites[s].charge;
+        }
+        if (fw->charged)
+            require(!(std::abs(qsum) > 1e-8), ALEATOR_INVALID_ARGUMENT,
+                    "MonteCarloEngine: GCMC species is not individually charge-neutral");
+        std::vector<ChainParams> params(n_chains);
+        for (int c = 0; c < n_chains; ++c) {
+            require(chains[c].temperature > 0.0, ALEATOR_INVALID_ARGUMENT,
+                    "MonteCarloEngine: temperatureKelvin must be positive");
+            const double f = chains[c].fugacity_pa * (1.0 / (1.380649e-23 * 1.0e30));
+            require(f > 0.0, ALEATOR_INVALID_ARGUMENT, "MonteCarloEngine: every fugacityPascal must be positive");
+            params[c] = {f, chains[c].temperature, chains[c].seed, chains[c].stream};
+        }
+        auto g = std::make_unique<aleator_gcmc>();
+        g->fw = fw;
+        g->n_chains = n_chains;
+        GcmcArgs& a = g->args;
+        hipStream_t s = fw->ctx->stream;
+        a.fx = fw->d_fx;
+        a.fy = fw->d_fy;
+        a.fz = fw->d_fz;
+        a.q = fw->d_q;
+        a.type = fw->d_type;
+        a.natoms = (int)fw->n;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                a.m[3 * i + j] = fw->lat.m[i][j];
+                a.inv[3 * i + j] = fw->lat.inv[i][j];
+            }
+        a.volume = fw->lat.volume;
+        a.rc_lj = fw->rc;
+        a.rc2_lj = fw->rc * fw->rc;
+        a.trunc = fw->trunc;
+        a.charged = fw->charged ? 1 : 0;
+        a.alpha = fw->alpha;
+        a.rc2_real = fw->rc_real * fw->rc_real;
+        a.kc = kCoulomb;
+        a.recip_scale = 2.0 * kCoulomb * (2.0 * kPi / std::abs(fw->lat.volume));
+        a.self_pref = -kCoulomb * (fw->alpha / std::sqrt(kPi));
+        a.fw_self = fw->self_energy;
+        a.kn = reinterpret_cast<const int4*>(fw->d_kn);
+        a.kw = fw->d_kw;
+        a.fw_sre = fw->d_sre;
+        a.fw_sim = fw->d_sim;
+        a.nk_half = fw->nk_half;
+        a.kmax = fw->kmax;
+        a.ns = n_sites;
+        for (int k = 0; k < n_sites; ++k) {
+            a.local[k][0] = sites[k].x;
+            a.local[k][1] = sites[k].y;
+            a.local[k][2] = sites[k].z;
+            a.charge[k] = sites[k].charge;
+        }
+        a.max_translation = max_translation;
+        a.max_rotation = max_rotation;
+        a.max_mol = max_molecules;
+        // LJ constants: site vs framework type, and site vs site
+        std::vector<double> tab(4 * kMaxSites * kMaxTypes, 0.0), gg(4 * kMaxSites * kMaxSites, 0.0);
+        for (int k = 0; k < n_sites; ++k) {
+            for (size_t t = 0; t < fw->type_species.size(); ++t) {
+                double v[4];
+                lj_pair_constants(*fw, sites[k].lj_species, fw->type_species[t], v[0], v[1], v[2], v[3]);
+                for (int c = 0; c < 4; ++c) tab[(c * kMaxSites + k) * kMaxTypes + t] = v[c];
+            }
+            for (int u = 0; u < n_sites; ++u) {
+                double v[4];
+                lj_pair_constants(*fw, sites[k].lj_species, sites[u].lj_species, v[0], v[1], v[2], v[3]);
+                for (int c = 0; c < 4; ++c) gg[(c * kMaxSites + k) * kMaxSites + u] = v[c];
+            }
+        }
+        auto upload = [&](auto** dst, const auto* src, size_t n) {
+            using T = std::remove_pointer_t<std::remove_pointer_t<decltype(dst)>>;
+            ALEATOR_HIP_CHECK(hipMalloc(dst, std::max<size_t>(n, 1) * sizeof(T)));
+            if (n) ALEATOR_HIP_CHECK(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+        };
+        try {
+            upload(&g->d_ljtab, tab.data(), tab.size());
+            upload(&g->d_ljgg, gg.data(), gg.size());
+            upload(&g->d_params, params.data(), params.size());
+            std::vector<ChainState> st(n_chains);
+            std::memset(st.data(), 0, st.size() * sizeof(ChainState));
+            for (auto& c : st) c.self_energy = fw->charged ? fw->self_energy : 0.0;
+            upload(&g->d_state, st.data(), st.size());
+            ALEATOR_HIP_CHECK(hipMalloc(&g->d_pos, (size_t)n_chains * max_molecules * n_sites * 3 * sizeof(double)));
+            if (fw->charged) {
+                const size_t nk = (size_t)fw->nk_half;
+                ALEATOR_HIP_CHECK(hipMalloc(&g->d_sre, n_chains * nk * sizeof(double)));
+                ALEATOR_HIP_CHECK(hipMalloc(&g->d_sim, n_chains * nk * sizeof(double)));
+                for (int c = 0; c < n_chains; ++c) {
+                    ALEATOR_HIP_CHECK(hipMemcpyAsync(g->d_sre + c * nk, fw->d_sre, nk * sizeof(double),
+                                                     hipMemcpyDeviceToDevice, s));
+                    ALEATOR_HIP_CHECK(hipMemcpyAsync(g->d_sim + c * nk, fw->d_sim, nk * sizeof(double),
+                                                     hipMemcpyDeviceToDevice, s));
+                }
+            }
+            ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_state,
+                            (void*)g->d_pos, (void*)g->d_sre, (void*)g->d_sim})
+                if (p) (void)hipFree(p);
+            throw;
+        }
+        a.ljtab = g->d_ljtab;
+        a.ljgg = g->d_ljgg;
+        a.params = g->d_params;
+        a.state = g->d_state;
+        a.mol_pos = g->d_pos;
+        a.sre = g->d_sre;
+        a.sim = g->d_sim;
+        *out = g.release();
+    });
+}
+
+aleator_status aleator_gpu_gcmc_run(aleator_gcmc* g, uint64_t n_steps, int32_t* trace_device) {
+    return guarded([&] {
+        require(g != nullptr, ALEATOR_INVALID_ARGUMENT, "null argument");
+        if (n_steps == 0) return;
+        GcmcArgs a = g->args;
+        a.n_steps = n_steps;
+        a.n_trace = trace_device;
+        Context& ctx = *g->fw->ctx;
+        ctx.time_begin();
+        launch_gcmc(a, g->n_chains, ctx.stream);
+        ctx.time_end();
+    });
+}
+
+aleator_status aleator_gpu_gcmc_stats(aleator_gcmc* g, aleator_gcmc_stats* out) {
+    return guarded([&] {
+        require(g && out, ALEATOR_INVALID_ARGUMENT, "null argument");
+        const auto st = download_states(g);
+        for (int c = 0; c < g->n_chains; ++c) {
+            out[c].n_molecules = st[c].n_mol;
+            out[c].error = st[c].error;
+            out[c].steps = st[c].steps;
+            out[c].sum_molecules = st[c].sum_n;
+            for (int i = 0; i < 4; ++i) {
+                out[c].attempted[i] = st[c].attempted[i];
+                out[c].accepted[i] = st[c].accepted[i];
+            }
+        }
+    });
+}
+
+aleator_status aleator_gpu_gcmc_reset_stats(aleator_gcmc* g) {
+    return guarded([&] {
+        require(g != nullptr, ALEATOR_INVALID_ARGUMENT, "null argument");
+        auto st = download_states(g);
+        for (auto& c : st) {
+            c.sum_n = 0.0;
+            c.steps = 0;
+            std::memset(c.attempted, 0, sizeof(c.attempted));
+            std::memset(c.accepted, 0, sizeof(c.accepted));
+        }
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(g->d_state, st.data(), st.size() * sizeof(ChainState), hipMemcpyHostToDevice,
+                                         g->fw->ctx->stream));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(g->fw->ctx->stream));
+    });
+}
+
+aleator_status aleator_gpu_gcmc_positions(aleator_gcmc* g, int32_t chain, double* xyz, uint64_t capacity,
+                                          int32_t* n_molecules) {
+    return guarded([&] {
+        require(g && n_molecules, ALEATOR_INVALID_ARGUMENT, "null argument");
+        require(chain >= 0 && chain < g->n_chains, ALEATOR_OUT_OF_RANGE, "chain index out of range");
+        const auto st = download_states(g);
+        const int n = st[chain].n_mol;
+        *n_molecules = n;
+        const size_t need = (size_t)n * g->args.ns * 3;
+        if (!xyz) return;
+        require(capacity >= need, ALEATOR_INVALID_ARGUMENT, "positions: capacity too small");
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(xyz, g->d_pos + (size_t)chain * g->args.max_mol * g->args.ns * 3,
+                                         need * sizeof(double), hipMemcpyDeviceToHost, g->fw->ctx->stream));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(g->fw->ctx->stream));
+    });
+}
+
+aleator_status aleator_gpu_gcmc_electrostatic_energy(aleator_gcmc* g, int32_t chain, double* energy) {
+    return guarded([&] {
+        require(g && energy, ALEATOR_INVALID_ARGUMENT, "null argument");
+        require(chain >= 0 && chain < g->n_chains, ALEATOR_OUT_OF_RANGE, "chain index out of range");
+        Framework& fw = *g->fw;
+        if (!fw.charged) {
+            *energy = 0.0;
+            return;
+        }
+        const auto st = download_states(g);
+        const size_t nkh = (size_t)fw.nk_half, nk = fw.kset.size();
+        std::vector<double> hr(nkh), hi(nkh);
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(hr.data(), g->d_sre + chain * nkh, nkh * sizeof(double),
+                                         hipMemcpyDeviceToHost, fw.ctx->stream));
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(hi.data(), g->d_sim + chain * nkh, nkh * sizeof(double),
+                                         hipMemcpyDeviceToHost, fw.ctx->stream));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(fw.ctx->stream));
+        // committedEnergy (ewald_incremental_state.cc:85-93) over the full reference-order set
+        double sum = 0.0;
+        for (size_t t = 0; t < nk; ++t) {
+            const size_t h = t < nkh ? t : nk - 1 - t;
+            sum += fw.kset.weight[t] * (hr[h] * hr[h] + hi[h] * hi[h]);
+        }
+        const double recip = sum * kCoulomb * (2.0 * kPi / std::abs(fw.lat.volume));
+        *energy = recip + st[chain].self_energy + st[chain].exclusion_energy;
+    });
+}
+
+aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* g) {
+    return guarded([&] {
+        if (!g) return;
+        (void)hipStreamSynchronize(g->fw->ctx->stream);
+        for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_state, (void*)g->d_pos,
+                        (void*)g->d_sre, (void*)g->d_sim})
+            if (p) (void)hipFree(p);
+        delete g;
+    });
+}
+
+}  // extern "C"

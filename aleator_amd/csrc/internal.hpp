@@ -21,6 +21,32 @@ struct Error : std::runtime_error {
 
 [[noreturn]] inline void fail(aleator_status s, const std::string& msg) { throw Error(s, msg); }
 
+// Message of the last failure on the calling thread (aleator_gpu_last_error).
+std::string& last_error_slot();
+
+// Runs `f`, mapping exceptions onto the status codes of the C boundary.
+template <typename F>
+aleator_status guarded(F&& f) {
+    try {
+        f();
+        last_error_slot().clear();
+        return ALEATOR_OK;
+    } catch (const Error& e) {
+        last_error_slot() = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        last_error_slot() = "host allocation failed";
+        return ALEATOR_RUNTIME_ERROR;
+    } catch (const std::exception& e) {
+        last_error_slot() = e.what();
+        return ALEATOR_RUNTIME_ERROR;
+    }
+}
+
+inline void require(bool ok, aleator_status s, const char* msg) {
+    if (!ok) fail(s, msg);
+}
+
 #define ALEATOR_HIP_CHECK(expr)                                                                 \
     do {                                                                                        \
         hipError_t e_ = (expr);                                                                 \

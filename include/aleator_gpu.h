@@ -221,6 +221,70 @@ aleator_status aleator_gpu_ewald_state_recompute(aleator_ewald_state* st,
 aleator_status aleator_gpu_ewald_state_raw(aleator_ewald_state* st, double* out, uint64_t capacity,
                                            uint64_t* written);
 
+/* ---- batched grand-canonical Monte Carlo chains ------------------------ */
+/* Many independent single-species GCMC chains against one device framework, each with its
+ * own Philox4x32Rng(seed, stream), molecule list and S(k) state, advanced together in one
+ * persistent launch.  Each chain follows MonteCarloEngine::run (monte_carlo_engine.cc:405-439)
+ * with one species: translation / rotation / insertion / deletion at 25 % each, the
+ * reference's draw order per move (:441-671), acceptance by gcmc_acceptance.hpp:28-101,
+ * Ewald resync every 100 accepted moves (:380-403).  Energies are the exact direct sums (no
+ * energy grid): LJ against framework and guests, real-space Ewald, S(k) reciprocal, self and
+ * exclusion terms. */
+typedef struct aleator_gcmc aleator_gcmc;
+typedef struct {
+    double fugacity_pa; /* reservoir fugacity (e.g. aleator_peng_robinson_fugacity) */
+    double temperature;
+    uint64_t seed;
+    uint64_t stream;
+} aleator_gcmc_chain;
+typedef struct {
+    int32_t n_molecules;
+    int32_t error; /* 1: the chain needed more than max_molecules molecules and stopped */
+    uint64_t steps;
+    double sum_molecules; /* sum over steps of the molecule count after the step */
+    uint64_t attempted[4]; /* translation, rotation, insertion, deletion */
+    uint64_t accepted[4];
+} aleator_gcmc_stats;
+aleator_status aleator_gpu_gcmc_create(aleator_framework* fw, const aleator_molecule_site* sites,
+                                       int32_t n_sites, const aleator_gcmc_chain* chains,
+                                       int32_t n_chains, double max_translation, double max_rotation,
+                                       int32_t max_molecules, aleator_gcmc** out);
+/* Advances every chain by n_steps in one launch on the context stream (asynchronous).
+ * trace_device (nullable): int32 [n_chains][n_steps] molecule count after each step. */
+aleator_status aleator_gpu_gcmc_run(aleator_gcmc* gcmc, uint64_t n_steps, int32_t* trace_device);
+aleator_status aleator_gpu_gcmc_stats(aleator_gcmc* gcmc, aleator_gcmc_stats* out /* [n_chains] */);
+aleator_status aleator_gpu_gcmc_reset_stats(aleator_gcmc* gcmc);
+/* Site positions of one chain in molecule order, [n_molecules][n_sites][3]; xyz may be NULL
+ * to query the count. */
+aleator_status aleator_gpu_gcmc_positions(aleator_gcmc* gcmc, int32_t chain, double* xyz,
+                                          uint64_t capacity, int32_t* n_molecules);
+/* committedEnergy of the chain's Ewald state (ewald_incremental_state.cc:85-93). */
+aleator_status aleator_gpu_gcmc_electrostatic_energy(aleator_gcmc* gcmc, int32_t chain, double* energy);
+aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* gcmc);
+
+/* ---- host utilities of the path (no device work) ------------------------ */
+/* minimumSupercellReplication (core/geometry/supercell.cc:7-20): smallest n_i >= 1 with
+ * n_i * L_perp_i > 2 * cutoff. */
+aleator_status aleator_supercell_replication(const aleator_lattice* lattice, double cutoff, int32_t n[3]);
+/* replicateSupercell (core/geometry/supercell.cc:22-60): images in i, j, k order, each a copy
+ * of `in` translated by i*a0 + j*a1 + k*a2; output arrays hold in->n*nx*ny*nz entries. */
+aleator_status aleator_replicate_supercell(const aleator_particles* in, const aleator_lattice* lattice,
+                                           int32_t nx, int32_t ny, int32_t nz, double* x, double* y,
+                                           double* z, double* charge, uint32_t* species,
+                                           aleator_lattice* out_lattice);
+/* PengRobinson::fugacityPascal (engines/monte_carlo/peng_robinson.cc:10-110). */
+aleator_status aleator_peng_robinson_fugacity(double critical_temperature, double critical_pressure_pa,
+                                              double acentric_factor, double temperature,
+                                              double pressure_pa, double* fugacity_pa);
+/* LennardJones::tailEnergyCorrection (pairwise/lennard_jones.cc:218-257) of the framework plus
+ * one inserted molecule minus that of the framework alone: the constant analytic tail term of
+ * one insertion.  The reference's Widom loop does not add it; a caller that wants it multiplies
+ * K_H by exp(-delta/T). */
+aleator_status aleator_lj_tail_insertion_delta(const aleator_lj_params* lj, const aleator_lattice* lattice,
+                                               const uint32_t* framework_species, uint64_t n_framework,
+                                               const aleator_molecule_site* sites, int32_t n_sites,
+                                               double* delta);
+
 /* ---- synthetic frameworks (no reference counterpart) ------------------ */
 /* Uniform fractional placement from Philox4x32Rng(seed, stream), rejecting a candidate
  * closer than min_dist (minimum image) to an earlier atom; species assigned in index order

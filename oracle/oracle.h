@@ -179,6 +179,34 @@ double orc_widom(const orc_lj* lj, const orc_particles* fw, const orc_lattice* l
                  double rc_real, uint64_t seed, uint64_t stream, uint64_t first_call,
                  uint64_t n_trials, double temperature, double* du_out);
 
+/* ---- GCMC move loop (monte_carlo_engine.cc:380-439, 441-671, 814-848) --- */
+typedef struct {
+    orc_lj lj;
+    double *eps, *sig;
+    orc_lattice lat;
+    size_t n, cap, n_fw;
+    double *x, *y, *z, *q;
+    uint32_t* sp;
+    size_t n_mol, mol_cap, ns;
+    size_t* mol; /* [n_mol][ns] particle indices, molecule order as the reference's vector */
+    orc_mol_site* sites;
+    double fugacity_internal, temperature, max_translation, max_rotation;
+    int charged;
+    double alpha, rc_real;
+    orc_ewald_state ew;
+    size_t moves_since_resync;
+    orc_rng rng;
+    uint64_t attempted[4], accepted[4]; /* translate, rotate, insert, delete */
+} orc_gcmc;
+
+int orc_gcmc_init(orc_gcmc* g, const orc_lj* lj, const orc_particles* fw, const orc_lattice* lat,
+                  const orc_mol_site* sites, size_t n_sites, double fugacity_pa, double temperature,
+                  double max_translation, double max_rotation, double alpha, double rc_real, int kmax,
+                  int charged, uint64_t seed, uint64_t stream);
+void orc_gcmc_run(orc_gcmc* g, uint64_t steps, double* n_trace);
+double orc_gcmc_electrostatic_energy(const orc_gcmc* g);
+void orc_gcmc_free(orc_gcmc* g);
+
 /* ---- Synthetic frameworks (no reference counterpart) ----------------- */
 /* Uniform fractional placement from Philox(seed, stream), rejecting candidates
  * closer than min_dist to an earlier atom; types in index order by proportion;

@@ -381,6 +381,43 @@ int ref_coulomb_nodes(double alpha, double rc_real, int kmax, const double* m, u
     });
 }
 
+// Runs the reference engine's single-species GCMC for `steps` steps and copies out the
+// final particle arrays (framework + guests), molecule count, and committed Ewald energy.
+int ref_gcmc_run(uint32_t ns, const double* eps, const double* sig, double rc, int trunc, int mixing,
+                 const double* m, uint64_t n, const double* x, const double* y, const double* z,
+                 const double* q, const uint32_t* species, int n_sites, const double* site_xyz,
+                 const uint32_t* site_species, const double* site_q, double fugacity_pa,
+                 double temperature, double max_translation, double max_rotation, int charged,
+                 double alpha, double rc_real, int kmax, uint64_t seed, uint64_t stream,
+                 uint64_t steps, uint64_t cap, double* ox, double* oy, double* oz, double* oq,
+                 uint64_t* n_particles, uint64_t* n_molecules, double* elec_energy) {
+    return guarded([&] {
+        const auto lj = makeLj(ns, eps, sig, rc, trunc, mixing);
+        auto rng = std::make_unique<core::Philox4x32Rng>();
+        rng->seed(seed, stream);
+        engines::MoleculeSpecies sp;
+        for (int s = 0; s < n_sites; ++s)
+            sp.sites.push_back({site_xyz[3 * s], site_xyz[3 * s + 1], site_xyz[3 * s + 2],
+                                site_species[s], site_q[s], 1.0});
+        std::shared_ptr<const forcefield::Ewald> ew;
+        if (charged) ew = std::make_shared<forcefield::Ewald>(alpha, rc_real, kmax);
+        engines::MonteCarloEngine eng(makeParticles(n, x, y, z, q, species), makeLattice(m), lj,
+                                      std::move(rng), temperature, sp, fugacity_pa,
+                                      max_translation, max_rotation, ew);
+        eng.run(steps);
+        const auto& p = eng.particles();
+        *n_particles = p.size();
+        *n_molecules = eng.moleculeCount();
+        *elec_energy = eng.electrostaticEnergyForTesting();
+        for (std::size_t i = 0; i < p.size() && i < cap; ++i) {
+            ox[i] = p.x[i];
+            oy[i] = p.y[i];
+            oz[i] = p.z[i];
+            oq[i] = p.charge[i];
+        }
+    });
+}
+
 int ref_pr_fugacity(double tc, double pc, double omega, double t, double p, double* out) {
     return guarded([&] {
         engines::PengRobinson pr({tc, pc, omega});

@@ -386,3 +386,97 @@ def c1_system(n_atoms=600):
     eps = np.array([t[0] for t in C0_TYPES] + [CO2_C[0], CO2_O[0]])
     sig = np.array([t[1] for t in C0_TYPES] + [CO2_C[1], CO2_O[1]])
     return fw, LJ(eps, sig, 12.0, TRUNCATED)
+
+
+# --------------------------------------------------------------------------- GCMC move loop
+class OrcRng(C.Structure):
+    _fields_ = [("key", C.c_uint32 * 2), ("ctr", C.c_uint32 * 4), ("call_index", C.c_uint64)]
+
+
+class OrcGcmc(C.Structure):
+    _fields_ = [("lj", OrcLJ), ("eps", dp), ("sig", dp), ("lat", OrcLattice), ("n", C.c_size_t),
+                ("cap", C.c_size_t), ("n_fw", C.c_size_t), ("x", dp), ("y", dp), ("z", dp), ("q", dp),
+                ("sp", u32p), ("n_mol", C.c_size_t), ("mol_cap", C.c_size_t), ("ns", C.c_size_t),
+                ("mol", szp), ("sites", C.c_void_p), ("fugacity_internal", C.c_double),
+                ("temperature", C.c_double), ("max_translation", C.c_double), ("max_rotation", C.c_double),
+                ("charged", C.c_int), ("alpha", C.c_double), ("rc_real", C.c_double), ("ew", OrcEwaldState),
+                ("moves_since_resync", C.c_size_t), ("rng", OrcRng),
+                ("attempted", C.c_uint64 * 4), ("accepted", C.c_uint64 * 4)]
+
+
+class Gcmc:
+    """Oracle GCMC chain (restates monte_carlo_engine.cc run/attempt*)."""
+
+    def __init__(self, lj: LJ, fw: Framework, sites, fugacity_pa, temperature, seed, stream, ewald=None,
+                 max_translation=1.5, max_rotation=0.5):
+        self.g = OrcGcmc()
+        l, self._k1 = lj_struct(lj)
+        p, self._k2 = particles(fw)
+        arr = (OrcMolSite * len(sites))()
+        for i, s in enumerate(sites):
+            arr[i] = OrcMolSite(s[0], s[1], s[2], int(s[3]), s[4])
+        alpha, rc_real, kmax = ewald if ewald is not None else (1.0, 1.0, 1)
+        rc = oracle().orc_gcmc_init(C.byref(self.g), C.byref(l), C.byref(p), C.byref(lattice(fw.m)), arr,
+                                    C.c_size_t(len(sites)), C.c_double(fugacity_pa), C.c_double(temperature),
+                                    C.c_double(max_translation), C.c_double(max_rotation), C.c_double(alpha),
+                                    C.c_double(rc_real), C.c_int(kmax), C.c_int(1 if ewald is not None else 0),
+                                    C.c_uint64(seed), C.c_uint64(stream))
+        if rc != 0:
+            raise RuntimeError("gcmc init failed")
+        oracle().orc_gcmc_electrostatic_energy.restype = C.c_double
+
+    def __del__(self):
+        try:
+            oracle().orc_gcmc_free(C.byref(self.g))
+        except Exception:
+            pass
+
+    def run(self, steps, trace=False):
+        tr = np.zeros(steps) if trace else None
+        oracle().orc_gcmc_run(C.byref(self.g), C.c_uint64(steps), ptr(tr) if trace else None)
+        return tr
+
+    @property
+    def n_molecules(self):
+        return self.g.n_mol
+
+    def particles(self):
+        n = self.g.n
+        return tuple(np.ctypeslib.as_array(a, (n,)).copy() for a in (self.g.x, self.g.y, self.g.z, self.g.q))
+
+    def molecule_sites(self):
+        n = self.g.n_mol * self.g.ns
+        return np.ctypeslib.as_array(self.g.mol, (n,)).copy().reshape(self.g.n_mol, self.g.ns) if n else \
+            np.zeros((0, self.g.ns), dtype=np.uint64)
+
+    def electrostatic_energy(self):
+        return oracle().orc_gcmc_electrostatic_energy(C.byref(self.g))
+
+    @property
+    def counts(self):
+        return list(self.g.attempted), list(self.g.accepted)
+
+
+def ref_gcmc_run(lj: LJ, fw: Framework, sites, fugacity_pa, temperature, seed, stream, steps, ewald=None,
+                 max_translation=1.5, max_rotation=0.5):
+    """The reference engine (oracle/_ref) run for `steps`: (molecule count, (x, y, z, q) of all
+    particles, committed Ewald energy)."""
+    sx = np.array([s[:3] for s in sites], dtype=np.float64).ravel()
+    ss = np.array([s[3] for s in sites], dtype=np.uint32)
+    sq = np.array([s[4] for s in sites], dtype=np.float64)
+    cap = fw.n + 3 * len(sites) * 2000
+    out = [np.zeros(cap) for _ in range(4)]
+    npart, nmol, e = C.c_uint64(), C.c_uint64(), C.c_double()
+    alpha, rc_real, kmax = ewald if ewald is not None else (1.0, 1.0, 1)
+    rc = ref().ref_gcmc_run(C.c_uint32(len(lj.eps)), ptr(lj.eps), ptr(lj.sigma), C.c_double(lj.cutoff),
+                            lj.truncation, lj.mixing, ptr(np.ascontiguousarray(fw.m)), C.c_uint64(fw.n), ptr(fw.x),
+                            ptr(fw.y), ptr(fw.z), ptr(fw.q), ptr(fw.species, u32p), len(sites), ptr(sx),
+                            ptr(ss, u32p), ptr(sq), C.c_double(fugacity_pa), C.c_double(temperature),
+                            C.c_double(max_translation), C.c_double(max_rotation), 1 if ewald is not None else 0,
+                            C.c_double(alpha), C.c_double(rc_real), kmax, C.c_uint64(seed), C.c_uint64(stream),
+                            C.c_uint64(steps), C.c_uint64(cap), ptr(out[0]), ptr(out[1]), ptr(out[2]), ptr(out[3]),
+                            C.byref(npart), C.byref(nmol), C.byref(e))
+    if rc != 0:
+        raise RuntimeError("reference engine failed")
+    n = npart.value
+    return nmol.value, tuple(a[:n] for a in out), e.value

@@ -52,6 +52,41 @@ def test_synthetic_framework_equals_oracle_generator(cell, n, sigma):
         assert np.array_equal(a, b)
 
 
+def test_supercell_utilities_equal_oracle():
+    prim, _ = ol.c1_system()
+    reps = aa.supercell_replication(prim.m, 12.0)
+    assert reps == ol.min_supercell(prim.m, 12.0) == (2, 2, 2)
+    m, x, y, z, q, sp = aa.replicate_supercell(prim.m, prim.x, prim.y, prim.z, prim.q, prim.species, reps)
+    ref = ol.replicate(prim, reps)
+    for a, b in zip((m, x, y, z, q, sp), (ref.m, ref.x, ref.y, ref.z, ref.q, ref.species)):
+        assert np.array_equal(a, b)
+    with pytest.raises(ValueError):
+        aa.supercell_replication(prim.m, -1.0)
+
+
+def test_peng_robinson_equals_oracle():
+    for bar in (0.1, 0.2259, 1.1525, 5.8799, 30.0):
+        got = aa.peng_robinson_fugacity(304.1282, 7377300.0, 0.22394, 298.0, bar * 1e5)
+        ref = ol.oracle().orc_pr_fugacity(C.c_double(304.1282), C.c_double(7377300.0), C.c_double(0.22394),
+                                          C.c_double(298.0), C.c_double(bar * 1e5))
+        assert got == ref
+
+
+def test_tail_insertion_delta_equals_oracle_difference():
+    fw, lj = ol.c0_system()
+    site = [(0.0, 0.0, 0.0, 4, 0.0)]
+    got = aa.lj_tail_insertion_delta(lj.eps, lj.sigma, lj.cutoff, fw.m, fw.species, site)
+    l, keep = ol.lj_struct(lj)
+    lat = ol.lattice(fw.m)
+    p0, k0 = ol.particles(fw)
+    with_guest = ol.Framework(fw.m, np.append(fw.x, 0.0), np.append(fw.y, 0.0), np.append(fw.z, 0.0),
+                              np.append(fw.q, 0.0), np.append(fw.species, np.uint32(4)))
+    p1, k1 = ol.particles(with_guest)
+    ref = ol.oracle().orc_lj_tail_energy(C.byref(l), C.byref(p1), C.byref(lat)) - \
+        ol.oracle().orc_lj_tail_energy(C.byref(l), C.byref(p0), C.byref(lat))
+    assert got == ref and got < 0.0
+
+
 def test_device_calls_without_a_context_are_rejected():
     lib = C.CDLL(aa.library_path())
     out = C.c_void_p()

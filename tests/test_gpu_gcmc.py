@@ -1,0 +1,78 @@
+"""Parity of the batched device GCMC chains with the oracle's restatement of the reference
+engine (MonteCarloEngine::run, monte_carlo_engine.cc:405-671), itself bitwise-equal to the
+compiled reference (test_oracle_vs_reference.py).
+
+With the same (seed, stream) a device chain takes the same draws in the same order, so it follows
+the same trajectory: the molecule count after every step must match exactly, and final
+positions and Ewald energy to 1e-9 relative.  (A flip would need two energies that differ only
+by rounding to straddle an acceptance threshold; these seeds are checked not to.)"""
+import numpy as np
+import pytest
+
+import aleator_amd as aa
+import oracle_lib as ol
+from gpu_helpers import device_framework
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_positions(g: ol.Gcmc):
+    x, y, z, _ = g.particles()
+    sites = g.molecule_sites()
+    return np.stack([np.stack([x[sites[:, k]], y[sites[:, k]], z[sites[:, k]]], axis=1)
+                     for k in range(sites.shape[1])], axis=1) if len(sites) else np.zeros((0, sites.shape[1], 3))
+
+
+def test_gcmc_methane_chains_follow_oracle(ctx):
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff, lj.truncation = 6.5, ol.SHIFTED
+    sites = [(0.0, 0.0, 0.0, 4, 0.0)]
+    dev = device_framework(ctx, fw, lj)
+    chains = [(1e8, 300.0, 5, 0), (3e7, 300.0, 6, 2), (1e8, 250.0, 7, 1)]
+    gc = aa.Gcmc(dev, sites, chains, max_molecules=256)
+    trace = gc.run(3000, trace=True)
+    stats = gc.stats()
+    for c, (f, t, seed, stream) in enumerate(chains):
+        g = ol.Gcmc(lj, fw, sites, f, t, seed, stream)
+        ref_trace = g.run(3000, trace=True)
+        assert np.array_equal(trace[c], ref_trace.astype(np.int32)), f"chain {c} trajectory diverged"
+        assert stats[c]["n_molecules"] == g.n_molecules > 0
+        att, acc = g.counts
+        assert stats[c]["attempted"] == att and stats[c]["accepted"] == acc
+        pos = gc.positions(c)
+        ref = oracle_positions(g)
+        assert np.allclose(pos, ref, rtol=0, atol=1e-9)
+
+
+def test_gcmc_charged_co2_chain_follows_oracle(ctx):
+    m = ol.cell_from_params(16.0, 17.0, 18.0, 80.0, 95.0, 100.0)
+    fw = ol.synth_framework(m, 120, seed=4, charge_sigma=0.5)
+    eps = np.array([t[0] for t in ol.C0_TYPES] + [ol.CO2_C[0], ol.CO2_O[0]])
+    sig = np.array([t[1] for t in ol.C0_TYPES] + [ol.CO2_C[1], ol.CO2_O[1]])
+    lj = ol.LJ(eps, sig, 7.5)
+    dev = device_framework(ctx, fw, lj, ewald=(0.35, 7.5, 5))
+    chains = [(1e6, 298.0, 3, 1), (5e6, 298.0, 4, 2)]
+    gc = aa.Gcmc(dev, ol.co2_sites(), chains, max_molecules=256)
+    trace = gc.run(1500, trace=True)
+    stats = gc.stats()
+    for c, (f, t, seed, stream) in enumerate(chains):
+        g = ol.Gcmc(lj, fw, ol.co2_sites(), f, t, seed, stream, ewald=(0.35, 7.5, 5))
+        ref_trace = g.run(1500, trace=True)
+        assert np.array_equal(trace[c], ref_trace.astype(np.int32)), f"chain {c} trajectory diverged"
+        assert stats[c]["n_molecules"] == g.n_molecules > 0
+        assert np.allclose(gc.positions(c), oracle_positions(g), rtol=0, atol=1e-9)
+        e_dev, e_ref = gc.electrostatic_energy(c), g.electrostatic_energy()
+        assert abs(e_dev - e_ref) <= 1e-9 * abs(e_ref)
+
+
+def test_gcmc_errors(ctx):
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff = 6.5
+    dev = device_framework(ctx, fw, lj)
+    with pytest.raises(ValueError):
+        aa.Gcmc(dev, [(0, 0, 0, 4, 0.0)], [(1e5, -1.0, 1, 0)])
+    with pytest.raises(ValueError):
+        aa.Gcmc(dev, [(0, 0, 0, 4, 0.0)], [(0.0, 300.0, 1, 0)])
+    gc = aa.Gcmc(dev, [(0, 0, 0, 4, 0.0)], [(1e9, 300.0, 1, 0)], max_molecules=2)
+    gc.run(2000)
+    assert gc.stats()[0]["error"] == 1  # capacity exceeded is reported, not silently dropped

@@ -220,6 +220,32 @@ def test_tail_and_pair_energy_bitwise():
     assert out.value == ol.oracle().orc_lj_tail_energy(C.byref(l), C.byref(p), C.byref(ol.lattice(fw.m)))
 
 
+@pytest.mark.parametrize("fugacity", [1e6, 1e7, 1e8])
+def test_gcmc_methane_trajectory_bitwise(fugacity):
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff, lj.truncation = 6.5, ol.SHIFTED
+    sites = [(0.0, 0.0, 0.0, 4, 0.0)]
+    g = ol.Gcmc(lj, fw, sites, fugacity, 300.0, 5, 0)
+    g.run(4000)
+    n_ref, parts, _ = ol.ref_gcmc_run(lj, fw, sites, fugacity, 300.0, 5, 0, 4000)
+    assert g.n_molecules == n_ref
+    assert all(np.array_equal(a, b) for a, b in zip(g.particles(), parts))
+
+
+def test_gcmc_charged_co2_trajectory_bitwise():
+    m = ol.cell_from_params(16.0, 17.0, 18.0, 80.0, 95.0, 100.0)
+    fw = ol.synth_framework(m, 120, seed=4, charge_sigma=0.5)
+    eps = np.array([t[0] for t in ol.C0_TYPES] + [ol.CO2_C[0], ol.CO2_O[0]])
+    sig = np.array([t[1] for t in ol.C0_TYPES] + [ol.CO2_C[1], ol.CO2_O[1]])
+    lj = ol.LJ(eps, sig, 7.5)
+    g = ol.Gcmc(lj, fw, ol.co2_sites(), 1e6, 298.0, 3, 1, ewald=(0.35, 7.5, 5))
+    g.run(1500)
+    n_ref, parts, e_ref = ol.ref_gcmc_run(lj, fw, ol.co2_sites(), 1e6, 298.0, 3, 1, 1500, ewald=(0.35, 7.5, 5))
+    assert g.n_molecules == n_ref and n_ref > 0
+    assert all(np.array_equal(a, b) for a, b in zip(g.particles(), parts))
+    assert g.electrostatic_energy() == e_ref
+
+
 def test_peng_robinson_bitwise():
     out = C.c_double()
     for bar in (0.1, 1.1525, 30.0):
