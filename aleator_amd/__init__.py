@@ -309,6 +309,81 @@ class Framework:
         return (s / n_trials) / temperature
 
 
+class GridPlan:
+    """Reusable (batched, optionally graph-captured, optionally autotuned) grid build over device
+    outputs.  outputs: device pointers, one per framework, each sized (n_guest + coulomb) * nodes
+    doubles."""
+
+    def __init__(self, ctx: Context, frameworks, guests, n_points, outputs, coulomb=False, precision=FP64,
+                 cap=1.0e5, grid_lattices=None, tile=-1, use_graph=False):
+        n = len(frameworks)
+        fws = (C.c_void_p * n)(*[f.handle.value for f in frameworks])
+        g = np.ascontiguousarray(guests, dtype=np.uint32)
+        pts = np.ascontiguousarray(n_points, dtype=np.int32).reshape(n, 3)
+        outs = (C.c_void_p * n)(*[int(o) for o in outputs])
+        lats = None
+        if grid_lattices is not None:
+            lats = (_Lattice * n)(*[_lattice(m) for m in grid_lattices])
+        self._keep = (frameworks, g, pts, outs, lats)
+        self._h = C.c_void_p()
+        _check(_load().aleator_gpu_grid_plan_create(ctx.handle, C.c_int32(n), fws, lats,
+                                                    _p(g, _u32p) if len(g) else None, C.c_int32(len(g)),
+                                                    C.c_int32(1 if coulomb else 0), _p(pts, _i32p),
+                                                    C.c_int32(precision), C.c_double(cap), outs, C.c_int32(tile),
+                                                    C.c_int32(1 if use_graph else 0), C.byref(self._h)))
+
+    def execute(self):
+        _check(_load().aleator_gpu_grid_plan_execute(self._h))
+
+    def tile(self):
+        t = C.c_int32()
+        ms = (C.c_float * 4)()
+        _check(_load().aleator_gpu_grid_plan_tile(self._h, C.byref(t), ms))
+        return t.value, list(ms)
+
+    def close(self):
+        if self._h:
+            _load().aleator_gpu_grid_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """A plain hipMalloc'd buffer for device-output entry points (no torch needed)."""
+
+    def __init__(self, n_doubles: int):
+        self.nbytes = int(n_doubles) * 8
+        self.ptr = C.c_void_p()
+        if _hip_rt().hipMalloc(C.byref(self.ptr), C.c_size_t(max(self.nbytes, 8))) != 0:
+            raise DeviceError("hipMalloc failed")
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.nbytes // 8)
+        _hip_rt().hipDeviceSynchronize()
+        if _hip_rt().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(self.nbytes), 2) != 0:
+            raise DeviceError("hipMemcpy failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _hip_rt().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def __int__(self):
+        return self.ptr.value
+
+
 class _ChainDesc(C.Structure):
     _fields_ = [("fugacity_pa", C.c_double), ("temperature", C.c_double), ("seed", C.c_uint64),
                 ("stream", C.c_uint64)]

@@ -1,0 +1,67 @@
+// grid.hpp -- descriptors and jobs of the grid build (see grid.hip).
+#pragma once
+
+#include <vector>
+
+#include "internal.hpp"
+
+namespace aleator_gpu {
+
+constexpr int kNumTileShapes = 4;  // 8x8x4, 8x8x8 (2/thread), 8x8x16 (4/thread), 4x8x16 (2/thread)
+
+// Everything one framework's tile kernel reads; stored in device memory, one per framework.
+struct GridArgs {
+    int n[3];
+    double mg[9];   // grid lattice rows
+    double mi[9];   // image (framework) lattice rows
+    double rep[3];  // image fractional = grid fractional / rep
+    const double* fx;
+    const double* fy;
+    const double* fz;
+    const double* q;
+    const int32_t* type;
+    int natoms;
+    double search2;
+    double rc_lj, rc2_lj;
+    double rc2_real, alpha, kc;
+    double cap;
+    int trunc;
+    const double* ljtab;  // [4][kMaxGuests][kMaxTypes]: sigma^2, 4 eps, V(rc), V'(rc)
+    const double2* t2;    // [2K+1][nz][ny]
+    const double2* e0;    // [2K+1][nx]
+    int kmax;
+    double recip_pref;
+    double* out;
+    long long stride;
+};
+
+// One framework's build: host-side constants, their device placement, pre-kernels.
+struct GridJob {
+    GridArgs args{};
+    int ng = 0;
+    bool coulomb = false;
+    int precision = 0;
+    int blocks = 0;
+    int shape = 0;
+    int nk = 0;
+    double rmax = 0.0, lperp[3] = {0, 0, 0};
+    std::vector<double> ljtab;
+    std::vector<double2> cube, e0, e1, e2;
+    size_t bytes = 0, off_c = 0, off_e0 = 0, off_e1 = 0, off_e2 = 0, off_t1 = 0, off_t2 = 0;
+    char* dev_base = nullptr;
+
+    void prepare(Framework& fw, const GridRequest& req);
+    double tile_radius(int shape) const;
+    bool shape_fits(int shape) const;
+    void set_shape(int s);
+    int shape_used() const { return shape; }
+    void bind(char* base);                 // device addresses of this job's `bytes`
+    void upload(hipStream_t s) const;      // host constants -> device
+    void launch_pre(hipStream_t s) const;  // reciprocal contractions (Coulomb only)
+};
+
+int tile_blocks(const int32_t n[3], int shape);
+void launch_tiles(int precision, int shape, int ng, bool coulomb, const GridArgs* d_descs, const int* d_prefix,
+                  int n_desc, int blocks, hipStream_t s);
+
+}  // namespace aleator_gpu

@@ -92,6 +92,8 @@ struct Context {
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     void* ensure_scratch(size_t bytes);
+    // grid tile shape of the per-call grid entry points (set by a grid plan's autotuning)
+    int tile_shape = 0;
     // optional HIP-event timing of each call's dominant kernel, on `stream`
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

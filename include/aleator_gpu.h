@@ -161,6 +161,28 @@ aleator_status aleator_gpu_grid_build_device(aleator_framework* fw,
                                              int32_t coulomb, const int32_t n_points[3],
                                              int32_t precision, double cap, double* out_device);
 
+/* A reusable build of the grids of one or many frameworks (no reference counterpart as one
+ * call; each framework's tables equal aleator_gpu_grid_build_device's).  Creation uploads every
+ * constant once; execute only launches kernels -- one tile launch over all frameworks -- and,
+ * with use_graph, replays them from a captured hipGraph.  tile: a tile-shape index in
+ * [0, 4), or -1 to time every shape that fits on the context stream and keep the fastest (the
+ * choice is also used by later per-call builds on the context).  Per framework f:
+ * grid_lattices[f] (or NULL for all: the framework lattices), n_points[3f..3f+2], and
+ * out_device[f] with (n_guest + coulomb) * nodes doubles. */
+typedef struct aleator_grid_plan aleator_grid_plan;
+aleator_status aleator_gpu_grid_plan_create(aleator_ctx* ctx, int32_t n_frameworks,
+                                            aleator_framework* const* frameworks,
+                                            const aleator_lattice* grid_lattices,
+                                            const uint32_t* guest_species, int32_t n_guest,
+                                            int32_t coulomb, const int32_t* n_points, int32_t precision,
+                                            double cap, double* const* out_device, int32_t tile,
+                                            int32_t use_graph, aleator_grid_plan** out);
+aleator_status aleator_gpu_grid_plan_execute(aleator_grid_plan* plan);
+/* The tile shape in use and, when the plan autotuned, the median time (ms) of each shape
+ * (-1 where a shape was not tried); tuning_ms may be NULL, else holds 4 floats. */
+aleator_status aleator_gpu_grid_plan_tile(const aleator_grid_plan* plan, int32_t* tile, float* tuning_ms);
+aleator_status aleator_gpu_grid_plan_destroy(aleator_grid_plan* plan);
+
 /* ---- Widom insertion -------------------------------------------------- */
 /* Batched Widom test-particle insertion of a rigid molecule, as the reference loop
  * (monte_carlo_engine.cc:880-916): trial t takes uniform draws from

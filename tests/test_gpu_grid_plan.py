@@ -1,0 +1,81 @@
+"""Grid plans: one batched launch over many frameworks, hipGraph replay and tile autotuning must
+give the tables of the per-call build (FrameworkEnergyGrid ctor, framework_energy_grid.cc:193-249).
+A given tile shape is bitwise deterministic; different shapes agree to rounding (1e-12)."""
+import numpy as np
+import pytest
+
+import aleator_amd as aa
+import oracle_lib as ol
+from gpu_helpers import device_framework, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def frameworks(ctx, n=3):
+    out = []
+    rng = np.random.default_rng(3)
+    eps = np.array([t[0] for t in ol.C0_TYPES] + [ol.METHANE[0]])
+    sig = np.array([t[1] for t in ol.C0_TYPES] + [ol.METHANE[1]])
+    for i in range(n):
+        cell = (rng.uniform(10, 16), rng.uniform(10, 16), rng.uniform(10, 16),
+                *(rng.uniform(70, 110, 3) if i % 2 else (90.0, 90.0, 90.0)))
+        m = ol.cell_from_params(*cell)
+        fw = ol.synth_framework(m, int(rng.integers(50, 200)), seed=10 + i, stream=0)
+        reps = ol.min_supercell(m, 6.0)
+        sc = ol.replicate(fw, reps)
+        out.append((fw, sc, ol.LJ(eps, sig, 6.0), ol.grid_counts(m, 0.5)))
+    return out
+
+
+def test_batched_plan_equals_per_call_builds(ctx):
+    systems = frameworks(ctx)
+    devs = [device_framework(ctx, sc, lj) for fw, sc, lj, counts in systems]
+    bufs = [aa.DeviceBuffer(int(np.prod(c))) for _, _, _, c in systems]
+    for shape in range(4):
+        plan = aa.GridPlan(ctx, devs, [4], [c for *_, c in systems], [int(b) for b in bufs], precision=aa.FP64,
+                           grid_lattices=[fw.m for fw, *_ in systems], tile=shape)
+        plan.execute()
+        for (fw, sc, lj, counts), dev, buf in zip(systems, devs, bufs):
+            got = buf.to_host().reshape(counts[::-1])
+            ref = dev.grid([4], counts, precision=aa.FP64, grid_lattice=fw.m)[0]
+            assert rel_err(got, ref).max() <= 1e-12
+        plan.close()
+    # and the oracle on a sample of one framework
+    fw, sc, lj, counts = systems[1]
+    nodes = np.arange(0, int(np.prod(counts)), 97, dtype=np.int64)
+    ref = ol.grid_nodes_lj(lj, sc, fw.m, counts, 4, nodes)
+    assert rel_err(bufs[1].to_host()[nodes], ref).max() <= 1e-10
+
+
+def test_graph_replay_is_bitwise_and_repeatable(ctx):
+    systems = frameworks(ctx, 2)
+    devs = [device_framework(ctx, sc, lj) for fw, sc, lj, counts in systems]
+    bufs = [aa.DeviceBuffer(int(np.prod(c))) for _, _, _, c in systems]
+    eager = aa.GridPlan(ctx, devs, [4], [c for *_, c in systems], [int(b) for b in bufs], precision=aa.MIXED,
+                        grid_lattices=[fw.m for fw, *_ in systems], tile=1)
+    eager.execute()
+    ref = [b.to_host() for b in bufs]
+    graph = aa.GridPlan(ctx, devs, [4], [c for *_, c in systems], [int(b) for b in bufs], precision=aa.MIXED,
+                        grid_lattices=[fw.m for fw, *_ in systems], tile=1, use_graph=True)
+    for replay in range(3):
+        for b in bufs:  # eager work between replays
+            ctx_zero = np.zeros(b.nbytes // 8)
+            aa._hip_rt().hipMemcpy(b.ptr, ctx_zero.ctypes.data_as(aa.C.c_void_p), aa.C.c_size_t(b.nbytes), 1)
+        graph.execute()
+        for b, r in zip(bufs, ref):
+            assert np.array_equal(b.to_host(), r), f"replay {replay}"
+
+
+def test_autotune_picks_a_fitting_shape(ctx):
+    fw, lj = ol.c0_system()
+    dev = device_framework(ctx, fw, lj)
+    counts = ol.grid_counts(fw.m, 0.2)
+    buf = aa.DeviceBuffer(int(np.prod(counts)))
+    plan = aa.GridPlan(ctx, [dev], [4], [counts], [int(buf)], precision=aa.MIXED, tile=-1)
+    shape, ms = plan.tile()
+    print(f"autotune c0 MIXED: chose {shape}, medians (ms) {ms}")
+    assert 0 <= shape < 4 and ms[shape] > 0
+    plan.execute()
+    got = buf.to_host()
+    ref = dev.grid([4], counts, precision=aa.MIXED)[0].ravel()
+    assert rel_err(got, ref).max() <= 1e-6
