@@ -115,6 +115,14 @@ def grid_point_counts(lattice, spacing: float) -> tuple[int, int, int]:
     return tuple(int(v) for v in n)
 
 
+def philox_uniforms(seed: int, stream: int, first_call_index: int, n: int) -> np.ndarray:
+    """Philox4x32Rng(seed, stream) uniforms at call indices first .. first+n-1."""
+    out = np.zeros(n)
+    _check(_load().aleator_philox_uniforms(C.c_uint64(seed), C.c_uint64(stream), C.c_uint64(first_call_index),
+                                           C.c_uint64(n), _p(out)))
+    return out
+
+
 def supercell_replication(lattice, cutoff: float) -> tuple[int, int, int]:
     """minimumSupercellReplication (supercell.cc:7-20)."""
     n = np.zeros(3, dtype=np.int32)

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "aleator_gpu.h"
+#include "device.hpp"
 
 #pragma clang fp contract(off)
 
@@ -61,6 +62,13 @@ double tail_energy(const aleator_lj_params* lj, const std::vector<double>& count
 }  // namespace
 
 extern "C" {
+
+aleator_status aleator_philox_uniforms(uint64_t seed, uint64_t stream, uint64_t first_call_index, uint64_t n,
+                                      double* out) {
+    if (n && !out) return ALEATOR_INVALID_ARGUMENT;
+    for (uint64_t i = 0; i < n; ++i) out[i] = aleator_gpu::philox_uniform(seed, stream, first_call_index + i);
+    return ALEATOR_OK;
+}
 
 aleator_status aleator_supercell_replication(const aleator_lattice* lattice, double cutoff, int32_t n[3]) {
     if (!lattice || !n || !(cutoff > 0.0)) return ALEATOR_INVALID_ARGUMENT;

@@ -163,8 +163,12 @@ def main():
     nx, ny, nz = w["counts"]
     nodes = nx * ny * nz
     out = torch.empty(3 * nodes, dtype=torch.float64, device="cuda")
-    step = lambda: fw.grid_device([4, 5], w["counts"], out.data_ptr(), coulomb=True, precision=prec,
-                                  grid_lattice=w["prim"])
+    # the build is planned once (constants uploaded, tile shape autotuned, launches captured in a
+    # hipGraph); a step replays it: device-resident input to device-resident tables
+    plan = aa.GridPlan(ctx, [fw], [4, 5], [w["counts"]], [out.data_ptr()], coulomb=True, precision=prec,
+                       grid_lattices=[w["prim"]], tile=-1, use_graph=True)
+    tile, tuning = plan.tile()
+    step = plan.execute
     for _ in range(args.warmup):
         step()
     sync()
@@ -179,7 +183,11 @@ def main():
     value = 3 * nodes / t_step
 
     # ---------------- extras
-    extra = {"nodes_per_s": nodes / t_step, "tile_kernel_ms": k_avg * 1e3, "pairs_per_node": nbar}
+    extra = {"nodes_per_s": nodes / t_step, "tile_kernel_ms": k_avg * 1e3, "pairs_per_node": nbar,
+             "tile_shape": tile, "autotune_ms_per_shape": tuning}
+    t = timed(lambda: fw.grid_device([4, 5], w["counts"], out.data_ptr(), coulomb=True, precision=prec,
+                                     grid_lattice=w["prim"]), 5, 1, sync)
+    extra["c1_grid_per_call_entry_point_pts_per_s"] = 3 * nodes / t
     c0 = c0_workload(aa)
     fw0 = aa.Framework(ctx, c0["m"], c0["x"], c0["y"], c0["z"], c0["sp"], c0["eps"], c0["sig"], c0["rc"])
     n0 = int(np.prod(c0["counts"]))
@@ -200,6 +208,51 @@ def main():
     t = timed(lambda: fw.widom_device(co2, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
                                       precision=prec), 3, 1, sync)
     extra[f"c1_widom_co2_{n_trials}_{args.precision}_inserts_per_s"] = n_trials / t
+
+    from aleator_amd import workloads as wl
+
+    # c2: 256 frameworks, one batched (graph-captured, autotuned) launch per step, FP32 pair terms
+    try:
+        fws2, eps2, sig2 = wl.c2(256)
+        devs = [aa.Framework(ctx, f["sup"], f["x"], f["y"], f["z"], f["sp"], eps2, sig2, f["rc"]) for f in fws2]
+        counts2 = [f["counts"] for f in fws2]
+        nodes2 = [int(np.prod(c)) for c in counts2]
+        buf2 = torch.empty(sum(nodes2), dtype=torch.float64, device="cuda")
+        offs = np.concatenate([[0], np.cumsum(nodes2)[:-1]])
+        ptrs = [buf2.data_ptr() + 8 * int(o) for o in offs]
+        plan2 = aa.GridPlan(ctx, devs, [4], counts2, ptrs, precision=aa.MIXED,
+                            grid_lattices=[f["prim"] for f in fws2], tile=-1, use_graph=True)
+        t = timed(plan2.execute, 3, 1, sync)
+        extra["c2_batched_256_frameworks_pts_per_s"] = sum(nodes2) / t
+        extra["c2_nodes"] = sum(nodes2)
+        extra["c2_ms_per_batch"] = t * 1e3
+        extra["c2_tile_shape"] = plan2.tile()[0]
+        plan2.close()
+        del buf2, devs
+    except Exception as e:  # report, do not hide the headline
+        extra["c2_error"] = repr(e)
+
+    # c3: 64 CO2 GCMC chains on the charged framework, one persistent launch
+    try:
+        chains = wl.c3_chains()
+        gc = aa.Gcmc(fw, wl.CO2_SITES, chains, max_molecules=1024)
+        gc.run(500)
+        sync()
+        gc.reset_stats()
+        n_steps = 2000
+        t0 = time.perf_counter()
+        gc.run(n_steps)
+        sync()
+        dt = time.perf_counter() - t0
+        st = gc.stats()
+        extra["c3_chain_steps_per_s"] = len(chains) * n_steps / dt
+        extra["c3_gpu_seconds_per_1000_steps_all_chains"] = dt / n_steps * 1000
+        extra["c3_mean_N_by_pressure"] = [float(np.mean([st[8 * i + s]["sum_molecules"] / st[8 * i + s]["steps"]
+                                                         for s in range(8)])) for i in range(8)]
+        extra["c3_errors"] = sum(s["error"] for s in st)
+        gc.close()
+    except Exception as e:
+        extra["c3_error"] = repr(e)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -285,6 +285,10 @@ aleator_status aleator_gpu_gcmc_electrostatic_energy(aleator_gcmc* gcmc, int32_t
 aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* gcmc);
 
 /* ---- host utilities of the path (no device work) ------------------------ */
+/* Philox4x32Rng(seed, stream).nextUniform() at call indices first .. first+n-1
+ * (core/math/counter_based_rng.cc:56-79), on the host; the device kernels draw the same values. */
+aleator_status aleator_philox_uniforms(uint64_t seed, uint64_t stream, uint64_t first_call_index,
+                                      uint64_t n, double* out);
 /* minimumSupercellReplication (core/geometry/supercell.cc:7-20): smallest n_i >= 1 with
  * n_i * L_perp_i > 2 * cutoff. */
 aleator_status aleator_supercell_replication(const aleator_lattice* lattice, double cutoff, int32_t n[3]);
