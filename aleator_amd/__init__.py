@@ -370,12 +370,21 @@ class DeviceBuffer:
         if _hip_rt().hipMalloc(C.byref(self.ptr), C.c_size_t(max(self.nbytes, 8))) != 0:
             raise DeviceError("hipMalloc failed")
 
-    def to_host(self) -> np.ndarray:
+    @classmethod
+    def from_array(cls, a) -> "DeviceBuffer":
+        """A device copy of a host array (its bytes, any dtype)."""
+        a = np.ascontiguousarray(a)
+        buf = cls((a.nbytes + 7) // 8)
+        if a.nbytes and _hip_rt().hipMemcpy(buf.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) != 0:
+            raise DeviceError("hipMemcpy failed")
+        return buf
+
+    def to_host(self, dtype=np.float64) -> np.ndarray:
         out = np.empty(self.nbytes // 8)
         _hip_rt().hipDeviceSynchronize()
         if _hip_rt().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(self.nbytes), 2) != 0:
             raise DeviceError("hipMemcpy failed")
-        return out
+        return out.view(dtype)
 
     def free(self):
         if self.ptr:
@@ -470,6 +479,17 @@ class Gcmc:
     def reset_stats(self):
         _check(_load().aleator_gpu_gcmc_reset_stats(self._h))
 
+    def set_proposal(self, weights, nb, reps, uniform_mix=0.05):
+        """Biased insertion from bin weights [nb0][nb1][nb2] over the cell divided by reps
+        (aleator_gpu_gcmc_set_proposal); None restores uniform insertion."""
+        nb_a = (C.c_int32 * 3)(*[int(v) for v in nb])
+        reps_a = (C.c_int32 * 3)(*[int(v) for v in reps])
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64).ravel()
+        if w is not None and w.size != int(np.prod(nb)):
+            raise ValueError("weights must hold nb[0]*nb[1]*nb[2] entries")
+        _check(_load().aleator_gpu_gcmc_set_proposal(self._h, None if w is None else _p(w), nb_a, reps_a,
+                                                     C.c_double(uniform_mix)))
+
     def positions(self, chain: int) -> np.ndarray:
         n = C.c_int32()
         _check(_load().aleator_gpu_gcmc_positions(self._h, C.c_int32(chain), None, C.c_uint64(0), C.byref(n)))
@@ -482,6 +502,95 @@ class Gcmc:
         out = C.c_double()
         _check(_load().aleator_gpu_gcmc_electrostatic_energy(self._h, C.c_int32(chain), C.byref(out)))
         return out.value
+
+
+class Mlp:
+    """Fourier-feature MLP (64 -> 4 x 256 SiLU -> 1) trained and evaluated on the device with
+    BF16 MFMA GEMMs (aleator_gpu_mlp_*).  Row inputs are fractional positions, double [n][3]."""
+
+    def __init__(self, ctx: Context, max_rows: int = 4096, seed: int | None = 0):
+        self.ctx = ctx
+        self.max_rows = int(max_rows)
+        self._h = C.c_void_p()
+        _check(_load().aleator_gpu_mlp_create(ctx.handle, C.c_int32(self.max_rows), C.byref(self._h)))
+        if seed is not None:
+            self.init(seed)
+
+    @staticmethod
+    def num_params() -> int:
+        f = _load().aleator_gpu_mlp_num_params
+        f.restype = C.c_int64
+        return int(f())
+
+    @staticmethod
+    def frequencies() -> np.ndarray:
+        out = np.zeros(96, dtype=np.int8)
+        _check(_load().aleator_gpu_mlp_frequencies(out.ctypes.data_as(C.c_void_p)))
+        return out.reshape(32, 3)
+
+    def init(self, seed: int):
+        _check(_load().aleator_gpu_mlp_init(self._h, C.c_uint64(seed)))
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        _check(_load().aleator_gpu_mlp_get_params(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.num_params():
+            raise ValueError("wrong parameter count")
+        _check(_load().aleator_gpu_mlp_set_params(self._h, p.ctypes.data_as(C.c_void_p)))
+
+    def grads(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        _check(_load().aleator_gpu_mlp_get_grads(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def predict_device(self, frac_ptr: int, n: int, out_ptr: int):
+        _check(_load().aleator_gpu_mlp_predict(self._h, C.c_void_p(int(frac_ptr)), C.c_int64(n),
+                                               C.c_void_p(int(out_ptr))))
+
+    def predict(self, frac) -> np.ndarray:
+        frac = np.ascontiguousarray(frac, dtype=np.float64).reshape(-1, 3)
+        d_in = DeviceBuffer.from_array(frac)
+        d_out = DeviceBuffer((len(frac) + 1) // 2)
+        self.predict_device(int(d_in), len(frac), int(d_out))
+        return d_out.to_host(np.float32)[:len(frac)].copy()
+
+    def loss_grad_device(self, frac_ptr: int, target_ptr: int, n: int) -> float:
+        loss = C.c_float()
+        _check(_load().aleator_gpu_mlp_loss_grad(self._h, C.c_void_p(int(frac_ptr)), C.c_void_p(int(target_ptr)),
+                                                 C.c_int32(n), C.byref(loss)))
+        return loss.value
+
+    def loss_grad(self, frac, target) -> float:
+        frac = np.ascontiguousarray(frac, dtype=np.float64).reshape(-1, 3)
+        d_f = DeviceBuffer.from_array(frac)
+        d_t = DeviceBuffer.from_array(np.ascontiguousarray(target, dtype=np.float64))
+        return self.loss_grad_device(int(d_f), int(d_t), len(frac))
+
+    def train_device(self, frac_ptr: int, target_ptr: int, n_samples: int, batch: int, n_iters: int, lr: float,
+                     seed: int = 0, stream: int = 0, trace: bool = True):
+        losses = np.zeros(max(n_iters, 1), dtype=np.float32)
+        _check(_load().aleator_gpu_mlp_train(self._h, C.c_void_p(int(frac_ptr)), C.c_void_p(int(target_ptr)),
+                                             C.c_int64(n_samples), C.c_int32(batch), C.c_int32(n_iters),
+                                             C.c_float(lr), C.c_uint64(seed), C.c_uint64(stream),
+                                             losses.ctypes.data_as(C.c_void_p) if trace else None))
+        return losses[:n_iters] if trace else None
+
+    def close(self):
+        if self._h:
+            _load().aleator_gpu_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _charged_sites(sites):

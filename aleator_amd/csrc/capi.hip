@@ -18,6 +18,9 @@
 #include "aleator_gpu.h"
 #include "internal.hpp"
 
+// Host arithmetic below keeps the reference's operation order: no fused multiply-adds.
+#pragma clang fp contract(off)
+
 struct aleator_ctx : aleator_gpu::Context {};
 struct aleator_framework : aleator_gpu::Framework {};
 struct aleator_ewald_state : aleator_gpu::EwaldState {};

@@ -73,13 +73,38 @@ __device__ inline void block_sum4(double v[4], double* lds /* [4][4] */) {
     __syncthreads();
 }
 
-// Trial positions of one move, in the reference's arithmetic order (no contraction) and draw
-// order: translation (:454-461), rotation about the centroid (:512-532, Rodrigues :40-44),
-// insertion (:568-590, Shoemake quaternion :56-72).  Deletion draws nothing here.
-__device__ void trial_positions(const GcmcArgs& a, const ChainParams& P, uint64_t& call, int move,
-                                const double old_c[kMaxSites][3], double new_c[kMaxSites][3]) {
+// Bin of the insertion density holding a molecule's anchor (the centroid of its sites):
+// fractional coordinates f = c . inv, folded into one primitive image, then binned.
+__device__ int anchor_bin(const GcmcArgs& a, const double c[kMaxSites][3]) {
 #pragma clang fp contract(off)
     const int ns = a.ns;
+    double cen[3] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < ns; ++s)
+        for (int j = 0; j < 3; ++j) cen[j] = cen[j] + c[s][j];
+    const double inv_n = 1.0 / (double)ns;
+    int b[3];
+    for (int j = 0; j < 3; ++j) {
+        const double f = (cen[0] * inv_n) * a.inv[j] + (cen[1] * inv_n) * a.inv[3 + j] + (cen[2] * inv_n) * a.inv[6 + j];
+        double t = f * (double)a.prop_reps[j];
+        t = t - floor(t);
+        const double kf = t * (double)a.prop_nb[j];
+        const int k = kf >= 0.0 ? (int)kf : 0;  // also maps a NaN coordinate into the table
+        b[j] = k < a.prop_nb[j] - 1 ? k : a.prop_nb[j] - 1;
+    }
+    return (b[0] * a.prop_nb[1] + b[1]) * a.prop_nb[2] + b[2];
+}
+
+// Trial positions of one move, in the reference's arithmetic order (no contraction) and draw
+// order: translation (:454-461), rotation about the centroid (:512-532, Rodrigues :40-44),
+// insertion (:568-590, Shoemake quaternion :56-72).  Deletion draws nothing here.  With an
+// insertion density, the insertion anchor comes from it (draws: bin by inverse CDF, image,
+// position in the bin) and `bias` receives V p(anchor) for insertion and deletion.
+__device__ void trial_positions(const GcmcArgs& a, const ChainParams& P, uint64_t& call, int move,
+                                const double old_c[kMaxSites][3], double new_c[kMaxSites][3], double& bias) {
+#pragma clang fp contract(off)
+    const int ns = a.ns;
+    bias = 1.0;
+    if (move == 3 && a.prop_cdf) bias = a.prop_fac[anchor_bin(a, old_c)];
     if (move == 0) {
         double d[3];
         for (int j = 0; j < 3; ++j) d[j] = (2.0 * uni(P, call) - 1.0) * a.max_translation;
@@ -109,9 +134,34 @@ __device__ void trial_positions(const GcmcArgs& a, const ChainParams& P, uint64_
         }
     } else if (move == 2) {
         // attemptInsertion (:568-590)
-        const double u0 = uni(P, call), u1 = uni(P, call), u2 = uni(P, call);
         double ref[3];
-        for (int j = 0; j < 3; ++j) ref[j] = a.m[j] * u0 + a.m[3 + j] * u1 + a.m[6 + j] * u2;
+        if (a.prop_cdf) {
+            const double ub = uni(P, call);
+            int lo = 0, hi = a.prop_nbins - 1;  // smallest bin with ub < cdf[bin]; cdf[last] = 1
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (ub < a.prop_cdf[mid])
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            bias = a.prop_fac[lo];
+            const int bn[3] = {lo / (a.prop_nb[1] * a.prop_nb[2]), (lo / a.prop_nb[2]) % a.prop_nb[1], lo % a.prop_nb[2]};
+            const int n_img = a.prop_reps[0] * a.prop_reps[1] * a.prop_reps[2];
+            int img = (int)(uni(P, call) * (double)n_img);
+            img = img < n_img - 1 ? img : n_img - 1;
+            const int im[3] = {img / (a.prop_reps[1] * a.prop_reps[2]), (img / a.prop_reps[2]) % a.prop_reps[1],
+                               img % a.prop_reps[2]};
+            double f[3];
+            for (int j = 0; j < 3; ++j) {
+                const double u = uni(P, call);
+                f[j] = ((double)im[j] + ((double)bn[j] + u) / (double)a.prop_nb[j]) / (double)a.prop_reps[j];
+            }
+            for (int j = 0; j < 3; ++j) ref[j] = a.m[j] * f[0] + a.m[3 + j] * f[1] + a.m[6 + j] * f[2];
+        } else {
+            const double u0 = uni(P, call), u1 = uni(P, call), u2 = uni(P, call);
+            for (int j = 0; j < 3; ++j) ref[j] = a.m[j] * u0 + a.m[3 + j] * u1 + a.m[6 + j] * u2;
+        }
         if (ns == 1) {
             for (int j = 0; j < 3; ++j) new_c[0][j] = ref[j] + a.local[0][j];
         } else {
@@ -170,6 +220,7 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
         const int move = pick < 0.25 ? 0 : (pick < 0.5 ? 1 : (pick < 0.75 ? 2 : 3));
         bool skip = (move != 2 && n_mol == 0) || (move == 1 && ns < 2);
         double old_c[kMaxSites][3], new_c[kMaxSites][3];
+        double bias = 1.0;  // V p(anchor) of the insertion density (1: uniform insertion)
         int mi = -1;
         if (!skip) {
             if (move != 2) {
@@ -177,7 +228,7 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
                 for (int s = 0; s < ns; ++s)
                     for (int j = 0; j < 3; ++j) old_c[s][j] = pos[((size_t)mi * ns + s) * 3 + j];
             }
-            trial_positions(a, P, call, move, old_c, new_c);
+            trial_positions(a, P, call, move, old_c, new_c, bias);
         }
         if (skip) {
             sum_n += (double)n_mol;
@@ -313,9 +364,11 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
             } else if (move == 2) {
                 du = e4[1] + a.kc * e4[3] + recip_total;
                 ratio = (P.fugacity_internal * volume / T) / (double)(n_mol + 1) * exp(-du / T);
+                if (a.prop_cdf) ratio = ratio / bias;
             } else {
                 du = -(e4[0] + a.kc * e4[2]) + recip_total;
                 ratio = (double)n_mol / (P.fugacity_internal * volume / T) * exp(-du / T);
+                if (a.prop_cdf) ratio = ratio * bias;
             }
             s_accept = uni(P, call) < (ratio < 1.0 ? ratio : 1.0) ? 1 : 0;
         } else {

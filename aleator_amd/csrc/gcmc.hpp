@@ -61,6 +61,10 @@ struct GcmcArgs {
     double* sim;
     uint64_t n_steps;
     int32_t* n_trace;  // optional [chain][n_steps]
+    // insertion density (null: uniform insertion); bins over the cell divided by prop_reps
+    const double* prop_cdf;  // [prop_nbins] cumulative bin probabilities, last = 1
+    const double* prop_fac;  // [prop_nbins] n_bins * (cdf[b] - cdf[b-1]) = V p(r) in bin b
+    int prop_nb[3], prop_reps[3], prop_nbins;
 };
 
 void launch_gcmc(const GcmcArgs& a, int n_chains, hipStream_t s);

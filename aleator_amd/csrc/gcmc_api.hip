@@ -27,6 +27,7 @@ struct aleator_gcmc {
     double* d_sim = nullptr;
     double* d_ljtab = nullptr;
     double* d_ljgg = nullptr;
+    double* d_prop = nullptr;  // [2][n_bins]: cdf, then n_bins * p
 };
 
 namespace aleator_gpu {
@@ -37,6 +38,35 @@ std::vector<ChainState> download_states(aleator_gcmc* g) {
                                      g->fw->ctx->stream));
     ALEATOR_HIP_CHECK(hipStreamSynchronize(g->fw->ctx->stream));
     return st;
+}
+
+// Bin probabilities p = (1 - mix) w / sum(w) + mix / n, their running sum (the CDF, last entry
+// set to 1) and the acceptance factors n (cdf[b] - cdf[b-1]): the probability the inverse-CDF
+// draw actually gives bin b, so the correction matches the sampler exactly.
+void proposal_tables(const double* w, size_t n, double mix, std::vector<double>& cdf, std::vector<double>& fac) {
+#pragma clang fp contract(off)
+    double total = 0.0;
+    for (size_t b = 0; b < n; ++b) {
+        require(std::isfinite(w[b]) && w[b] >= 0.0, ALEATOR_INVALID_ARGUMENT,
+                "gcmc proposal: weights must be finite and non-negative");
+        total += w[b];
+    }
+    require(total > 0.0 && std::isfinite(total), ALEATOR_INVALID_ARGUMENT, "gcmc proposal: weights sum to zero");
+    cdf.resize(n);
+    fac.resize(n);
+    double run = 0.0;
+    for (size_t b = 0; b < n; ++b) {
+        run += (1.0 - mix) * (w[b] / total) + mix / (double)n;
+        cdf[b] = run;
+    }
+    cdf[n - 1] = 1.0;
+    double prev = 0.0;
+    for (size_t b = 0; b < n; ++b) {
+        require(cdf[b] > prev, ALEATOR_INVALID_ARGUMENT,
+                "gcmc proposal: a bin has zero probability (raise uniform_mix or its weight)");
+        fac[b] = (double)n * (cdf[b] - prev);
+        prev = cdf[b];
+    }
 }
 }  // namespace
 }  // namespace aleator_gpu
@@ -185,6 +215,62 @@ aleator_status aleator_gpu_gcmc_run(aleator_gcmc* g, uint64_t n_steps, int32_t* 
     });
 }
 
+aleator_status aleator_gpu_gcmc_set_proposal(aleator_gcmc* g, const double* w, const int32_t nb[3],
+                                             const int32_t reps[3], double uniform_mix) {
+    return guarded([&] {
+        require(g != nullptr, ALEATOR_INVALID_ARGUMENT, "null argument");
+        hipStream_t s = g->fw->ctx->stream;
+        if (!w) {
+            ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+            if (g->d_prop) ALEATOR_HIP_CHECK(hipFree(g->d_prop));
+            g->d_prop = nullptr;
+            g->args.prop_cdf = g->args.prop_fac = nullptr;
+            g->args.prop_nbins = 0;
+            return;
+        }
+        require(nb && reps, ALEATOR_INVALID_ARGUMENT, "null argument");
+        size_t n = 1;
+        for (int i = 0; i < 3; ++i) {
+            require(nb[i] >= 1 && reps[i] >= 1, ALEATOR_INVALID_ARGUMENT, "gcmc proposal: bins and images must be >= 1");
+            n *= (size_t)nb[i];
+        }
+        require(n <= (1u << 26), ALEATOR_INVALID_ARGUMENT, "gcmc proposal: at most 2^26 bins");
+        require((size_t)reps[0] * reps[1] * reps[2] <= (1u << 20), ALEATOR_INVALID_ARGUMENT,
+                "gcmc proposal: at most 2^20 images");
+        require(uniform_mix >= 0.0 && uniform_mix <= 1.0, ALEATOR_INVALID_ARGUMENT,
+                "gcmc proposal: uniform_mix must be in [0, 1]");
+        const GcmcArgs& a = g->args;
+        for (int j = 0; j < 3; ++j) {
+            double c = 0.0;
+            for (int k = 0; k < a.ns; ++k) c += a.local[k][j];
+            require(std::abs(c / a.ns) <= 1e-9, ALEATOR_INVALID_ARGUMENT,
+                    "gcmc proposal: the species' site offsets must be centred (their mean is the anchor)");
+        }
+        std::vector<double> cdf, fac;
+        proposal_tables(w, n, uniform_mix, cdf, fac);
+        double* d = nullptr;
+        ALEATOR_HIP_CHECK(hipMalloc(&d, 2 * n * sizeof(double)));
+        try {
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(d, cdf.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(d + n, fac.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+            ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        if (g->d_prop) ALEATOR_HIP_CHECK(hipFree(g->d_prop));
+        g->d_prop = d;
+        GcmcArgs& m = g->args;
+        m.prop_cdf = d;
+        m.prop_fac = d + n;
+        m.prop_nbins = (int)n;
+        for (int i = 0; i < 3; ++i) {
+            m.prop_nb[i] = nb[i];
+            m.prop_reps[i] = reps[i];
+        }
+    });
+}
+
 aleator_status aleator_gpu_gcmc_stats(aleator_gcmc* g, aleator_gcmc_stats* out) {
     return guarded([&] {
         require(g && out, ALEATOR_INVALID_ARGUMENT, "null argument");
@@ -268,7 +354,7 @@ aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* g) {
         if (!g) return;
         (void)hipStreamSynchronize(g->fw->ctx->stream);
         for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_state, (void*)g->d_pos,
-                        (void*)g->d_sre, (void*)g->d_sim})
+                        (void*)g->d_sre, (void*)g->d_sim, (void*)g->d_prop})
             if (p) (void)hipFree(p);
         delete g;
     });

@@ -283,6 +283,49 @@ aleator_status aleator_gpu_gcmc_positions(aleator_gcmc* gcmc, int32_t chain, dou
 /* committedEnergy of the chain's Ewald state (ewald_incremental_state.cc:85-93). */
 aleator_status aleator_gpu_gcmc_electrostatic_energy(aleator_gcmc* gcmc, int32_t chain, double* energy);
 aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* gcmc);
+/* No reference counterpart (learned proposal): makes every later insertion draw its anchor
+ * point from a piecewise-constant density instead of uniformly, and corrects both insertion
+ * and deletion for it (R_ins x 1/(V p(r)), R_del x V p(r_del), on top of
+ * gcmc_acceptance.hpp:28-51), so detailed balance holds with the exact energies.  The density
+ * lives on nb[0] x nb[1] x nb[2] equal bins of the cell divided by reps[i] along axis i and
+ * repeats over the reps[0]*reps[1]*reps[2] images (a table over the primitive cell of a
+ * supercell): p(bin) = (1 - uniform_mix) * w[bin] / sum(w) + uniform_mix / n_bins, with
+ * w >= 0 (row-major, axis 2 fastest).  Every bin must keep p > 0.  The anchor of a molecule is
+ * the centroid of its sites, so the species' site offsets must be centred (mean 0).
+ * Insertion draw order: bin (1 uniform, inverse CDF), image (1), position in the bin (3),
+ * orientation as before, acceptance.  w = NULL restores uniform insertion. */
+aleator_status aleator_gpu_gcmc_set_proposal(aleator_gcmc* gcmc, const double* w, const int32_t nb[3],
+                                             const int32_t reps[3], double uniform_mix);
+
+/* ---- learned insertion density: Fourier-feature MLP (no reference counterpart) ----
+ * 64 features (sin, cos of 2 pi k.f for 32 integer vectors k, aleator_gpu_mlp_frequencies)
+ * of the fractional position f -> 4 x 256 SiLU -> 1, trained and evaluated on the device with
+ * BF16 MFMA GEMMs accumulating in FP32; FP32 master weights and Adam.  Parameters are one
+ * packed float vector: W1[64][256] b1[256] W2[256][256] b2 W3 b3 W4 b4 w5[256] b5[1]. */
+typedef struct aleator_mlp aleator_mlp;
+int64_t aleator_gpu_mlp_num_params(void);
+/* frequency vectors, int8 [32][3] */
+aleator_status aleator_gpu_mlp_frequencies(int8_t* out);
+/* max_rows: rows per device pass (activation memory ~ 3.1 KB per row) */
+aleator_status aleator_gpu_mlp_create(aleator_ctx* ctx, int32_t max_rows, aleator_mlp** out);
+aleator_status aleator_gpu_mlp_destroy(aleator_mlp* mlp);
+/* He-uniform weights from Philox4x32Rng(seed, 0), zero biases; resets Adam */
+aleator_status aleator_gpu_mlp_init(aleator_mlp* mlp, uint64_t seed);
+aleator_status aleator_gpu_mlp_set_params(aleator_mlp* mlp, const float* params /* host */);
+aleator_status aleator_gpu_mlp_get_params(aleator_mlp* mlp, float* params /* host */);
+/* gradient of the last loss_grad / train iteration, host */
+aleator_status aleator_gpu_mlp_get_grads(aleator_mlp* mlp, float* grads);
+/* out_device[i] = net(frac_device[i]) for n rows of fractional xyz (double [n][3]) */
+aleator_status aleator_gpu_mlp_predict(aleator_mlp* mlp, const double* frac_device, int64_t n, float* out_device);
+/* mean squared error over n <= max_rows rows and its gradient (kept on the device) */
+aleator_status aleator_gpu_mlp_loss_grad(aleator_mlp* mlp, const double* frac_device, const double* target_device,
+                                         int32_t n, float* loss);
+/* n_iters Adam steps (lr) on minibatches of `batch` rows drawn with replacement from the
+ * n_samples rows (row = floor(u n), u = Philox4x32Rng(seed, stream) at call iter*batch + r);
+ * loss_trace (host, nullable) receives each iteration's minibatch loss. */
+aleator_status aleator_gpu_mlp_train(aleator_mlp* mlp, const double* frac_device, const double* target_device,
+                                     int64_t n_samples, int32_t batch, int32_t n_iters, float lr, uint64_t seed,
+                                     uint64_t stream, float* loss_trace);
 
 /* ---- host utilities of the path (no device work) ------------------------ */
 /* Philox4x32Rng(seed, stream).nextUniform() at call indices first .. first+n-1

@@ -96,6 +96,8 @@ void orc_gcmc_free(orc_gcmc* g) {
     free(g->sites);
     free(g->eps);
     free(g->sig);
+    free(g->prop_cdf);
+    free(g->prop_fac);
     if (g->charged) orc_ewald_state_free(&g->ew);
     memset(g, 0, sizeof(*g));
 }
@@ -297,12 +299,130 @@ static void quat_rotate(double u0, double u1, double u2, const double v[3], doub
     out[2] = r.v[2];
 }
 
+/* ---- insertion density (no reference counterpart) ---------------------------------------- */
+
+/* smallest bin with u < cdf[bin] (cdf[last] = 1) */
+static size_t proposal_draw_bin(const orc_gcmc* g, double u) {
+    size_t lo = 0, hi = g->prop_nbins - 1;
+    while (lo < hi) {
+        const size_t mid = (lo + hi) / 2;
+        if (u < g->prop_cdf[mid])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+/* bin of a molecule's anchor: centroid of its sites -> fractional (c . inv, inv = cofactor
+ * rows / volume) -> folded into one primitive image -> binned */
+static size_t proposal_anchor_bin(const orc_gcmc* g, const size_t* sites) {
+    double cen[3] = {0.0, 0.0, 0.0};
+    for (size_t s = 0; s < g->ns; ++s) {
+        cen[0] = cen[0] + g->x[sites[s]];
+        cen[1] = cen[1] + g->y[sites[s]];
+        cen[2] = cen[2] + g->z[sites[s]];
+    }
+    const double inv_n = 1.0 / (double)g->ns;
+    size_t bin = 0;
+    for (int j = 0; j < 3; ++j) {
+        const double f = (cen[0] * inv_n) * g->prop_inv[0][j] + (cen[1] * inv_n) * g->prop_inv[1][j] +
+                         (cen[2] * inv_n) * g->prop_inv[2][j];
+        double t = f * (double)g->prop_reps[j];
+        t = t - floor(t);
+        const double kf = t * (double)g->prop_nb[j];
+        int k = kf >= 0.0 ? (int)kf : 0;
+        if (k > g->prop_nb[j] - 1) k = g->prop_nb[j] - 1;
+        bin = bin * (size_t)g->prop_nb[j] + (size_t)k;
+    }
+    return bin;
+}
+
+int orc_gcmc_set_proposal(orc_gcmc* g, const double* w, const int nb[3], const int reps[3], double mix) {
+    free(g->prop_cdf);
+    free(g->prop_fac);
+    g->prop_cdf = g->prop_fac = NULL;
+    g->prop_nbins = 0;
+    if (!w) return 0;
+    if (!(mix >= 0.0 && mix <= 1.0)) return -1;
+    size_t n = 1;
+    for (int i = 0; i < 3; ++i) {
+        if (nb[i] < 1 || reps[i] < 1) return -1;
+        n *= (size_t)nb[i];
+    }
+    double total = 0.0;
+    for (size_t b = 0; b < n; ++b) {
+        if (!(w[b] >= 0.0) || !isfinite(w[b])) return -1;
+        total += w[b];
+    }
+    if (!(total > 0.0)) return -1;
+    double* cdf = (double*)malloc(n * sizeof(double));
+    double* fac = (double*)malloc(n * sizeof(double));
+    double run = 0.0;
+    for (size_t b = 0; b < n; ++b) {
+        run += (1.0 - mix) * (w[b] / total) + mix / (double)n;
+        cdf[b] = run;
+    }
+    cdf[n - 1] = 1.0;
+    double prev = 0.0;
+    for (size_t b = 0; b < n; ++b) {
+        if (!(cdf[b] > prev)) {
+            free(cdf);
+            free(fac);
+            return -1;
+        }
+        /* the probability the inverse-CDF draw gives bin b */
+        fac[b] = (double)n * (cdf[b] - prev);
+        prev = cdf[b];
+    }
+    g->prop_cdf = cdf;
+    g->prop_fac = fac;
+    g->prop_nbins = n;
+    for (int i = 0; i < 3; ++i) {
+        g->prop_nb[i] = nb[i];
+        g->prop_reps[i] = reps[i];
+    }
+    /* inverse of the row-vector cell: column j = (a_{j+1} x a_{j+2}) / V */
+    const double(*m)[3] = g->lat.m;
+    const double cx = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    const double cy = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    const double cz = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    const double vol = m[0][0] * cx + m[0][1] * cy + m[0][2] * cz;
+    for (int j = 0; j < 3; ++j) {
+        const double* a = m[(j + 1) % 3];
+        const double* b = m[(j + 2) % 3];
+        const double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        for (int r = 0; r < 3; ++r) g->prop_inv[r][j] = c[r] / vol;
+    }
+    return 0;
+}
+
 /* attemptInsertion (monte_carlo_engine.cc:556-619), single species */
 static void attempt_insertion(orc_gcmc* g) {
     const double(*m)[3] = g->lat.m;
-    const double u0 = uni(g), u1 = uni(g), u2 = uni(g);
     double ref[3];
-    for (int j = 0; j < 3; ++j) ref[j] = m[0][j] * u0 + m[1][j] * u1 + m[2][j] * u2;
+    double bias = 1.0;
+    if (g->prop_cdf) {
+        const size_t bin = proposal_draw_bin(g, uni(g));
+        bias = g->prop_fac[bin];
+        const int nb1 = g->prop_nb[1], nb2 = g->prop_nb[2];
+        const int bn[3] = {(int)(bin / (size_t)(nb1 * nb2)), (int)((bin / (size_t)nb2) % (size_t)nb1),
+                           (int)(bin % (size_t)nb2)};
+        const int r1 = g->prop_reps[1], r2 = g->prop_reps[2];
+        const int n_img = g->prop_reps[0] * r1 * r2;
+        int img = (int)(uni(g) * (double)n_img);
+        if (img > n_img - 1) img = n_img - 1;
+        const int im[3] = {img / (r1 * r2), (img / r2) % r1, img % r2};
+        double f[3];
+        for (int j = 0; j < 3; ++j) {
+            const double u = uni(g);
+            f[j] = ((double)im[j] + ((double)bn[j] + u) / (double)g->prop_nb[j]) / (double)g->prop_reps[j];
+        }
+        for (int j = 0; j < 3; ++j) ref[j] = m[0][j] * f[0] + m[1][j] * f[1] + m[2][j] * f[2];
+    } else {
+        const double u0 = uni(g), u1 = uni(g), u2 = uni(g);
+        for (int j = 0; j < 3; ++j) ref[j] = m[0][j] * u0 + m[1][j] * u1 + m[2][j] * u2;
+    }
     double q0 = 0, q1 = 0, q2 = 0;
     if (g->ns > 1) {
         q0 = uni(g);
@@ -327,8 +447,9 @@ static void attempt_insertion(orc_gcmc* g) {
     }
     const double du = disp + real + recip;
     const double volume = fabs(orc_lattice_volume(&g->lat));
-    const double ratio = (g->fugacity_internal * volume / g->temperature) / (double)(g->n_mol + 1) *
-                         exp(-du / g->temperature);
+    double ratio = (g->fugacity_internal * volume / g->temperature) / (double)(g->n_mol + 1) *
+                   exp(-du / g->temperature);
+    if (g->prop_cdf) ratio = ratio / bias;
     g->attempted[2]++;
     if (uni(g) < clamp_prob(ratio)) {
         if (g->charged) orc_ewald_state_commit(&g->ew, NULL, 0, cs, g->ns);
@@ -381,8 +502,9 @@ static void attempt_deletion(orc_gcmc* g) {
     }
     const double du = -(u_int + real) + recip;
     const double volume = fabs(orc_lattice_volume(&g->lat));
-    const double ratio = (double)count_before / (g->fugacity_internal * volume / g->temperature) *
-                         exp(-du / g->temperature);
+    double ratio = (double)count_before / (g->fugacity_internal * volume / g->temperature) *
+                   exp(-du / g->temperature);
+    if (g->prop_cdf) ratio = ratio * g->prop_fac[proposal_anchor_bin(g, sites)];
     g->attempted[3]++;
     if (uni(g) < clamp_prob(ratio)) {
         if (g->charged) orc_ewald_state_commit(&g->ew, cs, g->ns, NULL, 0);

@@ -197,6 +197,11 @@ typedef struct {
     size_t moves_since_resync;
     orc_rng rng;
     uint64_t attempted[4], accepted[4]; /* translate, rotate, insert, delete */
+    /* insertion density (no reference counterpart); prop_cdf NULL = uniform insertion */
+    double *prop_cdf, *prop_fac;
+    int prop_nb[3], prop_reps[3];
+    size_t prop_nbins;
+    double prop_inv[3][3];
 } orc_gcmc;
 
 int orc_gcmc_init(orc_gcmc* g, const orc_lj* lj, const orc_particles* fw, const orc_lattice* lat,
@@ -204,6 +209,11 @@ int orc_gcmc_init(orc_gcmc* g, const orc_lj* lj, const orc_particles* fw, const 
                   double max_translation, double max_rotation, double alpha, double rc_real, int kmax,
                   int charged, uint64_t seed, uint64_t stream);
 void orc_gcmc_run(orc_gcmc* g, uint64_t steps, double* n_trace);
+/* Biased insertion from a piecewise-constant density over nb bins of the cell divided by reps,
+ * repeated over the reps images: p(bin) = (1 - mix) w / sum(w) + mix / n_bins.  Insertion
+ * acceptance is divided by V p(anchor), deletion acceptance multiplied by V p(centroid).
+ * w = NULL restores uniform insertion.  Returns -1 on invalid input. */
+int orc_gcmc_set_proposal(orc_gcmc* g, const double* w, const int nb[3], const int reps[3], double mix);
 double orc_gcmc_electrostatic_energy(const orc_gcmc* g);
 void orc_gcmc_free(orc_gcmc* g);
 
@@ -215,6 +225,19 @@ int orc_synth_framework(const double m[3][3], size_t n_atoms, uint64_t seed, uin
                         double min_dist, const double* proportions, uint32_t n_types,
                         double charge_sigma, uint64_t charge_stream, double* x, double* y,
                         double* z, double* q, uint32_t* species);
+
+/* ---- Learned insertion density network (no reference counterpart) ---- */
+#define ORC_MLP_IN 64
+#define ORC_MLP_FREQS 32
+#define ORC_MLP_HIDDEN 256
+#define ORC_MLP_PARAMS (ORC_MLP_IN * ORC_MLP_HIDDEN + ORC_MLP_HIDDEN + 3 * (ORC_MLP_HIDDEN * ORC_MLP_HIDDEN + ORC_MLP_HIDDEN) + ORC_MLP_HIDDEN + 1)
+void orc_mlp_frequencies(int8_t out[3 * ORC_MLP_FREQS]);
+/* y[r] = network(frac[r]) for n rows of fractional xyz; bf16 != 0 rounds GEMM operands to
+ * bfloat16 where the device does */
+void orc_mlp_forward(const double* params, const double* frac, size_t n, int bf16, double* y);
+/* mean squared error against target and its gradient grad[ORC_MLP_PARAMS] */
+double orc_mlp_loss_grad(const double* params, const double* frac, const double* target, size_t n, int bf16,
+                         double* grad);
 
 /* ---- Peng-Robinson (peng_robinson.cc:10-110) ------------------------- */
 double orc_pr_fugacity(double tc, double pc, double omega, double temperature, double pressure);

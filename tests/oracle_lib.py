@@ -90,6 +90,7 @@ def oracle():
         _oracle.orc_widom.restype = C.c_double
         _oracle.orc_rng_next_uniform.restype = C.c_double
         _oracle.orc_rng_next_u64.restype = C.c_uint64
+        _oracle.orc_mlp_loss_grad.restype = C.c_double
         _oracle.orc_pr_fugacity.restype = C.c_double
     return _oracle
 
@@ -401,7 +402,9 @@ class OrcGcmc(C.Structure):
                 ("temperature", C.c_double), ("max_translation", C.c_double), ("max_rotation", C.c_double),
                 ("charged", C.c_int), ("alpha", C.c_double), ("rc_real", C.c_double), ("ew", OrcEwaldState),
                 ("moves_since_resync", C.c_size_t), ("rng", OrcRng),
-                ("attempted", C.c_uint64 * 4), ("accepted", C.c_uint64 * 4)]
+                ("attempted", C.c_uint64 * 4), ("accepted", C.c_uint64 * 4),
+                ("prop_cdf", dp), ("prop_fac", dp), ("prop_nb", C.c_int * 3), ("prop_reps", C.c_int * 3),
+                ("prop_nbins", C.c_size_t), ("prop_inv", C.c_double * 9)]
 
 
 class Gcmc:
@@ -430,6 +433,16 @@ class Gcmc:
             oracle().orc_gcmc_free(C.byref(self.g))
         except Exception:
             pass
+
+    def set_proposal(self, w, nb, reps, mix):
+        """Biased insertion from bin weights w (nb bins over the cell / reps); None = uniform."""
+        self._w = None if w is None else np.ascontiguousarray(w, dtype=np.float64).ravel()
+        nb_a = (C.c_int * 3)(*[int(v) for v in nb])
+        reps_a = (C.c_int * 3)(*[int(v) for v in reps])
+        rc = oracle().orc_gcmc_set_proposal(C.byref(self.g), None if w is None else ptr(self._w), nb_a, reps_a,
+                                            C.c_double(mix))
+        if rc != 0:
+            raise ValueError("invalid proposal")
 
     def run(self, steps, trace=False):
         tr = np.zeros(steps) if trace else None
@@ -480,3 +493,55 @@ def ref_gcmc_run(lj: LJ, fw: Framework, sites, fugacity_pa, temperature, seed, s
         raise RuntimeError("reference engine failed")
     n = npart.value
     return nmol.value, tuple(a[:n] for a in out), e.value
+
+
+# ---------------------------------------------------------------- learned insertion density network
+MLP_PARAMS = 64 * 256 + 256 + 3 * (256 * 256 + 256) + 256 + 1
+
+
+def mlp_frequencies() -> np.ndarray:
+    out = np.zeros(96, dtype=np.int8)
+    oracle().orc_mlp_frequencies(out.ctypes.data_as(C.c_void_p))
+    return out.reshape(32, 3)
+
+
+def mlp_forward(params, frac, bf16: bool) -> np.ndarray:
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    f = np.ascontiguousarray(frac, dtype=np.float64).reshape(-1, 3)
+    y = np.zeros(len(f))
+    oracle().orc_mlp_forward(ptr(p), ptr(f), C.c_size_t(len(f)), C.c_int(1 if bf16 else 0), ptr(y))
+    return y
+
+
+def mlp_loss_grad(params, frac, target, bf16: bool):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    f = np.ascontiguousarray(frac, dtype=np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    g = np.zeros(MLP_PARAMS)
+    loss = oracle().orc_mlp_loss_grad(ptr(p), ptr(f), ptr(t), C.c_size_t(len(f)), C.c_int(1 if bf16 else 0), ptr(g))
+    return loss, g
+
+
+def mlp_offsets():
+    """(name, start, size) of each packed parameter tensor."""
+    sizes = [("W1", 64 * 256), ("b1", 256)] + sum([[(f"W{l}", 256 * 256), (f"b{l}", 256)] for l in (2, 3, 4)], []) + \
+        [("w5", 256), ("b5", 1)]
+    out, o = [], 0
+    for name, n in sizes:
+        out.append((name, o, n))
+        o += n
+    return out
+
+
+def mlp_he_init(seed: int) -> np.ndarray:
+    """He-uniform weights from Philox(seed, 0) at call index = parameter index, zero biases
+    (the initialisation aleator_gpu_mlp_init documents)."""
+    p = np.zeros(MLP_PARAMS)
+    u = uniforms(seed, 0, 0, MLP_PARAMS)
+    for name, o, n in mlp_offsets():
+        if name.startswith("b"):
+            continue
+        fan_in = 64 if name == "W1" else 256
+        a = np.sqrt(3.0 / fan_in) if name == "w5" else np.sqrt(6.0 / fan_in)
+        p[o:o + n] = (2.0 * u[o:o + n] - 1.0) * a
+    return p.astype(np.float32)

@@ -76,3 +76,97 @@ def test_gcmc_errors(ctx):
     gc = aa.Gcmc(dev, [(0, 0, 0, 4, 0.0)], [(1e9, 300.0, 1, 0)], max_molecules=2)
     gc.run(2000)
     assert gc.stats()[0]["error"] == 1  # capacity exceeded is reported, not silently dropped
+
+
+def _proposal_weights(nb, seed):
+    rng = np.random.default_rng(seed)
+    return rng.random(int(np.prod(nb))) ** 4 + 1e-3  # strongly non-uniform, every bin > 0
+
+
+def test_biased_methane_chains_follow_oracle(ctx):
+    """Biased insertion (density over a 2x1x2 sub-cell, repeated): same draws, same trajectory."""
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff, lj.truncation = 6.5, ol.SHIFTED
+    sites = [(0.0, 0.0, 0.0, 4, 0.0)]
+    dev = device_framework(ctx, fw, lj)
+    chains = [(1e8, 300.0, 5, 0), (3e7, 300.0, 6, 2)]
+    nb, reps, mix = (6, 7, 5), (2, 1, 2), 0.1
+    w = _proposal_weights(nb, 1)
+    gc = aa.Gcmc(dev, sites, chains, max_molecules=256)
+    gc.set_proposal(w, nb, reps, mix)
+    trace = gc.run(3000, trace=True)
+    stats = gc.stats()
+    for c, (f, t, seed, stream) in enumerate(chains):
+        g = ol.Gcmc(lj, fw, sites, f, t, seed, stream)
+        g.set_proposal(w, nb, reps, mix)
+        ref_trace = g.run(3000, trace=True)
+        assert np.array_equal(trace[c], ref_trace.astype(np.int32)), f"chain {c} trajectory diverged"
+        att, acc = g.counts
+        assert stats[c]["attempted"] == att and stats[c]["accepted"] == acc
+        assert np.allclose(gc.positions(c), oracle_positions(g), rtol=0, atol=1e-9)
+
+
+def test_biased_charged_co2_chain_follows_oracle(ctx):
+    m = ol.cell_from_params(16.0, 17.0, 18.0, 80.0, 95.0, 100.0)
+    fw = ol.synth_framework(m, 120, seed=4, charge_sigma=0.5)
+    eps = np.array([t[0] for t in ol.C0_TYPES] + [ol.CO2_C[0], ol.CO2_O[0]])
+    sig = np.array([t[1] for t in ol.C0_TYPES] + [ol.CO2_C[1], ol.CO2_O[1]])
+    lj = ol.LJ(eps, sig, 7.5)
+    dev = device_framework(ctx, fw, lj, ewald=(0.35, 7.5, 5))
+    chains = [(5e6, 298.0, 4, 2)]
+    nb, reps, mix = (5, 5, 5), (1, 1, 1), 0.05
+    w = _proposal_weights(nb, 2)
+    gc = aa.Gcmc(dev, ol.co2_sites(), chains, max_molecules=256)
+    gc.set_proposal(w, nb, reps, mix)
+    trace = gc.run(1500, trace=True)
+    g = ol.Gcmc(lj, fw, ol.co2_sites(), 5e6, 298.0, 4, 2, ewald=(0.35, 7.5, 5))
+    g.set_proposal(w, nb, reps, mix)
+    ref_trace = g.run(1500, trace=True)
+    assert np.array_equal(trace[0], ref_trace.astype(np.int32))
+    assert gc.stats()[0]["n_molecules"] == g.n_molecules > 0
+    assert np.allclose(gc.positions(0), oracle_positions(g), rtol=0, atol=1e-9)
+    assert abs(gc.electrostatic_energy(0) - g.electrostatic_energy()) <= 1e-9 * abs(g.electrostatic_energy())
+
+
+def test_biased_insertion_ideal_gas_on_device(ctx):
+    """Detailed balance on the device: an ideal gas stays Poisson(f V / T) and uniform under a
+    proposal that favours one eighth of the cell 50:1."""
+    a, t, mean_n = 20.0, 300.0, 4.0
+    m = np.diag([a, a, a])
+    dev = aa.Framework(ctx, m, [1.0], [1.0], [1.0], np.zeros(1, dtype=np.uint32), [0.0, 0.0], [1.0, 1.0], 5.0)
+    f_pa = mean_n * t / a ** 3 * (1.380649e-23 * 1e30)
+    chains = [(f_pa, t, 100 + c, c) for c in range(32)]
+    gc = aa.Gcmc(dev, [(0.0, 0.0, 0.0, 1, 0.0)], chains, max_molecules=64)
+    w = np.ones(8)
+    w[0] = 50.0
+    gc.set_proposal(w, (2, 2, 2), (1, 1, 1), 0.0)
+    counts = np.zeros(8)
+    for _ in range(40):
+        gc.run(250)
+        for c in range(len(chains)):
+            f = gc.positions(c)[:, 0, :] / a
+            f -= np.floor(f)
+            b = np.minimum((f * 2).astype(int), 1)
+            np.add.at(counts, (b[:, 0] * 2 + b[:, 1]) * 2 + b[:, 2], 1)
+    st = gc.stats()
+    mean = sum(s["sum_molecules"] for s in st) / sum(s["steps"] for s in st)
+    occ = counts / counts.sum()
+    print(f"ideal gas, biased insertion: <N> {mean:.3f} (expect {mean_n}), occupancy {np.round(occ, 3)}")
+    assert mean == pytest.approx(mean_n, abs=0.15)
+    assert np.all(np.abs(occ - 1 / 8) < 0.02)
+    assert all(s["error"] == 0 for s in st)
+
+
+def test_proposal_errors(ctx):
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff = 6.5
+    dev = device_framework(ctx, fw, lj)
+    gc = aa.Gcmc(dev, [(0, 0, 0, 4, 0.0)], [(1e5, 300.0, 1, 0)])
+    with pytest.raises(ValueError):
+        gc.set_proposal(np.array([1.0, 0.0]), (2, 1, 1), (1, 1, 1), 0.0)
+    with pytest.raises(ValueError):
+        gc.set_proposal(np.ones(2), (2, 1, 1), (1, 1, 1), 1.5)
+    off_centre = aa.Gcmc(dev, [(0.5, 0, 0, 4, 0.0), (1.5, 0, 0, 4, 0.0)], [(1e5, 300.0, 1, 0)])
+    with pytest.raises(ValueError):
+        off_centre.set_proposal(np.ones(2), (2, 1, 1), (1, 1, 1), 0.1)
+    gc.set_proposal(None, (1, 1, 1), (1, 1, 1))
