@@ -28,11 +28,39 @@ namespace aleator_gpu {
 
 namespace {
 
-constexpr int kGThreads = 256;
+constexpr int kGThreads = 512;  // 8 waves per chain: two per SIMD, 256 VGPRs each
 constexpr int kMaxKHalf = 2048;   // S(k) half set kept in LDS
 constexpr int kResyncChunk = 16;  // guest sites per resync pass
 
-__device__ inline double uni(const ChainParams& p, uint64_t& call) { return philox_uniform(p.seed, p.stream, call++); }
+constexpr int kDraws = 16;        // uniforms drawn ahead per step (a step takes at most 10)
+
+// The step's uniforms (Philox calls base .. base + kDraws - 1, drawn in parallel into LDS),
+// consumed in order by thread 0.
+struct Draws {
+    const double* u;
+    int used;
+    __device__ double operator()() { return u[used++]; }
+};
+
+// Smallest b with u < cdf[b] over a strictly increasing cdf with cdf[n - 1] = 1 > u: the same
+// bin as a binary search, found by the 64 lanes of one wave probing 64 evenly spaced entries per
+// round (three dependent loads for 2^18 bins instead of eighteen).  Every lane returns it.
+__device__ int wave_cdf_search(const double* cdf, int n, double u) {
+    const int lane = threadIdx.x & 63;
+    int lo = 0, hi = n - 1;  // invariant: cdf[b] <= u for b < lo, u < cdf[hi]
+    while (lo < hi) {
+        const long long span = hi - lo + 1;
+        const int p = lo + (int)(((long long)(lane + 1) * span) / 64) - 1;
+        const bool above = p >= lo && u < cdf[p];
+        const unsigned long long mask = __ballot(above);
+        const int f = __ffsll(mask) - 1;  // lane 63 probes hi, so the mask is never empty
+        const int pf = lo + (int)(((long long)(f + 1) * span) / 64) - 1;
+        const int pprev = f == 0 ? lo - 1 : lo + (int)(((long long)f * span) / 64) - 1;
+        lo = pprev + 1;
+        hi = pf;
+    }
+    return lo;
+}
 
 __device__ inline void frac_of(const GcmcArgs& a, const double c[3], double f[3]) {
     for (int j = 0; j < 3; ++j) f[j] = c[0] * a.inv[j] + c[1] * a.inv[3 + j] + c[2] * a.inv[6 + j];
@@ -61,16 +89,89 @@ __device__ inline double lj_term(const GcmcArgs& a, const double* tab4, int stri
     return v;
 }
 
-// Reduce four doubles over the workgroup; every thread receives the totals.
-__device__ inline void block_sum4(double v[4], double* lds /* [4][4] */) {
+// Reduce four doubles over the workgroup of NW waves, in a fixed order; every thread receives
+// the totals.
+template <int NW>
+__device__ inline void block_sum4(double v[4], double* lds /* [NW][4] */) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int i = 0; i < 4; ++i) {
         const double s = wave_sum(v[i]);
         if (lane == 0) lds[wave * 4 + i] = s;
     }
     __syncthreads();
-    for (int i = 0; i < 4; ++i) v[i] = lds[i] + lds[4 + i] + lds[8 + i] + lds[12 + i];
+    for (int i = 0; i < 4; ++i) {
+        double t = 0.0;
+        for (int w = 0; w < NW; ++w) t += lds[4 * w + i];
+        v[i] = t;
+    }
     __syncthreads();
+}
+
+// Framework bin of a fractional position (wrapped into the cell).
+__device__ inline int framework_bin(const GcmcArgs& a, const double f[3]) {
+    int b[3];
+    for (int j = 0; j < 3; ++j) {
+        const double t = (f[j] - floor(f[j])) * (double)a.fb[j];
+        const int k = t >= 0.0 ? (int)t : 0;
+        b[j] = k < a.fb[j] - 1 ? k : a.fb[j] - 1;
+    }
+    return (b[0] * a.fb[1] + b[1]) * a.fb[2] + b[2];
+}
+
+__device__ inline void bin_centre(const GcmcArgs& a, int bin, double c[3]) {
+    const int b2 = bin % a.fb[2], b1 = (bin / a.fb[2]) % a.fb[1], b0 = bin / (a.fb[1] * a.fb[2]);
+    c[0] = (b0 + 0.5) / a.fb[0];
+    c[1] = (b1 + 0.5) / a.fb[1];
+    c[2] = (b2 + 0.5) / a.fb[2];
+}
+
+// Whether framework atom j lies within sqrt(reach2) of fractional point c (minimum image, no
+// contraction, so the counting and filling passes agree).
+__device__ inline bool atom_within(const GcmcArgs& a, int j, const double c[3], double reach2) {
+#pragma clang fp contract(off)
+    double d[3] = {a.fx[j] - c[0], a.fy[j] - c[1], a.fz[j] - c[2]};
+    for (int i = 0; i < 3; ++i) d[i] = d[i] - rint(d[i]);
+    double r2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double x = d[0] * a.m[k] + d[1] * a.m[3 + k] + d[2] * a.m[6 + k];
+        r2 = r2 + x * x;
+    }
+    return r2 <= reach2;
+}
+
+// One block per bin: how many framework atoms lie within `reach` of its centre.
+__global__ __launch_bounds__(256) void bin_count_kernel(const GcmcArgs a, double reach2, int32_t* counts) {
+    __shared__ int warp_tot[5];
+    double c[3];
+    bin_centre(a, blockIdx.x, c);
+    int n = 0;
+    for (int j = threadIdx.x; j < a.natoms; j += blockDim.x)
+        n += atom_within(a, j, c, reach2) ? 1 : 0;
+    int tot = 0;
+    (void)block_exclusive_scan(n, warp_tot, &tot);
+    if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+// One block per bin: its atoms, in ascending index order, as (fx, fy, fz, q) and type at start[bin].
+__global__ __launch_bounds__(256) void bin_fill_kernel(const GcmcArgs a, double reach2, const int32_t* start,
+                                                       double4* pos, int32_t* type) {
+    __shared__ int warp_tot[5];
+    double c[3];
+    bin_centre(a, blockIdx.x, c);
+    int base = start[blockIdx.x];
+    const int limit = start[blockIdx.x + 1];
+    for (int j0 = 0; j0 < a.natoms; j0 += blockDim.x) {
+        const int j = j0 + threadIdx.x;
+        const int in = (j < a.natoms && atom_within(a, j, c, reach2)) ? 1 : 0;
+        int tot = 0;
+        const int off = block_exclusive_scan(in, warp_tot, &tot);
+        if (in && base + off < limit) {
+            pos[base + off] = double4{a.fx[j], a.fy[j], a.fz[j], a.q[j]};
+            type[base + off] = a.type[j];
+        }
+        base += tot;
+        __syncthreads();
+    }
 }
 
 // Bin of the insertion density holding a molecule's anchor (the centroid of its sites):
@@ -97,9 +198,10 @@ __device__ int anchor_bin(const GcmcArgs& a, const double c[kMaxSites][3]) {
 // Trial positions of one move, in the reference's arithmetic order (no contraction) and draw
 // order: translation (:454-461), rotation about the centroid (:512-532, Rodrigues :40-44),
 // insertion (:568-590, Shoemake quaternion :56-72).  Deletion draws nothing here.  With an
-// insertion density, the insertion anchor comes from it (draws: bin by inverse CDF, image,
-// position in the bin) and `bias` receives V p(anchor) for insertion and deletion.
-__device__ void trial_positions(const GcmcArgs& a, const ChainParams& P, uint64_t& call, int move,
+// insertion density, the insertion anchor comes from it (draws: bin by inverse CDF, found ahead
+// as prop_bin, image, position in the bin) and `bias` receives V p(anchor) for insertion and
+// deletion.
+__device__ void trial_positions(const GcmcArgs& a, Draws& uni, int move, int prop_bin,
                                 const double old_c[kMaxSites][3], double new_c[kMaxSites][3], double& bias) {
 #pragma clang fp contract(off)
     const int ns = a.ns;
@@ -107,7 +209,7 @@ __device__ void trial_positions(const GcmcArgs& a, const ChainParams& P, uint64_
     if (move == 3 && a.prop_cdf) bias = a.prop_fac[anchor_bin(a, old_c)];
     if (move == 0) {
         double d[3];
-        for (int j = 0; j < 3; ++j) d[j] = (2.0 * uni(P, call) - 1.0) * a.max_translation;
+        for (int j = 0; j < 3; ++j) d[j] = (2.0 * uni() - 1.0) * a.max_translation;
         for (int s = 0; s < ns; ++s)
             for (int j = 0; j < 3; ++j) new_c[s][j] = old_c[s][j] + d[j];
     } else if (move == 1) {
@@ -117,12 +219,12 @@ __device__ void trial_positions(const GcmcArgs& a, const ChainParams& P, uint64_
             for (int j = 0; j < 3; ++j) cen[j] = cen[j] + old_c[s][j];
         const double inv_n = 1.0 / (double)ns;
         for (int j = 0; j < 3; ++j) cen[j] = cen[j] * inv_n;
-        const double cos_t = 2.0 * uni(P, call) - 1.0;
+        const double cos_t = 2.0 * uni() - 1.0;
         const double om = 1.0 - cos_t * cos_t;
         const double sin_t = sqrt((0.0 < om) ? om : 0.0);
-        const double phi = 2.0 * kPi * uni(P, call);
+        const double phi = 2.0 * kPi * uni();
         const double ax[3] = {sin_t * cos(phi), sin_t * sin(phi), cos_t};
-        const double angle = (2.0 * uni(P, call) - 1.0) * a.max_rotation;
+        const double angle = (2.0 * uni() - 1.0) * a.max_rotation;
         const double c = cos(angle), sn = sin(angle);
         for (int s = 0; s < ns; ++s) {
             const double v[3] = {old_c[s][0] - cen[0], old_c[s][1] - cen[1], old_c[s][2] - cen[2]};
@@ -136,36 +238,29 @@ __device__ void trial_positions(const GcmcArgs& a, const ChainParams& P, uint64_
         // attemptInsertion (:568-590)
         double ref[3];
         if (a.prop_cdf) {
-            const double ub = uni(P, call);
-            int lo = 0, hi = a.prop_nbins - 1;  // smallest bin with ub < cdf[bin]; cdf[last] = 1
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (ub < a.prop_cdf[mid])
-                    hi = mid;
-                else
-                    lo = mid + 1;
-            }
+            (void)uni();  // the bin draw: prop_bin is the smallest bin with this uniform < cdf[bin]
+            const int lo = prop_bin;
             bias = a.prop_fac[lo];
             const int bn[3] = {lo / (a.prop_nb[1] * a.prop_nb[2]), (lo / a.prop_nb[2]) % a.prop_nb[1], lo % a.prop_nb[2]};
             const int n_img = a.prop_reps[0] * a.prop_reps[1] * a.prop_reps[2];
-            int img = (int)(uni(P, call) * (double)n_img);
+            int img = (int)(uni() * (double)n_img);
             img = img < n_img - 1 ? img : n_img - 1;
             const int im[3] = {img / (a.prop_reps[1] * a.prop_reps[2]), (img / a.prop_reps[2]) % a.prop_reps[1],
                                img % a.prop_reps[2]};
             double f[3];
             for (int j = 0; j < 3; ++j) {
-                const double u = uni(P, call);
+                const double u = uni();
                 f[j] = ((double)im[j] + ((double)bn[j] + u) / (double)a.prop_nb[j]) / (double)a.prop_reps[j];
             }
             for (int j = 0; j < 3; ++j) ref[j] = a.m[j] * f[0] + a.m[3 + j] * f[1] + a.m[6 + j] * f[2];
         } else {
-            const double u0 = uni(P, call), u1 = uni(P, call), u2 = uni(P, call);
+            const double u0 = uni(), u1 = uni(), u2 = uni();
             for (int j = 0; j < 3; ++j) ref[j] = a.m[j] * u0 + a.m[3 + j] * u1 + a.m[6 + j] * u2;
         }
         if (ns == 1) {
             for (int j = 0; j < 3; ++j) new_c[0][j] = ref[j] + a.local[0][j];
         } else {
-            const double v0 = uni(P, call), v1 = uni(P, call), v2 = uni(P, call);
+            const double v0 = uni(), v1 = uni(), v2 = uni();
             const double r1 = sqrt(1.0 - v0), r2 = sqrt(v0);
             const double th1 = 2.0 * kPi * v1, th2 = 2.0 * kPi * v2;
             const double qw = r2 * cos(th2), qx = r1 * sin(th1), qy = r1 * cos(th1), qz = r2 * sin(th2);
@@ -182,13 +277,25 @@ __device__ void trial_positions(const GcmcArgs& a, const ChainParams& P, uint64_
     }
 }
 
-__global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a) {
+// One workgroup of NT threads per chain.  Thread 0 takes every random draw of the chain, builds
+// the move's trial sites and publishes them through LDS (fractional coordinates and charges of
+// the removed rows, then the added rows); all threads then split the energy scans and the k-sum;
+// thread 0 applies the acceptance rule and publishes the decision.
+template <int NT>
+__global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
+    constexpr int NW = NT / 64;
     __shared__ double s_re[kMaxKHalf], s_im[kMaxKHalf];
     __shared__ cplx phase[3 * (2 * kMaxK + 1) * 2 * kMaxSites];
     __shared__ cplx rphase[3 * (2 * kMaxK + 1) * kResyncChunk];
-    __shared__ double site4[2 * kMaxSites * 4];
-    __shared__ double red[16];
-    __shared__ int s_accept;
+    __shared__ double site4[2 * kMaxSites * 4];  // removed rows, then added rows: f0 f1 f2 q
+    __shared__ double oldc[kMaxSites][3];        // cartesian positions of the removed sites (thread 0)
+    __shared__ double newc[kMaxSites][3];        // cartesian positions of the added sites
+    __shared__ double red[4 * NW];
+    __shared__ int ctl[3];                        // move (-1: nothing to attempt), molecule, accept
+    __shared__ double s_u[kDraws];                // this step's uniforms
+    __shared__ uint64_t s_call;                   // Philox call index of the next step's first draw
+    __shared__ int s_rbeg[2 * kMaxSites], s_rlen[2 * kMaxSites];  // framework list of each site row
+    __shared__ double s_lj[4 * kMaxSites * kMaxTypes];             // site vs framework type LJ constants
 
     const int chain = blockIdx.x;
     const int tid = threadIdx.x;
@@ -199,115 +306,152 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
     const int nkk = 2 * a.kmax + 1;
 
     if (a.charged)
-        for (int t = tid; t < a.nk_half; t += kGThreads) {
+        for (int t = tid; t < a.nk_half; t += NT) {
             s_re[t] = a.sre[(size_t)chain * a.nk_half + t];
             s_im[t] = a.sim[(size_t)chain * a.nk_half + t];
         }
+    for (int t = tid; t < 4 * kMaxSites * kMaxTypes; t += NT) s_lj[t] = a.ljtab[t];
     __syncthreads();
 
-    uint64_t call = st->call;
+    // tracked by every thread
     int n_mol = st->n_mol;
     int since_resync = st->moves_since_resync;
+    int error = st->error;
+    if (tid == 0) s_call = st->call;
+    // used by thread 0 only
     double self_e = st->self_energy, excl_e = st->exclusion_energy;
     double sum_n = st->sum_n;
     uint64_t att[4] = {st->attempted[0], st->attempted[1], st->attempted[2], st->attempted[3]};
     uint64_t acc[4] = {st->accepted[0], st->accepted[1], st->accepted[2], st->accepted[3]};
-    int error = st->error;
     const double volume = fabs(a.volume);
+    double bias = 1.0;  // V p(anchor) of the insertion density (1: uniform insertion), thread 0
+    double pend_self = 0.0, pend_excl = 0.0;  // self / exclusion deltas of the pending move, thread 0
 
+    __syncthreads();
     for (uint64_t step = 0; step < a.n_steps && !error; ++step) {
-        const double pick = uni(P, call);
-        const int move = pick < 0.25 ? 0 : (pick < 0.5 ? 1 : (pick < 0.75 ? 2 : 3));
-        bool skip = (move != 2 && n_mol == 0) || (move == 1 && ns < 2);
-        double old_c[kMaxSites][3], new_c[kMaxSites][3];
-        double bias = 1.0;  // V p(anchor) of the insertion density (1: uniform insertion)
-        int mi = -1;
-        if (!skip) {
-            if (move != 2) {
-                mi = (int)(uni(P, call) * (double)n_mol);
-                for (int s = 0; s < ns; ++s)
-                    for (int j = 0; j < 3; ++j) old_c[s][j] = pos[((size_t)mi * ns + s) * 3 + j];
+        // ---------------- this step's uniforms, drawn ahead in parallel
+        const uint64_t call = s_call;
+        if (tid < kDraws) s_u[tid] = philox_uniform(P.seed, P.stream, call + tid);
+        __syncthreads();
+        // the insertion bin of a biased insertion (the draw after the move pick), by wave 0
+        int prop_bin = 0;
+        if (a.prop_cdf && tid < 64 && s_u[0] >= 0.5 && s_u[0] < 0.75)
+            prop_bin = wave_cdf_search(a.prop_cdf, a.prop_nbins, s_u[1]);
+        Draws uni{s_u, 0};
+        // ---------------- trial sites (thread 0)
+        if (tid == 0) {
+            const double pick = uni();
+            const int move = pick < 0.25 ? 0 : (pick < 0.5 ? 1 : (pick < 0.75 ? 2 : 3));
+            const bool skip = (move != 2 && n_mol == 0) || (move == 1 && ns < 2);
+            int mi = -1;
+            if (!skip) {
+                if (move != 2) {
+                    mi = (int)(uni() * (double)n_mol);
+                    for (int s = 0; s < ns; ++s)
+                        for (int j = 0; j < 3; ++j) oldc[s][j] = pos[((size_t)mi * ns + s) * 3 + j];
+                }
+                trial_positions(a, uni, move, prop_bin, oldc, newc, bias);
+                int row = 0;
+                if (move != 2)
+                    for (int s = 0; s < ns; ++s, ++row) {
+                        frac_of(a, oldc[s], &site4[4 * row]);
+                        site4[4 * row + 3] = a.charge[s];
+                    }
+                if (move != 3)
+                    for (int s = 0; s < ns; ++s, ++row) {
+                        frac_of(a, newc[s], &site4[4 * row]);
+                        site4[4 * row + 3] = a.charge[s];
+                    }
             }
-            trial_positions(a, P, call, move, old_c, new_c, bias);
+            ctl[0] = skip ? -1 : move;
+            ctl[1] = mi;
+            if (skip) s_call = call + (uint64_t)uni.used;
         }
-        if (skip) {
-            sum_n += (double)n_mol;
-            if (a.n_trace && tid == 0) a.n_trace[(size_t)chain * a.n_steps + step] = n_mol;
+        __syncthreads();
+        const int move = ctl[0], mi = ctl[1];
+        if (move < 0) {
+            if (tid == 0) {
+                sum_n += (double)n_mol;
+                if (a.n_trace) a.n_trace[(size_t)chain * a.n_steps + step] = n_mol;
+            }
+            __syncthreads();  // ctl, s_u and s_call are rewritten next step
             continue;
         }
         const bool has_old = move != 2, has_new = move != 3;
-        // fractional coordinates of the old (rows 0..ns-1) and new (rows ns..2ns-1) sites
-        double fo[kMaxSites][3], fn[kMaxSites][3];
-        for (int s = 0; s < ns; ++s) {
-            if (has_old) frac_of(a, old_c[s], fo[s]);
-            if (has_new) frac_of(a, new_c[s], fn[s]);
-        }
+        const int n_rem = has_old ? ns : 0, n_add = has_new ? ns : 0, nsite = n_rem + n_add;
+
         // ---------------- pair energies: LJ and real-space Ewald, old and new
         double e4[4] = {0.0, 0.0, 0.0, 0.0};  // lj_old, lj_new, real_old, real_new
-        for (int j = tid; j < a.natoms; j += kGThreads) {
-            const double ax = a.fx[j], ay = a.fy[j], az = a.fz[j], qj = a.q[j];
-            const int t = a.type[j];
-            for (int s = 0; s < ns; ++s) {
-                if (has_old) {
-                    const double r2 = dist2(a, ax - fo[s][0], ay - fo[s][1], az - fo[s][2]);
-                    if (r2 <= a.rc2_lj) e4[0] += lj_term(a, a.ljtab, kMaxSites * kMaxTypes, s * kMaxTypes + t, r2);
-                    if (a.charged && r2 <= a.rc2_real) {
-                        const double r = sqrt(r2);
-                        e4[2] += a.charge[s] * qj * (erfc(a.alpha * r) / r);
-                    }
+        // framework: every (site row, atom of that row's bin list) pair in one loop, so that the
+        // rows' loads are in flight together (all atoms per row without lists)
+        if (tid < nsite) {
+            int beg = 0, len = a.natoms;
+            if (a.bin_start) {
+                const double f[3] = {site4[4 * tid], site4[4 * tid + 1], site4[4 * tid + 2]};
+                const int b = framework_bin(a, f);
+                beg = a.bin_start[b];
+                len = a.bin_start[b + 1] - beg;
+            }
+            s_rbeg[tid] = beg;
+            s_rlen[tid] = len;
+        }
+        __syncthreads();
+        {
+            int row = 0, row_start = 0, row_len = s_rlen[0];
+            for (int i = tid;; i += NT) {
+                while (row < nsite && i >= row_start + row_len) {
+                    row_start += row_len;
+                    if (++row < nsite) row_len = s_rlen[row];
                 }
-                if (has_new) {
-                    const double r2 = dist2(a, ax - fn[s][0], ay - fn[s][1], az - fn[s][2]);
-                    if (r2 <= a.rc2_lj) e4[1] += lj_term(a, a.ljtab, kMaxSites * kMaxTypes, s * kMaxTypes + t, r2);
-                    if (a.charged && r2 <= a.rc2_real) {
-                        const double r = sqrt(r2);
-                        e4[3] += a.charge[s] * qj * (erfc(a.alpha * r) / r);
-                    }
+                if (row >= nsite) break;
+                const int k = s_rbeg[row] + (i - row_start);
+                double4 at;
+                int ty;
+                if (a.bin_start) {
+                    at = a.bin_pos[k];
+                    ty = a.bin_type[k];
+                } else {
+                    at = double4{a.fx[k], a.fy[k], a.fz[k], a.q[k]};
+                    ty = a.type[k];
+                }
+                const bool is_old = row < n_rem;
+                const int s = is_old ? row : row - n_rem;
+                const double r2 = dist2(a, at.x - site4[4 * row], at.y - site4[4 * row + 1], at.z - site4[4 * row + 2]);
+                if (r2 <= a.rc2_lj)
+                    e4[is_old ? 0 : 1] += lj_term(a, s_lj, kMaxSites * kMaxTypes, s * kMaxTypes + ty, r2);
+                if (a.charged && r2 <= a.rc2_real) {
+                    const double r = sqrt(r2);
+                    e4[is_old ? 2 : 3] += a.charge[s] * at.w * (erfc(a.alpha * r) / r);
                 }
             }
         }
-        for (int g = tid; g < n_mol; g += kGThreads) {
+        // other guests
+        for (int g = tid; g < n_mol; g += NT) {
             if (g == mi) continue;
             for (int u = 0; u < ns; ++u) {
                 const double* gp = pos + ((size_t)g * ns + u) * 3;
                 const double gc[3] = {gp[0], gp[1], gp[2]};
                 double gf[3];
                 frac_of(a, gc, gf);
-                for (int s = 0; s < ns; ++s) {
+                for (int row = 0; row < nsite; ++row) {
+                    const bool is_old = row < n_rem;
+                    const int s = is_old ? row : row - n_rem;
                     const int idx = s * kMaxSites + u;
-                    if (has_old) {
-                        const double r2 = dist2(a, gf[0] - fo[s][0], gf[1] - fo[s][1], gf[2] - fo[s][2]);
-                        if (r2 <= a.rc2_lj) e4[0] += lj_term(a, a.ljgg, kMaxSites * kMaxSites, idx, r2);
-                        if (a.charged && r2 <= a.rc2_real) {
-                            const double r = sqrt(r2);
-                            e4[2] += a.charge[s] * a.charge[u] * (erfc(a.alpha * r) / r);
-                        }
-                    }
-                    if (has_new) {
-                        const double r2 = dist2(a, gf[0] - fn[s][0], gf[1] - fn[s][1], gf[2] - fn[s][2]);
-                        if (r2 <= a.rc2_lj) e4[1] += lj_term(a, a.ljgg, kMaxSites * kMaxSites, idx, r2);
-                        if (a.charged && r2 <= a.rc2_real) {
-                            const double r = sqrt(r2);
-                            e4[3] += a.charge[s] * a.charge[u] * (erfc(a.alpha * r) / r);
-                        }
+                    const double r2 = dist2(a, gf[0] - site4[4 * row], gf[1] - site4[4 * row + 1],
+                                            gf[2] - site4[4 * row + 2]);
+                    if (r2 <= a.rc2_lj) e4[is_old ? 0 : 1] += lj_term(a, a.ljgg, kMaxSites * kMaxSites, idx, r2);
+                    if (a.charged && r2 <= a.rc2_real) {
+                        const double r = sqrt(r2);
+                        e4[is_old ? 2 : 3] += a.charge[s] * a.charge[u] * (erfc(a.alpha * r) / r);
                     }
                 }
             }
         }
-        block_sum4(e4, red);
+        block_sum4<NW>(e4, red);
         // ---------------- reciprocal space: removed = old sites, added = new sites
         double recip = 0.0;
-        const int n_rem = has_old ? ns : 0, n_add = has_new ? ns : 0, nsite = n_rem + n_add;
         if (a.charged) {
-            if (tid < nsite) {
-                const double* f = tid < n_rem ? fo[tid] : fn[tid - n_rem];
-                site4[4 * tid] = f[0];
-                site4[4 * tid + 1] = f[1];
-                site4[4 * tid + 2] = f[2];
-                site4[4 * tid + 3] = a.charge[tid < n_rem ? tid : tid - n_rem];
-            }
-            __syncthreads();
-            for (int i = tid; i < 3 * nkk * nsite; i += kGThreads) {
+            for (int i = tid; i < 3 * nkk * nsite; i += NT) {
                 const int s = i % nsite, mm = (i / nsite) % nkk, axis = i / (nsite * nkk);
                 double sv, cv;
                 sincospi(2.0 * (double)(mm - a.kmax) * site4[4 * s + axis], &sv, &cv);
@@ -315,7 +459,7 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
             }
             __syncthreads();
             double accr = 0.0;
-            for (int t = tid; t < a.nk_half; t += kGThreads) {
+            for (int t = tid; t < a.nk_half; t += NT) {
                 const int4 n = a.kn[t];
                 double dre = 0.0, dim = 0.0;
                 for (int s = 0; s < nsite; ++s) {
@@ -328,10 +472,10 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
                 accr += a.kw[t] * (2.0 * (s_re[t] * dre + s_im[t] * dim) + (dre * dre + dim * dim));
             }
             double r4[4] = {accr, 0.0, 0.0, 0.0};
-            block_sum4(r4, red);
+            block_sum4<NW>(r4, red);
             recip = a.recip_scale * r4[0];
         }
-        // ---------------- acceptance (one thread), gcmc_acceptance.hpp:28-101
+        // ---------------- acceptance (thread 0), gcmc_acceptance.hpp:28-101
         if (tid == 0) {
             double self_d = 0.0, excl_d = 0.0;
             if (a.charged) {
@@ -342,18 +486,20 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
                     for (int j2 = i + 1; j2 < ns; ++j2) {
                         const double qq = a.charge[i] * a.charge[j2];
                         if (has_new) {
-                            const double r = sqrt(dist2(a, fn[j2][0] - fn[i][0], fn[j2][1] - fn[i][1], fn[j2][2] - fn[i][2]));
+                            const double* fi = &site4[4 * (n_rem + i)];
+                            const double* fj = &site4[4 * (n_rem + j2)];
+                            const double r = sqrt(dist2(a, fj[0] - fi[0], fj[1] - fi[1], fj[2] - fi[2]));
                             ex_new += qq * (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
                         }
                         if (has_old) {
-                            const double r = sqrt(dist2(a, fo[j2][0] - fo[i][0], fo[j2][1] - fo[i][1], fo[j2][2] - fo[i][2]));
+                            const double* fi = &site4[4 * i];
+                            const double* fj = &site4[4 * j2];
+                            const double r = sqrt(dist2(a, fj[0] - fi[0], fj[1] - fi[1], fj[2] - fi[2]));
                             ex_old += qq * (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
                         }
                     }
                 self_d = a.self_pref * ((has_new ? q2 : 0.0) - (has_old ? q2 : 0.0));
                 excl_d = -a.kc * (ex_new - ex_old);
-                red[0] = self_d;
-                red[1] = excl_d;
             }
             const double recip_total = recip + self_d + excl_d;
             const double T = P.temperature;
@@ -370,24 +516,27 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
                 ratio = (double)n_mol / (P.fugacity_internal * volume / T) * exp(-du / T);
                 if (a.prop_cdf) ratio = ratio * bias;
             }
-            s_accept = uni(P, call) < (ratio < 1.0 ? ratio : 1.0) ? 1 : 0;
-        } else {
-            call += 1;  // the accept draw, taken by thread 0
+            const bool accepted = uni() < (ratio < 1.0 ? ratio : 1.0);
+            s_call = call + (uint64_t)uni.used;
+            att[move] += 1;
+            if (accepted) acc[move] += 1;
+            pend_self = self_d;
+            pend_excl = excl_d;
+            ctl[2] = accepted ? 1 : 0;
         }
         __syncthreads();
-        const bool accept = s_accept != 0;
-        const double self_d = a.charged ? red[0] : 0.0, excl_d = a.charged ? red[1] : 0.0;
-        __syncthreads();
-        att[move] += 1;
-        if (accept) {
-            acc[move] += 1;
+        if (ctl[2]) {
             if (move == 2 && n_mol >= a.max_mol) {
                 error = 1;
                 break;
             }
+            if (tid == 0 && a.charged) {
+                self_e += pend_self;
+                excl_e += pend_excl;
+            }
             // commit S(k) (the same delta, recomputed, as commit* does, :161-172)
             if (a.charged) {
-                for (int t = tid; t < a.nk_half; t += kGThreads) {
+                for (int t = tid; t < a.nk_half; t += NT) {
                     const int4 n = a.kn[t];
                     double dre = 0.0, dim = 0.0;
                     for (int s = 0; s < nsite; ++s) {
@@ -400,19 +549,17 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
                     s_re[t] += dre;
                     s_im[t] += dim;
                 }
-                self_e += self_d;
-                excl_e += excl_d;
             }
             // positions
             if (move <= 1) {
-                if (tid < ns * 3) pos[((size_t)mi * ns + tid / 3) * 3 + tid % 3] = new_c[tid / 3][tid % 3];
+                if (tid < ns * 3) pos[(size_t)mi * ns * 3 + tid] = newc[tid / 3][tid % 3];
             } else if (move == 2) {
-                if (tid < ns * 3) pos[((size_t)n_mol * ns + tid / 3) * 3 + tid % 3] = new_c[tid / 3][tid % 3];
+                if (tid < ns * 3) pos[(size_t)n_mol * ns * 3 + tid] = newc[tid / 3][tid % 3];
                 n_mol += 1;
             } else {
                 // erase molecule mi, keeping the order of the others (:821)
                 const size_t first = (size_t)mi * ns * 3, count = (size_t)(n_mol - 1 - mi) * ns * 3;
-                for (size_t base = 0; base < count; base += kGThreads) {
+                for (size_t base = 0; base < count; base += NT) {
                     double v = 0.0;
                     if (base + tid < count) v = pos[first + ns * 3 + base + tid];
                     __syncthreads();
@@ -426,7 +573,7 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
             // maybeResyncEwald (:380-403)
             if (a.charged && ++since_resync >= 100) {
                 since_resync = 0;
-                for (int t = tid; t < a.nk_half; t += kGThreads) {
+                for (int t = tid; t < a.nk_half; t += NT) {
                     s_re[t] = a.fw_sre[t];
                     s_im[t] = a.fw_sim[t];
                 }
@@ -434,7 +581,7 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
                 for (int base = 0; base < total; base += kResyncChunk) {
                     const int cnt = min(kResyncChunk, total - base);
                     __syncthreads();
-                    for (int i = tid; i < 3 * nkk * kResyncChunk; i += kGThreads) {
+                    for (int i = tid; i < 3 * nkk * kResyncChunk; i += NT) {
                         const int s = i % kResyncChunk, mm = (i / kResyncChunk) % nkk, axis = i / (kResyncChunk * nkk);
                         cplx v = {0.0, 0.0};
                         if (s < cnt) {
@@ -449,7 +596,7 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
                         rphase[i] = v;
                     }
                     __syncthreads();
-                    for (int t = tid; t < a.nk_half; t += kGThreads) {
+                    for (int t = tid; t < a.nk_half; t += NT) {
                         const int4 n = a.kn[t];
                         double dre = 0.0, dim = 0.0;
                         for (int s = 0; s < cnt; ++s) {
@@ -465,37 +612,42 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
                     }
                 }
                 // self and exclusion from scratch (ewald_incremental_state.cc:230-246)
-                double q2 = 0.0;
-                for (int s = 0; s < ns; ++s) q2 += a.charge[s] * a.charge[s];
-                self_e = a.fw_self + a.self_pref * (q2 * (double)n_mol);
-                double ex = 0.0;
-                for (int g = 0; g < n_mol; ++g)
-                    for (int i = 0; i < ns; ++i)
-                        for (int j2 = i + 1; j2 < ns; ++j2) {
-                            double fi[3], fj[3];
-                            const double* pi = pos + ((size_t)g * ns + i) * 3;
-                            const double* pj = pos + ((size_t)g * ns + j2) * 3;
-                            const double ci[3] = {pi[0], pi[1], pi[2]}, cj[3] = {pj[0], pj[1], pj[2]};
-                            frac_of(a, ci, fi);
-                            frac_of(a, cj, fj);
-                            const double r = sqrt(dist2(a, fj[0] - fi[0], fj[1] - fi[1], fj[2] - fi[2]));
-                            ex += a.charge[i] * a.charge[j2] * (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
-                        }
-                excl_e = -a.kc * ex;
+                if (tid == 0) {
+                    double q2 = 0.0;
+                    for (int s = 0; s < ns; ++s) q2 += a.charge[s] * a.charge[s];
+                    self_e = a.fw_self + a.self_pref * (q2 * (double)n_mol);
+                    double ex = 0.0;
+                    for (int g = 0; g < n_mol; ++g)
+                        for (int i = 0; i < ns; ++i)
+                            for (int j2 = i + 1; j2 < ns; ++j2) {
+                                double fi[3], fj[3];
+                                const double* pi = pos + ((size_t)g * ns + i) * 3;
+                                const double* pj = pos + ((size_t)g * ns + j2) * 3;
+                                const double ci[3] = {pi[0], pi[1], pi[2]}, cj[3] = {pj[0], pj[1], pj[2]};
+                                frac_of(a, ci, fi);
+                                frac_of(a, cj, fj);
+                                const double r = sqrt(dist2(a, fj[0] - fi[0], fj[1] - fi[1], fj[2] - fi[2]));
+                                ex += a.charge[i] * a.charge[j2] *
+                                      (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
+                            }
+                    excl_e = -a.kc * ex;
+                }
                 __syncthreads();
             }
         }
-        sum_n += (double)n_mol;
-        if (a.n_trace && tid == 0) a.n_trace[(size_t)chain * a.n_steps + step] = n_mol;
+        if (tid == 0) {
+            sum_n += (double)n_mol;
+            if (a.n_trace) a.n_trace[(size_t)chain * a.n_steps + step] = n_mol;
+        }
     }
     __syncthreads();
     if (a.charged)
-        for (int t = tid; t < a.nk_half; t += kGThreads) {
+        for (int t = tid; t < a.nk_half; t += NT) {
             a.sre[(size_t)chain * a.nk_half + t] = s_re[t];
             a.sim[(size_t)chain * a.nk_half + t] = s_im[t];
         }
     if (tid == 0) {
-        st->call = call;
+        st->call = s_call;
         st->n_mol = n_mol;
         st->moves_since_resync = since_resync;
         st->self_energy = self_e;
@@ -512,9 +664,35 @@ __global__ __launch_bounds__(kGThreads) void gcmc_chain_kernel(const GcmcArgs a)
 
 }  // namespace
 
+void build_framework_bins(const GcmcArgs& a, double reach, int32_t** start, double4** pos, int32_t** type,
+                          hipStream_t s) {
+    const int nbins = a.fb[0] * a.fb[1] * a.fb[2];
+    int32_t* counts = nullptr;
+    ALEATOR_HIP_CHECK(hipMalloc(&counts, nbins * sizeof(int32_t)));
+    std::vector<int32_t> h(nbins), st(nbins + 1, 0);
+    try {
+        hipLaunchKernelGGL(bin_count_kernel, dim3(nbins), dim3(256), 0, s, a, reach * reach, counts);
+        ALEATOR_HIP_CHECK(hipGetLastError());
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(h.data(), counts, nbins * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipFree(counts);
+        throw;
+    }
+    (void)hipFree(counts);
+    for (int b = 0; b < nbins; ++b) st[b + 1] = st[b] + h[b];
+    ALEATOR_HIP_CHECK(hipMalloc(start, (nbins + 1) * sizeof(int32_t)));
+    ALEATOR_HIP_CHECK(hipMalloc(pos, std::max<size_t>(st[nbins], 1) * sizeof(double4)));
+    ALEATOR_HIP_CHECK(hipMalloc(type, std::max<size_t>(st[nbins], 1) * sizeof(int32_t)));
+    ALEATOR_HIP_CHECK(hipMemcpyAsync(*start, st.data(), (nbins + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(bin_fill_kernel, dim3(nbins), dim3(256), 0, s, a, reach * reach, *start, *pos, *type);
+    ALEATOR_HIP_CHECK(hipGetLastError());
+    ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 void launch_gcmc(const GcmcArgs& a, int n_chains, hipStream_t s) {
     if (a.nk_half > kMaxKHalf) fail(ALEATOR_NOT_IMPLEMENTED, "gcmc: more than 4096 reciprocal terms");
-    hipLaunchKernelGGL(gcmc_chain_kernel, dim3(n_chains), dim3(kGThreads), 0, s, a);
+    hipLaunchKernelGGL(gcmc_chain_kernel<kGThreads>, dim3(n_chains), dim3(kGThreads), 0, s, a);
     ALEATOR_HIP_CHECK(hipGetLastError());
 }
 

@@ -65,7 +65,18 @@ struct GcmcArgs {
     const double* prop_cdf;  // [prop_nbins] cumulative bin probabilities, last = 1
     const double* prop_fac;  // [prop_nbins] n_bins * (cdf[b] - cdf[b-1]) = V p(r) in bin b
     int prop_nb[3], prop_reps[3], prop_nbins;
+    // framework atoms within rc + half the longest bin diagonal of each bin centre, for bins
+    // fb[0] x fb[1] x fb[2] of the cell (CSR; null: every site scans every atom)
+    const int32_t* bin_start;  // [n_bins + 1]
+    const double4* bin_pos;    // list entries: fractional x, y, z and charge of the atom
+    const int32_t* bin_type;   // list entries: compact LJ type of the atom
+    int fb[3];
 };
+
+// Builds the framework bin lists of `a` (a.fb set) into device buffers owned by the caller:
+// start [n_bins + 1], then per entry the atom's (fx, fy, fz, q) and type, in bin order.
+void build_framework_bins(const GcmcArgs& a, double reach, int32_t** start, double4** pos, int32_t** type,
+                          hipStream_t s);
 
 void launch_gcmc(const GcmcArgs& a, int n_chains, hipStream_t s);
 

@@ -28,6 +28,9 @@ struct aleator_gcmc {
     double* d_ljtab = nullptr;
     double* d_ljgg = nullptr;
     double* d_prop = nullptr;  // [2][n_bins]: cdf, then n_bins * p
+    int32_t* d_bin_start = nullptr;
+    double4* d_bin_pos = nullptr;
+    int32_t* d_bin_type = nullptr;
 };
 
 namespace aleator_gpu {
@@ -184,12 +187,38 @@ aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_m
                 }
             }
             ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+            // framework bin lists: bins about 2 A wide; every atom within rc of a point in a bin is
+            // within rc + h of its centre (h: half the longest bin diagonal); used only while that
+            // reach stays below half the smallest perpendicular width, where the fractional
+            // minimum image is exact
+            const double rcmax = std::max(fw->rc, fw->charged ? fw->rc_real : 0.0);
+            double wmin = fw->lat.lperp[0];
+            for (int i = 0; i < 3; ++i) {
+                a.fb[i] = std::clamp((int)(fw->lat.lperp[i] / 2.0), 1, 64);
+                wmin = std::min(wmin, fw->lat.lperp[i]);
+            }
+            double hmax = 0.0;
+            const int sg[4][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {-1, 1, 1}};
+            for (auto& d : sg) {
+                double v[3] = {0.0, 0.0, 0.0};
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) v[j] += d[i] * fw->lat.m[i][j] / a.fb[i];
+                hmax = std::max(hmax, 0.5 * std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+            }
+            const double reach = rcmax + hmax;
+            if (reach < 0.5 * wmin) {
+                build_framework_bins(a, reach, &g->d_bin_start, &g->d_bin_pos, &g->d_bin_type, s);
+            }
         } catch (...) {
             for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_state,
-                            (void*)g->d_pos, (void*)g->d_sre, (void*)g->d_sim})
+                            (void*)g->d_pos, (void*)g->d_sre, (void*)g->d_sim, (void*)g->d_bin_start,
+                            (void*)g->d_bin_pos, (void*)g->d_bin_type})
                 if (p) (void)hipFree(p);
             throw;
         }
+        a.bin_start = g->d_bin_start;
+        a.bin_pos = g->d_bin_pos;
+        a.bin_type = g->d_bin_type;
         a.ljtab = g->d_ljtab;
         a.ljgg = g->d_ljgg;
         a.params = g->d_params;
@@ -354,7 +383,8 @@ aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* g) {
         if (!g) return;
         (void)hipStreamSynchronize(g->fw->ctx->stream);
         for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_state, (void*)g->d_pos,
-                        (void*)g->d_sre, (void*)g->d_sim, (void*)g->d_prop})
+                        (void*)g->d_sre, (void*)g->d_sim, (void*)g->d_prop, (void*)g->d_bin_start,
+                        (void*)g->d_bin_pos, (void*)g->d_bin_type})
             if (p) (void)hipFree(p);
         delete g;
     });

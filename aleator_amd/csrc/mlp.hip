@@ -61,11 +61,39 @@ struct GemmArgs {
     long long sbk, sbn;
     int m, n, k;
     int epilogue;
-    float* c;          // fp32 out [m][n] (kStoreF32, kBiasSilu: Z)
+    float* c;          // fp32 out [m][n] (kStoreF32, kBiasSilu: Z); split-K: partials [split][m][n]
     const float* bias; // [n]
     uint16_t* h;       // bf16 out [m][n] (kBiasSilu: H, kSiluGrad: dZ)
     const float* zref; // [m][n] (kSiluGrad)
+    int ones_row;      // row of A that reads as 1 (bias gradients as one more GEMM row), or -1
+    int kchunk;        // K per split (a multiple of kBK); gridDim.z splits
 };
+
+// Stage a kBM x kBK tile of A (rows m0.., columns k0..) and a kBK x kBN tile of B through LDS
+// with consecutive threads on consecutive addresses for either operand layout.
+__device__ inline void stage(const GemmArgs& g, int m0, int n0, int k0, int kend, uint16_t (*As)[kBK + 4],
+                             uint16_t (*Bs)[kBN + 4]) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kBM * kBK; i += 256) {
+        const int r = g.sak == 1 ? i / kBK : i % kBM;
+        const int c = g.sak == 1 ? i % kBK : i / kBM;
+        const int gm = m0 + r, gk = k0 + c;
+        uint16_t v = 0;
+        if (gk < kend) {
+            if (gm == g.ones_row)
+                v = 0x3f80;  // 1.0
+            else if (gm < g.m)
+                v = g.a[gm * g.sam + gk * g.sak];
+        }
+        As[r][c] = v;
+    }
+    for (int i = tid; i < kBK * kBN; i += 256) {
+        const int r = g.sbn == 1 ? i / kBN : i % kBK;
+        const int c = g.sbn == 1 ? i % kBN : i / kBK;
+        const int gk = k0 + r, gn = n0 + c;
+        Bs[r][c] = (gk < kend && gn < g.n) ? g.b[gk * g.sbk + gn * g.sbn] : (uint16_t)0;
+    }
+}
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
     __shared__ uint16_t As[kBM][kBK + 4];
@@ -73,20 +101,13 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int kbeg = blockIdx.z * g.kchunk;
+    const int kend = min(g.k, kbeg + g.kchunk);
     f32x4 acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.k; k0 += kBK) {
-        for (int i = tid; i < kBM * kBK; i += 256) {
-            const int r = i / kBK, c = i % kBK;
-            const int gm = m0 + r, gk = k0 + c;
-            As[r][c] = (gm < g.m && gk < g.k) ? g.a[gm * g.sam + gk * g.sak] : (uint16_t)0;
-        }
-        for (int i = tid; i < kBK * kBN; i += 256) {
-            const int r = i / kBN, c = i % kBN;
-            const int gk = k0 + r, gn = n0 + c;
-            Bs[r][c] = (gk < g.k && gn < g.n) ? g.b[gk * g.sbk + gn * g.sbn] : (uint16_t)0;
-        }
+    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+        stage(g, m0, n0, k0, kend, As, Bs);
         __syncthreads();
         const int kq = 4 * (lane >> 4), li = lane & 15;
         bf16x4 af[2], bfr[2];
@@ -99,16 +120,17 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
+    const int mrows = g.ones_row >= 0 ? g.ones_row + 1 : g.m;
     const int rq = 4 * (lane >> 4), cl = lane & 15;
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int r = 0; r < 4; ++r) {
                 const int gm = m0 + wm + 16 * i + rq + r, gn = n0 + wn + 16 * j + cl;
-                if (gm >= g.m || gn >= g.n) continue;
+                if (gm >= mrows || gn >= g.n) continue;
                 const float v = acc[i][j][r];
                 const long long o = (long long)gm * g.n + gn;
                 if (g.epilogue == kStoreF32) {
-                    g.c[o] = v;
+                    g.c[(long long)blockIdx.z * mrows * g.n + o] = v;
                 } else if (g.epilogue == kBiasSilu) {
                     const float z = v + g.bias[gn];
                     g.c[o] = z;
@@ -117,6 +139,15 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
                     g.h[o] = f2bf(v * silu_grad(g.zref[o]));
                 }
             }
+}
+
+// out[o] = sum over splits of partial[split][o], in split order
+__global__ void split_reduce_kernel(const float* partial, int splits, long long n, float* out) {
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    float s = 0.f;
+    for (int z = 0; z < splits; ++z) s += partial[z * n + o];
+    out[o] = s;
 }
 
 // Fourier features of fractional positions (optionally gathered through idx), bf16 [rows][64]
@@ -161,33 +192,41 @@ __global__ void head_kernel(const uint16_t* h4, const float* z4, const float* w5
     }
 }
 
-// column sums (bias gradients) of a bf16 [rows][cols] matrix, or of an fp32 vector (cols=1)
-__global__ void colsum_bf16_kernel(const uint16_t* m, int rows, int cols, float* out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// Output-layer gradients over one chunk of 256 rows (one block per chunk): partial[chunk][k] =
+// sum_r H4[r][k] dy[r] for k < 256, then sum_r dy[r] and sum_r (y - t)^2.
+constexpr int kHeadChunk = 256;
+__global__ void head_grad_partial_kernel(const uint16_t* h4, const float* dy, const float* loss_rows, int rows,
+                                         float* partial) {
+    const int k = threadIdx.x, r0 = blockIdx.x * kHeadChunk;
+    const int r1 = min(rows, r0 + kHeadChunk);
     float s = 0.f;
-    for (int r = 0; r < rows; ++r) s += bf2f(m[(long long)r * cols + c]);
-    out[c] = s;
+    for (int r = r0; r < r1; ++r) s += bf2f(h4[(long long)r * kMlpHidden + k]) * dy[r];
+    float* out = partial + (long long)blockIdx.x * (kMlpHidden + 2);
+    out[k] = s;
+    if (k == 0) {
+        float sd = 0.f, sl = 0.f;
+        for (int r = r0; r < r1; ++r) {
+            sd += dy[r];
+            sl += loss_rows[r];
+        }
+        out[kMlpHidden] = sd;
+        out[kMlpHidden + 1] = sl;
+    }
 }
 
-// dW5[k] = sum_r H4[r][k] dy[r]; db5 = sum_r dy[r]; loss = mean of loss_rows
-__global__ void head_grad_kernel(const uint16_t* h4, const float* dy, const float* loss_rows, int rows, float* dw5,
-                                 float* db5, float* loss_out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < kMlpHidden) {
-        float s = 0.f;
-        for (int r = 0; r < rows; ++r) s += bf2f(h4[(long long)r * kMlpHidden + k]) * dy[r];
+// dW5, db5 and the mean loss from the chunk partials
+__global__ void head_grad_reduce_kernel(const float* partial, int chunks, int rows, float* dw5, float* db5,
+                                        float* loss_out) {
+    const int k = threadIdx.x;  // 0 .. 257
+    if (k >= kMlpHidden + 2) return;
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * (kMlpHidden + 2) + k];
+    if (k < kMlpHidden)
         dw5[k] = s;
-    }
-    if (k == kMlpHidden) {
-        float s = 0.f, l = 0.f;
-        for (int r = 0; r < rows; ++r) {
-            s += dy[r];
-            l += loss_rows[r];
-        }
+    else if (k == kMlpHidden)
         db5[0] = s;
-        if (loss_out) *loss_out = l / (float)rows;
-    }
+    else if (loss_out)
+        *loss_out = s / (float)rows;
 }
 
 // Adam on fp32 master weights; refreshes the bf16 copies of the GEMM weights
@@ -220,9 +259,14 @@ __global__ void sample_kernel(uint64_t seed, uint64_t stream, uint64_t first, in
     idx[r] = (uint32_t)(v < n ? v : n - 1);
 }
 
-void gemm(const GemmArgs& g, hipStream_t s) {
-    dim3 grid((g.n + kBN - 1) / kBN, (g.m + kBM - 1) / kBM);
+// Launches C = A B split into at most `splits` K ranges (multiples of kBK); returns the count used.
+int gemm(GemmArgs g, hipStream_t s, int splits = 1) {
+    const int mrows = g.ones_row >= 0 ? g.ones_row + 1 : g.m;
+    g.kchunk = ((g.k + splits - 1) / splits + kBK - 1) / kBK * kBK;
+    splits = (g.k + g.kchunk - 1) / g.kchunk;
+    dim3 grid((g.n + kBN - 1) / kBN, (mrows + kBM - 1) / kBM, splits);
     hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, s, g);
+    return splits;
 }
 
 }  // namespace
@@ -288,6 +332,8 @@ Mlp::Mlp(Context* c, int max_rows) : ctx(c), cap(max_rows) {
         alloc(&loss_rows, cap);
         alloc(&idx, cap);
         alloc(&loss_hist, kLossChunk);
+        alloc(&partial, (size_t)kMaxSplits * (kMlpHidden + 1) * kMlpHidden);
+        alloc(&head_partial, (size_t)((cap + kHeadChunk - 1) / kHeadChunk) * (kMlpHidden + 2));
         ALEATOR_HIP_CHECK(hipMemcpyAsync(kvec, kv.data(), kv.size(), hipMemcpyHostToDevice, ctx->stream));
         ALEATOR_HIP_CHECK(hipMemsetAsync(w, 0, kMlpParams * sizeof(float), ctx->stream));
         ALEATOR_HIP_CHECK(hipMemsetAsync(grad, 0, kMlpParams * sizeof(float), ctx->stream));
@@ -341,6 +387,7 @@ void Mlp::forward(const double* frac, const uint32_t* rows_idx, int rows) {
         g.c = z[l];
         g.bias = w + kMlpOffsets[2 * l + 1];
         g.h = h[l];
+        g.ones_row = -1;
         gemm(g, s);
         in = h[l];
         kin = kMlpHidden;
@@ -351,10 +398,15 @@ void Mlp::backward(const double* target, const uint32_t* rows_idx, int rows, flo
     hipStream_t s = ctx->stream;
     hipLaunchKernelGGL(head_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, h[3], z[3], w + kMlpOffsets[8],
                        w + kMlpOffsets[9], target, rows_idx, rows, y, dy, dz[3], loss_rows, 1);
-    hipLaunchKernelGGL(head_grad_kernel, dim3(2), dim3(256), 0, s, h[3], dy, loss_rows, rows, grad + kMlpOffsets[8],
-                       grad + kMlpOffsets[9], loss_slot);
+    const int chunks = (rows + kHeadChunk - 1) / kHeadChunk;
+    hipLaunchKernelGGL(head_grad_partial_kernel, dim3(chunks), dim3(kMlpHidden), 0, s, h[3], dy, loss_rows, rows,
+                       head_partial);
+    hipLaunchKernelGGL(head_grad_reduce_kernel, dim3(1), dim3(kMlpHidden + 64), 0, s, head_partial, chunks, rows,
+                       grad + kMlpOffsets[8], grad + kMlpOffsets[9], loss_slot);
+    const int splits = std::max(1, std::min(kMaxSplits, rows / 256));
     for (int l = 3; l >= 0; --l) {
-        // dW_l = H_{l-1}^T dZ_l   ([kin x rows] x [rows x 256])
+        // [dW_l; db_l] = [H_{l-1}^T; 1^T] dZ_l   ([kin + 1 x rows] x [rows x 256]), split over rows;
+        // W_l and b_l are adjacent in the parameter vector, so the result lands on both
         const uint16_t* hin = l == 0 ? x : h[l - 1];
         const int kin = l == 0 ? kMlpIn : kMlpHidden;
         GemmArgs gw{};
@@ -368,10 +420,17 @@ void Mlp::backward(const double* target, const uint32_t* rows_idx, int rows, flo
         gw.n = kMlpHidden;
         gw.k = rows;
         gw.epilogue = kStoreF32;
-        gw.c = grad + kMlpOffsets[2 * l];
-        gemm(gw, s);
-        hipLaunchKernelGGL(colsum_bf16_kernel, dim3(1), dim3(256), 0, s, dz[l], rows, kMlpHidden,
-                           grad + kMlpOffsets[2 * l + 1]);
+        gw.ones_row = kin;
+        const long long n_out = (long long)(kin + 1) * kMlpHidden;
+        if (splits == 1) {
+            gw.c = grad + kMlpOffsets[2 * l];
+            gemm(gw, s);
+        } else {
+            gw.c = partial;
+            const int used = gemm(gw, s, splits);
+            hipLaunchKernelGGL(split_reduce_kernel, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, s, partial,
+                               used, n_out, grad + kMlpOffsets[2 * l]);
+        }
         if (l == 0) break;
         // dZ_{l-1} = (dZ_l W_l^T) * silu'(Z_{l-1})   ([rows x 256] x [256 x 256])
         GemmArgs gd{};
@@ -387,6 +446,7 @@ void Mlp::backward(const double* target, const uint32_t* rows_idx, int rows, flo
         gd.epilogue = kSiluGrad;
         gd.h = dz[l - 1];
         gd.zref = z[l - 1];
+        gd.ones_row = -1;
         gemm(gd, s);
     }
 }

@@ -11,6 +11,7 @@ namespace aleator_gpu {
 constexpr int kMlpIn = 64;       // 32 frequency vectors x (sin, cos)
 constexpr int kMlpFreqs = 32;
 constexpr int kMlpHidden = 256;  // 4 hidden layers
+constexpr int kMaxSplits = 32;      // row splits of the weight-gradient GEMMs
 constexpr int kLossChunk = 1 << 16;  // iterations whose losses are kept on the device at once
 constexpr int kMlpParams =kMlpIn * kMlpHidden + kMlpHidden + 3 * (kMlpHidden * kMlpHidden + kMlpHidden) + kMlpHidden + 1;
 
@@ -39,6 +40,8 @@ struct Mlp {
     float* loss_rows = nullptr;
     uint32_t* idx = nullptr;
     float* loss_hist = nullptr;  // per-iteration loss of the current train call chunk
+    float* partial = nullptr;    // split-K partial weight gradients [kMaxSplits][257][256]
+    float* head_partial = nullptr;  // output-layer partial sums [row chunk][258]
     int adam_t = 0;
 
     Mlp(Context* c, int max_rows);
