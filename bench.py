@@ -59,9 +59,10 @@ def c1_workload(aa):
     x, y, z, q, sp = aa.synth_framework(m, 600, seed=1, stream=0, charge_sigma=0.5, charge_stream=2)
     eps = np.array([t[0] for t in C0_TYPES] + [27.0, 79.0])
     sig = np.array([t[1] for t in C0_TYPES] + [2.80, 3.05])
-    sm, sx, sy, sz, sq, ss = replicate(m, x, y, z, q, sp, (2, 2, 2))
+    reps = aa.supercell_replication(m, 12.0)  # (2, 2, 2)
+    sm, sx, sy, sz, sq, ss = replicate(m, x, y, z, q, sp, reps)
     return dict(prim=m, sup=sm, x=sx, y=sy, z=sz, q=sq, sp=ss, eps=eps, sig=sig, rc=12.0,
-                alpha=0.265, kmax=8, counts=aa.grid_point_counts(m, 0.15))
+                alpha=0.265, kmax=8, counts=aa.grid_point_counts(m, 0.15), reps=reps)
 
 
 def c0_workload(aa):
@@ -81,6 +82,86 @@ def timed(fn, steps, warmup, sync):
         fn()
     sync()
     return (time.perf_counter() - t0) / steps
+
+
+def c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=1_000_000, learned_max=600_000, seg=20_000):
+    """Learned-density GCMC against plain uniform insertion on the 64 c3 chains (8 pressures x 8
+    seeds), both from empty cells.  The plain method judges the loading: its 8-seed mean loading per
+    pressure over the last segment of its run (plain_steps) with that mean's standard error sigma.
+    A method's time is the GPU-seconds until its segment-mean loading first reaches
+    (plain final - sigma) at every pressure; the learned time includes the density's construction
+    (two LJ grid tables, 2e6-sample training set, MLP training, bin table)."""
+    from aleator_amd import learned, workloads as wl
+
+    chains = wl.c3_chains()
+    nb = (64, 64, 64)
+
+    def once(fn):
+        sync()
+        t0 = time.perf_counter()
+        r = fn()
+        sync()
+        return r, time.perf_counter() - t0
+
+    tabs, t_grid = once(lambda: fw.grid([4, 5], w["counts"], precision=aa.MIXED, grid_lattice=w["prim"]))
+    frac, e = learned.sample_grid(tabs, [1.0, 2.0], w["counts"], 2_000_000, low_fraction=0.5, low_cut=1000.0)
+    ld = learned.LearnedDensity(ctx, max_rows=8192)
+    losses, t_train = once(lambda: ld.fit(frac, e, e_clip=1000.0, n_iters=3000, batch=4096, lr=1e-3))
+    wts, t_table = once(lambda: ld.bin_weights(nb, 298.0))
+    t_density = t_grid + t_train + t_table
+
+    def trajectory(weights, max_steps, target=None, sigma=None):
+        g = aa.Gcmc(fw, wl.CO2_SITES, chains, max_molecules=1024)
+        if weights is not None:
+            g.set_proposal(weights, nb, w["reps"], 0.05)
+        segs, total, att, acc = [], 0.0, 0, 0
+        for _ in range(max_steps // seg):
+            g.reset_stats()
+            _, dt = once(lambda: g.run(seg))
+            total += dt
+            st = g.stats()
+            n = np.array([s["sum_molecules"] / s["steps"] for s in st]).reshape(8, 8)
+            att += sum(s["attempted"][2] for s in st)
+            acc += sum(s["accepted"][2] for s in st)
+            segs.append((total, n.mean(axis=1), n.std(axis=1, ddof=1) / np.sqrt(8)))
+            if target is not None and np.all(segs[-1][1] >= target - sigma):
+                break
+        errors = sum(s["error"] for s in g.stats())
+        g.close()
+        return segs, acc / max(att, 1), errors
+
+    plain, plain_acc, e1 = trajectory(None, plain_steps)
+    target, sigma = plain[-1][1], plain[-1][2]
+    t_plain = next(t for t, m, _ in plain if np.all(m >= target - sigma))
+    lrn, lrn_acc, e2 = trajectory(wts, learned_max, target, sigma)
+    reached = bool(np.all(lrn[-1][1] >= target - sigma))
+    t_learned = lrn[-1][0] + t_density if reached else None
+    return {
+        "c4_density": {"train_samples": 2_000_000, "adam_steps": 3000, "batch": 4096,
+                       "final_loss_std_units": float(losses[-100:].mean()), "bins": list(nb),
+                       "gpu_s_grid_train_table": [t_grid, t_train, t_table]},
+        "c4_plain_steps": len(plain) * seg, "c4_plain_gpu_s": plain[-1][0],
+        "c4_plain_final_loading": target.round(3).tolist(), "c4_plain_final_sigma": sigma.round(3).tolist(),
+        "c4_plain_gpu_s_to_loading": t_plain,
+        "c4_learned_steps": len(lrn) * seg, "c4_learned_loading": lrn[-1][1].round(3).tolist(),
+        "c4_learned_gpu_s_to_loading_incl_training": t_learned,
+        "c4_speedup_gpu_s": (t_plain / t_learned) if t_learned else None,
+        "c4_insertion_acceptance": {"plain": plain_acc, "learned": lrn_acc},
+        "c4_errors": e1 + e2,
+    }
+
+
+def measured_traffic(tile_shape):
+    """HBM bytes per launch of the headline tile kernel, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a device copy of known size), when it was taken with
+    the tile shape this run uses; else None."""
+    import re
+    try:
+        with open(os.path.join(ROOT, "profiles", "grid_tile_kernel_c1_co2_3tables.md")) as f:
+            m = re.search(r"traffic_bytes_per_launch: (\d+) \(tile shape (\d+)\)", f.read())
+    except OSError:
+        return None
+    return float(m.group(1)) if m and int(m.group(2)) == tile_shape else None
 
 
 def cpu_model():
@@ -141,6 +222,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="only the headline build (profiling runs)")
+    ap.add_argument("--no-graph", action="store_true", help="launch the headline plan without its hipGraph")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (reference kernel for HBM byte counters)")
+    ap.add_argument("--c4-plain-steps", type=int, default=1_000_000)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -166,7 +253,7 @@ def main():
     # the build is planned once (constants uploaded, tile shape autotuned, launches captured in a
     # hipGraph); a step replays it: device-resident input to device-resident tables
     plan = aa.GridPlan(ctx, [fw], [4, 5], [w["counts"]], [out.data_ptr()], coulomb=True, precision=prec,
-                       grid_lattices=[w["prim"]], tile=-1, use_graph=True)
+                       grid_lattices=[w["prim"]], tile=-1, use_graph=not args.no_graph)
     tile, tuning = plan.tile()
     step = plan.execute
     for _ in range(args.warmup):
@@ -185,77 +272,94 @@ def main():
     # ---------------- extras
     extra = {"nodes_per_s": nodes / t_step, "tile_kernel_ms": k_avg * 1e3, "pairs_per_node": nbar,
              "tile_shape": tile, "autotune_ms_per_shape": tuning}
-    t = timed(lambda: fw.grid_device([4, 5], w["counts"], out.data_ptr(), coulomb=True, precision=prec,
-                                     grid_lattice=w["prim"]), 5, 1, sync)
-    extra["c1_grid_per_call_entry_point_pts_per_s"] = 3 * nodes / t
-    c0 = c0_workload(aa)
-    fw0 = aa.Framework(ctx, c0["m"], c0["x"], c0["y"], c0["z"], c0["sp"], c0["eps"], c0["sig"], c0["rc"])
-    n0 = int(np.prod(c0["counts"]))
-    out0 = torch.empty(n0, dtype=torch.float64, device="cuda")
-    for p, name in ((aa.MIXED, "mixed"), (aa.FP64, "fp64")):
-        t = timed(lambda: fw0.grid_device([4], c0["counts"], out0.data_ptr(), precision=p), 10, 2, sync)
-        extra[f"c0_ch4_grid_129^3_{name}_pts_per_s"] = n0 / t
-    du = torch.empty(10_000_000, dtype=torch.float64, device="cuda")
-    sm = torch.empty(1, dtype=torch.float64, device="cuda")
-    ch4 = [(0.0, 0.0, 0.0, 4, 0.0)]
-    for n_trials in (100_000, 10_000_000):
+    if args.copy_calibration:
+        # a plain device copy of known size, for scaling HBM byte counters (FETCH_SIZE, WRITE_SIZE)
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        sync()
+        extra["copy_calibration_bytes_each_way"] = src.numel() * 4
+    if not args.headline_only:
+        t = timed(lambda: fw.grid_device([4, 5], w["counts"], out.data_ptr(), coulomb=True, precision=prec,
+                                         grid_lattice=w["prim"]), 5, 1, sync)
+        extra["c1_grid_per_call_entry_point_pts_per_s"] = 3 * nodes / t
+        c0 = c0_workload(aa)
+        fw0 = aa.Framework(ctx, c0["m"], c0["x"], c0["y"], c0["z"], c0["sp"], c0["eps"], c0["sig"], c0["rc"])
+        n0 = int(np.prod(c0["counts"]))
+        out0 = torch.empty(n0, dtype=torch.float64, device="cuda")
         for p, name in ((aa.MIXED, "mixed"), (aa.FP64, "fp64")):
-            t = timed(lambda: fw0.widom_device(ch4, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
-                                               precision=p), 5, 1, sync)
-            extra[f"c0_widom_ch4_{n_trials}_{name}_inserts_per_s"] = n_trials / t
-    co2 = [(0.0, 0.0, -1.16, 5, -0.35), (0.0, 0.0, 0.0, 4, 0.70), (0.0, 0.0, 1.16, 5, -0.35)]
-    n_trials = 100_000
-    t = timed(lambda: fw.widom_device(co2, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
-                                      precision=prec), 3, 1, sync)
-    extra[f"c1_widom_co2_{n_trials}_{args.precision}_inserts_per_s"] = n_trials / t
+            t = timed(lambda: fw0.grid_device([4], c0["counts"], out0.data_ptr(), precision=p), 10, 2, sync)
+            extra[f"c0_ch4_grid_129^3_{name}_pts_per_s"] = n0 / t
+        du = torch.empty(10_000_000, dtype=torch.float64, device="cuda")
+        sm = torch.empty(1, dtype=torch.float64, device="cuda")
+        ch4 = [(0.0, 0.0, 0.0, 4, 0.0)]
+        for n_trials in (100_000, 10_000_000):
+            for p, name in ((aa.MIXED, "mixed"), (aa.FP64, "fp64")):
+                t = timed(lambda: fw0.widom_device(ch4, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
+                                                   precision=p), 5, 1, sync)
+                extra[f"c0_widom_ch4_{n_trials}_{name}_inserts_per_s"] = n_trials / t
+        co2 = [(0.0, 0.0, -1.16, 5, -0.35), (0.0, 0.0, 0.0, 4, 0.70), (0.0, 0.0, 1.16, 5, -0.35)]
+        n_trials = 100_000
+        t = timed(lambda: fw.widom_device(co2, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
+                                          precision=prec), 3, 1, sync)
+        extra[f"c1_widom_co2_{n_trials}_{args.precision}_inserts_per_s"] = n_trials / t
 
-    from aleator_amd import workloads as wl
+        from aleator_amd import workloads as wl
 
-    # c2: 256 frameworks, one batched (graph-captured, autotuned) launch per step, FP32 pair terms
-    try:
-        fws2, eps2, sig2 = wl.c2(256)
-        devs = [aa.Framework(ctx, f["sup"], f["x"], f["y"], f["z"], f["sp"], eps2, sig2, f["rc"]) for f in fws2]
-        counts2 = [f["counts"] for f in fws2]
-        nodes2 = [int(np.prod(c)) for c in counts2]
-        buf2 = torch.empty(sum(nodes2), dtype=torch.float64, device="cuda")
-        offs = np.concatenate([[0], np.cumsum(nodes2)[:-1]])
-        ptrs = [buf2.data_ptr() + 8 * int(o) for o in offs]
-        plan2 = aa.GridPlan(ctx, devs, [4], counts2, ptrs, precision=aa.MIXED,
-                            grid_lattices=[f["prim"] for f in fws2], tile=-1, use_graph=True)
-        t = timed(plan2.execute, 3, 1, sync)
-        extra["c2_batched_256_frameworks_pts_per_s"] = sum(nodes2) / t
-        extra["c2_nodes"] = sum(nodes2)
-        extra["c2_ms_per_batch"] = t * 1e3
-        extra["c2_tile_shape"] = plan2.tile()[0]
-        plan2.close()
-        del buf2, devs
-    except Exception as e:  # report, do not hide the headline
-        extra["c2_error"] = repr(e)
+        # c2: 256 frameworks, one batched (graph-captured, autotuned) launch per step, FP32 pair terms
+        try:
+            fws2, eps2, sig2 = wl.c2(256)
+            devs = [aa.Framework(ctx, f["sup"], f["x"], f["y"], f["z"], f["sp"], eps2, sig2, f["rc"]) for f in fws2]
+            counts2 = [f["counts"] for f in fws2]
+            nodes2 = [int(np.prod(c)) for c in counts2]
+            buf2 = torch.empty(sum(nodes2), dtype=torch.float64, device="cuda")
+            offs = np.concatenate([[0], np.cumsum(nodes2)[:-1]])
+            ptrs = [buf2.data_ptr() + 8 * int(o) for o in offs]
+            plan2 = aa.GridPlan(ctx, devs, [4], counts2, ptrs, precision=aa.MIXED,
+                                grid_lattices=[f["prim"] for f in fws2], tile=-1, use_graph=True)
+            t = timed(plan2.execute, 3, 1, sync)
+            extra["c2_batched_256_frameworks_pts_per_s"] = sum(nodes2) / t
+            extra["c2_nodes"] = sum(nodes2)
+            extra["c2_ms_per_batch"] = t * 1e3
+            extra["c2_tile_shape"] = plan2.tile()[0]
+            plan2.close()
+            del buf2, devs
+        except Exception as e:  # report, do not hide the headline
+            extra["c2_error"] = repr(e)
 
-    # c3: 64 CO2 GCMC chains on the charged framework, one persistent launch
-    try:
-        chains = wl.c3_chains()
-        gc = aa.Gcmc(fw, wl.CO2_SITES, chains, max_molecules=1024)
-        gc.run(500)
-        sync()
-        gc.reset_stats()
-        n_steps = 2000
-        t0 = time.perf_counter()
-        gc.run(n_steps)
-        sync()
-        dt = time.perf_counter() - t0
-        st = gc.stats()
-        extra["c3_chain_steps_per_s"] = len(chains) * n_steps / dt
-        extra["c3_gpu_seconds_per_1000_steps_all_chains"] = dt / n_steps * 1000
-        extra["c3_mean_N_by_pressure"] = [float(np.mean([st[8 * i + s]["sum_molecules"] / st[8 * i + s]["steps"]
-                                                         for s in range(8)])) for i in range(8)]
-        extra["c3_errors"] = sum(s["error"] for s in st)
-        gc.close()
-    except Exception as e:
-        extra["c3_error"] = repr(e)
+        # c3: 64 CO2 GCMC chains on the charged framework, one persistent launch
+        try:
+            chains = wl.c3_chains()
+            gc = aa.Gcmc(fw, wl.CO2_SITES, chains, max_molecules=1024)
+            # 2e4 cycles per chain, a cycle taken as one move attempt (the reference's run() counts
+            # steps); one persistent launch advances all 64 chains
+            n_steps = 20000
+            t0 = time.perf_counter()
+            gc.run(n_steps)
+            sync()
+            dt = time.perf_counter() - t0
+            st = gc.stats()
+            extra["c3_chain_steps_per_s"] = len(chains) * n_steps / dt
+            extra["c3_gpu_s_2e4_steps_all_chains"] = dt
+            extra["c3_insertion_acceptance"] = (sum(s["accepted"][2] for s in st) /
+                                                max(1, sum(s["attempted"][2] for s in st)))
+            extra["c3_mean_N_by_pressure"] = [float(np.mean([st[8 * i + s]["sum_molecules"] / st[8 * i + s]["steps"]
+                                                             for s in range(8)])) for i in range(8)]
+            extra["c3_errors"] = sum(s["error"] for s in st)
+            gc.close()
+        except Exception as e:
+            extra["c3_error"] = repr(e)
+
+        # c4: learned insertion density vs plain uniform insertion on the c3 chains
+        if not args.no_c4:
+            try:
+                extra.update(c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=args.c4_plain_steps))
+            except Exception as e:
+                extra["c4_error"] = repr(e)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.headline_only:
         cpu = cpu_baseline_c1(w)
         extra["speedup_vs_cpu_baseline"] = value / cpu["value"]
 
@@ -277,7 +381,7 @@ def main():
                                "Ewald alpha 0.265 kmax 8 (2204 k), rc 12 A",
                    "nodes": nodes, "tables": 3, "precision": args.precision},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(tile),
                      "kernel": "grid_tile_kernel<float,2,true>",
                      "flop_per_launch": flop},
         "cpu_baseline": cpu,
