@@ -5,10 +5,10 @@ generator (aleator_synth_framework) and host utilities only.
   c1  rhombohedral 24 A 60/60/60, 600 charged atoms -> 2x2x2 supercell (4800), CO2 probe,
       Ewald alpha 0.265 kmax 8 rc 12 A, 0.15 A grid over the primitive cell (131^3)
   c2  256 frameworks: per framework f, Philox4x32Rng(seed 2, stream 2f) draws the cell (edges
-      U(10, 40) A; orthorhombic when the first draw < 0.5, else angles U(60, 120) deg, redrawn
-      until the metric is valid and every angle between rows is in range), then a number density
-      U(0.03, 0.10) A^-3 sets N = clamp(round(rho V), 50, 2000); atoms from stream 2f+1; the
-      minimum supercell for rc 12.8 A; CH4 probe; 0.2 A grid over the primitive cell
+      U(10, 40) A; orthorhombic when the fourth draw < 0.5, else angles U(60, 120) deg; the whole
+      cell is redrawn until its metric is valid), then a number density U(0.03, 0.10) A^-3 sets
+      N = clamp(round(rho V), 50, 2000); atoms from stream 2f+1; the minimum supercell for
+      rc 12.8 A; CH4 probe; 0.2 A grid over the primitive cell
   c3  64 CO2 chains on the c1 framework at 298 K: 8 pressures 0.1 * 300^(i/7) bar x 8 seeds,
       fugacity from Peng-Robinson (CO2: Tc 304.1282 K, Pc 7377300 Pa, omega 0.22394)
 """
