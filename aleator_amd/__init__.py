@@ -115,6 +115,63 @@ def grid_point_counts(lattice, spacing: float) -> tuple[int, int, int]:
     return tuple(int(v) for v in n)
 
 
+class _CacheKey(C.Structure):
+    _fields_ = [("lattice", C.POINTER(_Lattice)), ("atoms", C.POINTER(_Particles)), ("lj", C.POINTER(_LJ)),
+                ("guest_species", _u32p), ("n_guest", C.c_int32), ("spacing", C.c_double)]
+
+
+class GridCacheKey:
+    """makeFrameworkEnergyGridCacheKey's inputs (framework_energy_grid.cc:171-191): Cartesian
+    atoms and species as given, lattice, per-species (epsilon, sigma), cutoff, truncation,
+    guest species, spacing."""
+
+    def __init__(self, lattice, x, y, z, species, epsilon, sigma, cutoff, truncation, guests, spacing):
+        self._keep = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z, epsilon, sigma)]
+        self._sp = np.ascontiguousarray(species, dtype=np.uint32)
+        self._g = np.ascontiguousarray(guests, dtype=np.uint32)
+        self._lat = _lattice(lattice)
+        xx, yy, zz, eps, sig = self._keep
+        self._p = _Particles(_p(xx), _p(yy), _p(zz), None, _p(self._sp, _u32p), len(xx))
+        self._lj = _LJ(_p(eps), _p(sig), len(eps), float(cutoff), int(truncation), 0)
+        self.key = _CacheKey(C.pointer(self._lat), C.pointer(self._p), C.pointer(self._lj),
+                             _p(self._g, _u32p) if len(self._g) else None, len(self._g), float(spacing))
+
+    def hash(self) -> int:
+        h = C.c_uint64()
+        _check(_load().aleator_grid_cache_key_hash(C.byref(self.key), C.byref(h)))
+        return h.value
+
+    def file_path(self, directory: str) -> str:
+        buf = C.create_string_buffer(4096)
+        _check(_load().aleator_grid_cache_file_path(directory.encode(), C.byref(self.key), buf, C.c_uint64(4096)))
+        return buf.value.decode()
+
+
+def grid_cache_serialize(path: str, key: GridCacheKey, n_points, species, tables):
+    """FrameworkEnergyGrid::serialize: tables [n_tables][nz][ny][nx], one per species."""
+    tables = np.ascontiguousarray(tables, dtype=np.float64)
+    sp = np.ascontiguousarray(species, dtype=np.uint32)
+    n = np.ascontiguousarray(n_points, dtype=np.int32)
+    _check(_load().aleator_grid_cache_serialize(path.encode(), C.byref(key.key), _p(n, _i32p), _p(sp, _u32p),
+                                                C.c_int32(len(sp)), _p(tables)))
+
+
+def grid_cache_load(directory: str, key: GridCacheKey):
+    """FrameworkEnergyGrid::loadFromCache: None on a miss, else (n_points, species, tables
+    [n_tables][nz][ny][nx]); RuntimeError for a mismatching or corrupt file."""
+    n = np.zeros(3, dtype=np.int32)
+    nt, hit = C.c_int32(), C.c_int32()
+    _check(_load().aleator_grid_cache_load(directory.encode(), C.byref(key.key), _p(n, _i32p), None, C.byref(nt),
+                                           None, C.c_uint64(0), C.byref(hit)))
+    if not hit.value:
+        return None
+    sp = np.zeros(max(nt.value, 1), dtype=np.uint32)
+    out = np.zeros((nt.value, n[2], n[1], n[0]))
+    _check(_load().aleator_grid_cache_load(directory.encode(), C.byref(key.key), _p(n, _i32p), _p(sp, _u32p),
+                                           C.byref(nt), _p(out), C.c_uint64(out.size), C.byref(hit)))
+    return tuple(int(v) for v in n), sp[:nt.value].copy(), out
+
+
 def philox_uniforms(seed: int, stream: int, first_call_index: int, n: int) -> np.ndarray:
     """Philox4x32Rng(seed, stream) uniforms at call indices first .. first+n-1."""
     out = np.zeros(n)
@@ -502,6 +559,26 @@ class Gcmc:
         out = C.c_double()
         _check(_load().aleator_gpu_gcmc_electrostatic_energy(self._h, C.c_int32(chain), C.byref(out)))
         return out.value
+
+
+def grid_interpolate_device(ctx: "Context", lattice, n_points, table_ptr: int, positions_ptr: int, n: int,
+                            out_ptr: int):
+    """FrameworkEnergyGrid::interpolate of one device table at n device positions [n][3]."""
+    lat = _lattice(lattice)
+    npts = np.ascontiguousarray(n_points, dtype=np.int32)
+    _check(_load().aleator_gpu_grid_interpolate(ctx.handle, C.byref(lat), _p(npts, _i32p), C.c_void_p(int(table_ptr)),
+                                                C.c_void_p(int(positions_ptr)), C.c_uint64(n),
+                                                C.c_void_p(int(out_ptr))))
+
+
+def grid_interpolate(ctx: "Context", lattice, n_points, table, positions) -> np.ndarray:
+    """Host convenience of grid_interpolate_device: table [nz][ny][nx], positions [n][3]."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    d_t = DeviceBuffer.from_array(np.ascontiguousarray(table, dtype=np.float64))
+    d_p = DeviceBuffer.from_array(pos)
+    d_o = DeviceBuffer(len(pos))
+    grid_interpolate_device(ctx, lattice, n_points, int(d_t), int(d_p), len(pos), int(d_o))
+    return d_o.to_host()[:len(pos)].copy()
 
 
 class Mlp:

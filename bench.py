@@ -232,8 +232,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the hot path shards by grid slab / trial range / chain with no exchange beyond a "
-              "final sum; see DESIGN.md)", file=sys.stderr)
+              "(the hot path would shard by grid slab, trial range or chain, with no exchange "
+              "beyond a final sum)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -183,6 +183,49 @@ aleator_status aleator_gpu_grid_plan_execute(aleator_grid_plan* plan);
 aleator_status aleator_gpu_grid_plan_tile(const aleator_grid_plan* plan, int32_t* tile, float* tuning_ms);
 aleator_status aleator_gpu_grid_plan_destroy(aleator_grid_plan* plan);
 
+/* ---- energy-grid cache file, format v1 -------------------------------------------------
+ * Byte-compatible with the reference's cache (framework_energy_grid.cc:31-191 key and
+ * helpers, :286-313 serialize, :315-322 cacheFilePath, :357-408 loadFromCache): grids built
+ * here load in the reference's engine and reference-built files load here.  The key is
+ * makeFrameworkEnergyGridCacheKey's (framework_energy_grid.cc:171-191): the framework atoms
+ * as given (Cartesian x, y, z and species; charges are not part of it), the lattice, every
+ * species' (epsilon, sigma), the cutoff, the truncation, the guest species (sorted and
+ * deduplicated here, as there) and the spacing; the mixing rule is not part of it, as in
+ * the reference. */
+typedef struct {
+    const aleator_lattice* lattice;
+    const aleator_particles* atoms;
+    const aleator_lj_params* lj;
+    const uint32_t* guest_species;
+    int32_t n_guest;
+    double spacing;
+} aleator_grid_cache_key;
+/* FrameworkEnergyGridCacheKey::hash (framework_energy_grid.cc:158-163): FNV-1a 64 of the key. */
+aleator_status aleator_grid_cache_key_hash(const aleator_grid_cache_key* key, uint64_t* hash);
+/* FrameworkEnergyGrid::cacheFilePath: directory/framework_energy_grid_v1_<hash hex>.bin. */
+aleator_status aleator_grid_cache_file_path(const char* directory, const aleator_grid_cache_key* key,
+                                            char* out, uint64_t capacity);
+/* FrameworkEnergyGrid::serialize: tables [n_tables][nz*ny*nx] (host), one per species,
+ * written in ascending species order; parent directories are created. */
+aleator_status aleator_grid_cache_serialize(const char* path, const aleator_grid_cache_key* key,
+                                            const int32_t n_points[3], const uint32_t* table_species,
+                                            int32_t n_tables, const double* tables);
+/* FrameworkEnergyGrid::loadFromCache for the file of `key` in `directory`: *hit = 0 when no
+ * such file exists (a miss, status OK); ALEATOR_RUNTIME_ERROR when it exists but its version
+ * or stored key differs, or it is truncated.  On a hit, n_points, n_tables and, when `tables`
+ * is not NULL (capacity in doubles), table_species and the tables in ascending species order. */
+aleator_status aleator_grid_cache_load(const char* directory, const aleator_grid_cache_key* key,
+                                       int32_t n_points[3], uint32_t* table_species, int32_t* n_tables,
+                                       double* tables, uint64_t capacity, int32_t* hit);
+/* FrameworkEnergyGrid::interpolate (framework_energy_grid.cc:251-284) on the device: trilinear
+ * interpolation of one table [nz][ny][nx] over the nodes (i/n) of `lattice` at n Cartesian
+ * positions (device, [n][3], any periodic image); writes n doubles to out_device on the
+ * context stream (asynchronous).  The reference's arithmetic, in its order. */
+aleator_status aleator_gpu_grid_interpolate(aleator_ctx* ctx, const aleator_lattice* lattice,
+                                            const int32_t n_points[3], const double* table_device,
+                                            const double* positions_device, uint64_t n,
+                                            double* out_device);
+
 /* ---- Widom insertion -------------------------------------------------- */
 /* Batched Widom test-particle insertion of a rigid molecule, as the reference loop
  * (monte_carlo_engine.cc:880-916): trial t takes uniform draws from

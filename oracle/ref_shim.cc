@@ -296,6 +296,42 @@ int ref_grid_build(uint32_t ns, const double* eps, const double* sig, double rc,
     });
 }
 
+// ---- FrameworkEnergyGrid cache: the reference's key hash, and its loader followed by its
+// interpolate at caller positions (a file that loads and interpolates bit-identically holds
+// the same key and tables). ----
+int ref_grid_cache_hash(uint32_t ns, const double* eps, const double* sig, double rc, int trunc, int mixing,
+                        const double* m, uint64_t n, const double* x, const double* y, const double* z,
+                        const uint32_t* species, const uint32_t* guests, int n_guest, double spacing,
+                        uint64_t* hash) {
+    return guarded([&] {
+        const auto lj = makeLj(ns, eps, sig, rc, trunc, mixing);
+        const auto p = makeParticles(n, x, y, z, nullptr, species);
+        const auto lat = makeLattice(m);
+        std::vector<uint32_t> g(guests, guests + n_guest);
+        *hash = engines::makeFrameworkEnergyGridCacheKey(*lj, p, lat, g, spacing).hash();
+    });
+}
+
+int ref_grid_cache_load_interpolate(uint32_t ns, const double* eps, const double* sig, double rc, int trunc,
+                                    int mixing, const double* m, uint64_t n, const double* x, const double* y,
+                                    const double* z, const uint32_t* species, const uint32_t* guests, int n_guest,
+                                    double spacing, const char* cache_dir, uint32_t guest, const double* pos,
+                                    uint64_t n_pos, double* out, int32_t* counts, int* hit) {
+    return guarded([&] {
+        const auto lj = makeLj(ns, eps, sig, rc, trunc, mixing);
+        const auto p = makeParticles(n, x, y, z, nullptr, species);
+        const auto lat = makeLattice(m);
+        std::vector<uint32_t> g(guests, guests + n_guest);
+        const auto key = engines::makeFrameworkEnergyGridCacheKey(*lj, p, lat, g, spacing);
+        const auto grid = engines::FrameworkEnergyGrid::loadFromCache(cache_dir, key);
+        *hit = grid.has_value() ? 1 : 0;
+        if (!grid) return;
+        const auto pc = grid->pointCounts();
+        for (int a = 0; a < 3; ++a) counts[a] = pc[a];
+        for (uint64_t i = 0; i < n_pos; ++i) out[i] = grid->interpolate(guest, {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
+    });
+}
+
 // Widom through the reference engine: returns K_H.
 int ref_widom(uint32_t ns, const double* eps, const double* sig, double rc, int trunc, int mixing,
               const double* m, uint64_t n, const double* x, const double* y, const double* z,
