@@ -95,3 +95,38 @@ extern "C" aleator_status aleator_gpu_grid_interpolate(aleator_ctx* ctx_in, cons
         ALEATOR_HIP_CHECK(hipGetLastError());
     });
 }
+
+extern "C" aleator_status aleator_gpu_grid_interpolate_host(aleator_ctx* ctx_in, const aleator_lattice* lattice,
+                                                            const int32_t n_points[3], const double* table,
+                                                            const double* positions, uint64_t n, double* out) {
+    return guarded([&] {
+        auto* ctx = reinterpret_cast<Context*>(ctx_in);
+        require(ctx && lattice && n_points && table && (n == 0 || (positions && out)), ALEATOR_INVALID_ARGUMENT,
+                "null argument");
+        for (int i = 0; i < 3; ++i)
+            require(n_points[i] >= 1, ALEATOR_INVALID_ARGUMENT, "interpolate: point counts must be >= 1");
+        if (n == 0) return;
+        const size_t nodes = (size_t)n_points[0] * n_points[1] * n_points[2];
+        double *dt = nullptr, *dp = nullptr, *dout = nullptr;
+        auto release = [&] {
+            for (double* p : {dt, dp, dout})
+                if (p) (void)hipFree(p);
+        };
+        try {
+            ALEATOR_HIP_CHECK(hipMalloc(&dt, nodes * sizeof(double)));
+            ALEATOR_HIP_CHECK(hipMalloc(&dp, 3 * n * sizeof(double)));
+            ALEATOR_HIP_CHECK(hipMalloc(&dout, n * sizeof(double)));
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(dt, table, nodes * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(dp, positions, 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            const aleator_status st = aleator_gpu_grid_interpolate(ctx_in, lattice, n_points, dt, dp, n, dout);
+            if (st != ALEATOR_OK) fail(st, last_error_slot());
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(out, dout, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            ALEATOR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(ctx->stream);
+            release();
+            throw;
+        }
+        release();
+    });
+}

@@ -225,6 +225,10 @@ aleator_status aleator_gpu_grid_interpolate(aleator_ctx* ctx, const aleator_latt
                                             const int32_t n_points[3], const double* table_device,
                                             const double* positions_device, uint64_t n,
                                             double* out_device);
+/* Same with host buffers (uploads the table and positions, synchronises). */
+aleator_status aleator_gpu_grid_interpolate_host(aleator_ctx* ctx, const aleator_lattice* lattice,
+                                                 const int32_t n_points[3], const double* table,
+                                                 const double* positions, uint64_t n, double* out);
 
 /* ---- Widom insertion -------------------------------------------------- */
 /* Batched Widom test-particle insertion of a rigid molecule, as the reference loop
