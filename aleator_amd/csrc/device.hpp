@@ -94,6 +94,39 @@ __device__ inline double block_sum(double v, double* lds) {
     return s;  // valid in thread 0
 }
 
+// ---------------------------------------------------------------- erfc(alpha r) / r
+template <typename R>
+__device__ inline R erfc_over_r(R alpha, R r);
+template <>
+__device__ inline double erfc_over_r<double>(double alpha, double r) {
+    return erfc(alpha * r) / r;
+}
+// FP32 erfc(z)/r without branches: erfc(z) = t exp(-z^2 + P(t)), t = 1/(1 + z/2), with the
+// degree-9 Chebyshev fit P of Press et al., Numerical Recipes in C (2nd ed.) section 6.2
+// ("erfcc", fractional error below 1.2e-7 for every z >= 0).  -z^2 is split as
+// -zh^2 - (z - zh)(z + zh) with zh = z rounded down to 12 significant bits, so zh^2 is exact
+// in FP32 and exp of the large part carries no argument rounding (the splitting fdlibm's
+// erf uses); the large exponent goes through the accurate expf, the small one (|.| < 2)
+// through the hardware exponential.  FP32 evaluation against the FP64 erfc: relative error
+// below 6e-7 on [0, 4].
+template <>
+__device__ inline float erfc_over_r<float>(float alpha, float r) {
+    const float z = alpha * r;
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * z);
+    float p = 0.17087277f;
+    p = -0.82215223f + t * p;
+    p = 1.48851587f + t * p;
+    p = -1.13520398f + t * p;
+    p = 0.27886807f + t * p;
+    p = -0.18628806f + t * p;
+    p = 0.09678418f + t * p;
+    p = 0.37409196f + t * p;
+    p = 1.00002368f + t * p;
+    const float zh = __uint_as_float(__float_as_uint(z) & 0xfffff000u);
+    const float small = -(z - zh) * (z + zh) + (-1.26551223f + t * p);
+    return t * expf(-zh * zh) * __expf(small) * __builtin_amdgcn_rcpf(r);
+}
+
 // ---------------------------------------------------------------- complex helpers
 struct cplx {
     double re, im;

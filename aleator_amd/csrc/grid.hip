@@ -60,17 +60,6 @@ __device__ inline float rcp<float>(float v) {
     return __builtin_amdgcn_rcpf(v);
 }
 
-template <typename R>
-__device__ inline R erfc_over_r(R alpha, R r);
-template <>
-__device__ inline double erfc_over_r<double>(double alpha, double r) {
-    return erfc(alpha * r) / r;
-}
-template <>
-__device__ inline float erfc_over_r<float>(float alpha, float r) {
-    return erfcf(alpha * r) * __builtin_amdgcn_rcpf(r);
-}
-
 template <typename R, int NG, bool COUL, int TX, int TY, int TZ, int NPT>
 __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __restrict__ descs,
                                                              const int* __restrict__ prefix, int n_desc) {

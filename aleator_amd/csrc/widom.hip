@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kWThreads) void widom_pair_kernel(const WidomArgs a
                 }
                 if (CHARGED && r2 <= rcr2) {
                     const R r = sqrt(r2);
-                    real += (double)((R)(a.charge[s] * sq[j]) * (R)erfc(alpha * r) / r);
+                    real += (double)((R)(a.charge[s] * sq[j]) * erfc_over_r<R>(alpha, r));
                 }
             }
         }
