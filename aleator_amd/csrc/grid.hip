@@ -67,6 +67,7 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
     constexpr int NZ = TZ * NPT;  // nodes per tile along z
     __shared__ Entries<R> ent;
     __shared__ double tab[4][kMaxGuests][kMaxTypes];
+    __shared__ R tabr[2][kMaxGuests][kMaxTypes];  // sigma^2 and 4 eps in the pair precision
     __shared__ int scan_tmp[kThreads / 64 + 1];
 
     // framework of this workgroup
@@ -81,7 +82,10 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
     const int tile = b - prefix[lo];
 
     const int tid = threadIdx.x;
-    for (int i = tid; i < 4 * kMaxGuests * kMaxTypes; i += kThreads) (&tab[0][0][0])[i] = a.ljtab[i];
+    for (int i = tid; i < 4 * kMaxGuests * kMaxTypes; i += kThreads) {
+        (&tab[0][0][0])[i] = a.ljtab[i];
+        if (i < 2 * kMaxGuests * kMaxTypes) (&tabr[0][0][0])[i] = (R)a.ljtab[i];
+    }
 
     const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
     const int t0 = (n0 + TX - 1) / TX, t1 = (n1 + TY - 1) / TY;
@@ -129,14 +133,15 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
     auto compute = [&](int count) {
         __syncthreads();
         if (any_valid) {
+#pragma unroll 2
             for (int e = 0; e < count; ++e) {
                 const R ex = ent.x[e], ey = ent.y[e], ez = ent.z[e];
                 const int t = ent.t[e];
                 R s2c[NG > 0 ? NG : 1], e4c[NG > 0 ? NG : 1];
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
-                    s2c[g] = (R)tab[0][g][t];
-                    e4c[g] = (R)tab[1][g][t];
+                    s2c[g] = tabr[0][g][t];
+                    e4c[g] = tabr[1][g][t];
                 }
 #pragma unroll
                 for (int p = 0; p < NPT; ++p) {
