@@ -190,15 +190,29 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
             h0 -= floor(h0);
             h1 -= floor(h1);
             h2 -= floor(h2);
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
+            if (a.single_image) {
+                // the search radius is below L_perp/2: only the nearest image (fractional
+                // offsets in [-1/2, 1/2)) can be within it
+                const int s = (h0 >= 0.5 ? 1 : 0) | (h1 >= 0.5 ? 2 : 0) | (h2 >= 0.5 ? 4 : 0);
                 const double g0 = h0 - (s & 1), g1 = h1 - ((s >> 1) & 1), g2 = h2 - (s >> 2);
                 const double cx = g0 * a.mi[0] + g1 * a.mi[3] + g2 * a.mi[6];
                 const double cy = g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7];
                 const double cz = g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8];
                 if (cx * cx + cy * cy + cz * cz <= a.search2) {
-                    mask |= 1u << s;
-                    ++cnt;
+                    mask = 1u << s;
+                    cnt = 1;
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    const double g0 = h0 - (s & 1), g1 = h1 - ((s >> 1) & 1), g2 = h2 - (s >> 2);
+                    const double cx = g0 * a.mi[0] + g1 * a.mi[3] + g2 * a.mi[6];
+                    const double cy = g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7];
+                    const double cz = g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8];
+                    if (cx * cx + cy * cy + cz * cz <= a.search2) {
+                        mask |= 1u << s;
+                        ++cnt;
+                    }
                 }
             }
         }
@@ -486,6 +500,9 @@ void GridJob::set_shape(int s) {
         fail(ALEATOR_INVALID_ARGUMENT, "grid: cutoff plus tile radius exceeds the perpendicular width");
     const double sr = rmax + tile_radius(s) + 1e-6 * (1.0 + rmax);
     args.search2 = sr * sr;
+    // below half the smallest perpendicular width at most one image of an atom, the nearest,
+    // is within the search radius (the fractional rounding image is then the nearest one)
+    args.single_image = (2.0 * sr < std::min(lperp[0], std::min(lperp[1], lperp[2]))) ? 1 : 0;
     blocks = tile_blocks(args.n, s);
     shape = s;
 }

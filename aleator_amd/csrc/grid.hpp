@@ -22,6 +22,7 @@ struct GridArgs {
     const int32_t* type;
     int natoms;
     double search2;
+    int single_image;  // search radius < L_perp/2: test only the nearest image of each atom
     double rc_lj, rc2_lj;
     double rc2_real, alpha, kc;
     double cap;
