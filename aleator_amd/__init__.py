@@ -608,6 +608,10 @@ class Mlp:
     def init(self, seed: int):
         _check(_load().aleator_gpu_mlp_init(self._h, C.c_uint64(seed)))
 
+    def use_graph(self, enable: bool):
+        """Replay training iterations from one captured hipGraph (default) or launch each kernel."""
+        _check(_load().aleator_gpu_mlp_use_graph(self._h, C.c_int32(1 if enable else 0)))
+
     @property
     def params(self) -> np.ndarray:
         out = np.zeros(self.num_params(), dtype=np.float32)

@@ -46,7 +46,7 @@ __device__ inline float silu_grad(float z) {
     return s * (1.0f + z * (1.0f - s));
 }
 
-constexpr int kBM = 64, kBN = 64, kBK = 16;
+constexpr int kBM = 64, kBN = 64, kBK = 16, kKC = 256;  // tile, MFMA K step, staged K chunk
 
 enum Epilogue : int {
     kStoreF32 = 0,   // C (fp32)
@@ -69,35 +69,86 @@ struct GemmArgs {
     int kchunk;        // K per split (a multiple of kBK); gridDim.z splits
 };
 
-// Stage a kBM x kBK tile of A (rows m0.., columns k0..) and a kBK x kBN tile of B through LDS
-// with consecutive threads on consecutive addresses for either operand layout.
-__device__ inline void stage(const GemmArgs& g, int m0, int n0, int k0, int kend, uint16_t (*As)[kBK + 4],
-                             uint16_t (*Bs)[kBN + 4]) {
+// Load one element of A / B (bounds and the ones row handled).
+__device__ inline uint16_t a_at(const GemmArgs& g, int m, int k, int kend) {
+    if (k >= kend) return 0;
+    if (m == g.ones_row) return 0x3f80;  // 1.0
+    return m < g.m ? g.a[m * g.sam + k * g.sak] : (uint16_t)0;
+}
+__device__ inline uint16_t b_at(const GemmArgs& g, int k, int n, int kend) {
+    return (k < kend && n < g.n) ? g.b[k * g.sbk + n * g.sbn] : (uint16_t)0;
+}
+
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+// Stage a kBM x kc chunk of A as As[m][k] and a kc x kBN chunk of B as Bs[n][k] (k contiguous in
+// both, the MFMA operand order), with 16-byte global loads along whichever of the two indices
+// is contiguous in memory; groups that cross a bound are loaded element by element.
+__device__ inline void stage(const GemmArgs& g, int m0, int n0, int k0, int kc, int kend, uint16_t (*As)[kKC + 8],
+                             uint16_t (*Bs)[kKC + 8]) {
     const int tid = threadIdx.x;
-    for (int i = tid; i < kBM * kBK; i += 256) {
-        const int r = g.sak == 1 ? i / kBK : i % kBM;
-        const int c = g.sak == 1 ? i % kBK : i / kBM;
-        const int gm = m0 + r, gk = k0 + c;
-        uint16_t v = 0;
-        if (gk < kend) {
-            if (gm == g.ones_row)
-                v = 0x3f80;  // 1.0
-            else if (gm < g.m)
-                v = g.a[gm * g.sam + gk * g.sak];
+    // A
+    if (g.sak == 1) {  // rows contiguous in k
+        for (int i = tid; i < kBM * (kc / 8); i += 256) {
+            const int r = i / (kc / 8), c8 = (i % (kc / 8)) * 8;
+            const int gm = m0 + r, gk = k0 + c8;
+            u16x8 v;
+            if (gm < g.m && gm != g.ones_row && gk + 8 <= kend) {
+                v = *reinterpret_cast<const u16x8*>(g.a + gm * g.sam + gk);
+            } else {
+                for (int t = 0; t < 8; ++t) v[t] = a_at(g, gm, gk + t, kend);
+            }
+            *reinterpret_cast<u16x8*>(&As[r][c8]) = v;
         }
-        As[r][c] = v;
+    } else if (g.sam != 1) {  // neither index contiguous: element by element
+        for (int i = tid; i < kBM * kc; i += 256) As[i / kc][i % kc] = a_at(g, m0 + i / kc, k0 + i % kc, kend);
+    } else {  // columns contiguous in m
+        for (int i = tid; i < (kBM / 8) * kc; i += 256) {
+            const int k = i / (kBM / 8), r8 = (i % (kBM / 8)) * 8;
+            const int gm = m0 + r8, gk = k0 + k;
+            u16x8 v;
+            if (gm + 8 <= g.m && gk < kend && (g.ones_row < gm || g.ones_row >= gm + 8)) {
+                v = *reinterpret_cast<const u16x8*>(g.a + gm + gk * g.sak);
+            } else {
+                for (int t = 0; t < 8; ++t) v[t] = a_at(g, gm + t, gk, kend);
+            }
+            for (int t = 0; t < 8; ++t) As[r8 + t][k] = v[t];
+        }
     }
-    for (int i = tid; i < kBK * kBN; i += 256) {
-        const int r = g.sbn == 1 ? i / kBN : i % kBK;
-        const int c = g.sbn == 1 ? i % kBN : i / kBK;
-        const int gk = k0 + r, gn = n0 + c;
-        Bs[r][c] = (gk < kend && gn < g.n) ? g.b[gk * g.sbk + gn * g.sbn] : (uint16_t)0;
+    // B
+    if (g.sbk == 1) {  // columns of B contiguous in k
+        for (int i = tid; i < kBN * (kc / 8); i += 256) {
+            const int n = i / (kc / 8), c8 = (i % (kc / 8)) * 8;
+            const int gn = n0 + n, gk = k0 + c8;
+            u16x8 v;
+            if (gn < g.n && gk + 8 <= kend) {
+                v = *reinterpret_cast<const u16x8*>(g.b + gk + gn * g.sbn);
+            } else {
+                for (int t = 0; t < 8; ++t) v[t] = b_at(g, gk + t, gn, kend);
+            }
+            *reinterpret_cast<u16x8*>(&Bs[n][c8]) = v;
+        }
+    } else if (g.sbn != 1) {
+        for (int i = tid; i < kBN * kc; i += 256) Bs[i / kc][i % kc] = b_at(g, k0 + i % kc, n0 + i / kc, kend);
+    } else {  // rows of B contiguous in n
+        for (int i = tid; i < (kBN / 8) * kc; i += 256) {
+            const int k = i / (kBN / 8), n8 = (i % (kBN / 8)) * 8;
+            const int gn = n0 + n8, gk = k0 + k;
+            u16x8 v;
+            if (gn + 8 <= g.n && gk < kend) {
+                v = *reinterpret_cast<const u16x8*>(g.b + gk * g.sbk + gn);
+            } else {
+                for (int t = 0; t < 8; ++t) v[t] = b_at(g, gk, gn + t, kend);
+            }
+            for (int t = 0; t < 8; ++t) Bs[n8 + t][k] = v[t];
+        }
     }
 }
 
+// One workgroup: a kBM x kBN block of C over the K range of its split, staged kKC at a time.
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
-    __shared__ uint16_t As[kBM][kBK + 4];
-    __shared__ uint16_t Bs[kBK][kBN + 4];
+    __shared__ __attribute__((aligned(16))) uint16_t As[kBM][kKC + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[kBN][kKC + 8];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -106,18 +157,19 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
     f32x4 acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
-        stage(g, m0, n0, k0, kend, As, Bs);
+    const int kq = 4 * (lane >> 4), li = lane & 15;
+    for (int k0 = kbeg; k0 < kend; k0 += kKC) {
+        const int kc = min(kKC, (kend - k0 + 15) / 16 * 16);  // whole MFMA steps; the tail is zero
+        stage(g, m0, n0, k0, kc, kend, As, Bs);
         __syncthreads();
-        const int kq = 4 * (lane >> 4), li = lane & 15;
-        bf16x4 af[2], bfr[2];
-        for (int i = 0; i < 2; ++i)
-            for (int t = 0; t < 4; ++t) af[i][t] = (short)As[wm + 16 * i + li][kq + t];
-        for (int j = 0; j < 2; ++j)
-            for (int t = 0; t < 4; ++t) bfr[j][t] = (short)Bs[kq + t][wn + 16 * j + li];
-        for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        for (int kk = 0; kk < kc; kk += kBK) {
+            bf16x4 af[2], bfr[2];
+            for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x4*>(&As[wm + 16 * i + li][kk + kq]);
+            for (int j = 0; j < 2; ++j) bfr[j] = *reinterpret_cast<const bf16x4*>(&Bs[wn + 16 * j + li][kk + kq]);
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
         __syncthreads();
     }
     const int mrows = g.ones_row >= 0 ? g.ones_row + 1 : g.m;
@@ -192,14 +244,15 @@ __global__ void head_kernel(const uint16_t* h4, const float* z4, const float* w5
     }
 }
 
-// Output-layer gradients over one chunk of 256 rows (one block per chunk): partial[chunk][k] =
+// Output-layer gradients over one chunk of kHeadChunk rows (one block per chunk): partial[chunk][k] =
 // sum_r H4[r][k] dy[r] for k < 256, then sum_r dy[r] and sum_r (y - t)^2.
-constexpr int kHeadChunk = 256;
+constexpr int kHeadChunk = 32;
 __global__ void head_grad_partial_kernel(const uint16_t* h4, const float* dy, const float* loss_rows, int rows,
                                          float* partial) {
     const int k = threadIdx.x, r0 = blockIdx.x * kHeadChunk;
     const int r1 = min(rows, r0 + kHeadChunk);
     float s = 0.f;
+#pragma unroll 8
     for (int r = r0; r < r1; ++r) s += bf2f(h4[(long long)r * kMlpHidden + k]) * dy[r];
     float* out = partial + (long long)blockIdx.x * (kMlpHidden + 2);
     out[k] = s;
@@ -215,25 +268,32 @@ __global__ void head_grad_partial_kernel(const uint16_t* h4, const float* dy, co
 }
 
 // dW5, db5 and the mean loss from the chunk partials
+// The mean loss goes to loss_hist[iteration mod kLossChunk] (the iteration counter counters[0]).
+// One wave per output k (258 blocks of 64): lanes take chunks l, l + 64, ..., then a fixed-order
+// wave reduction.
 __global__ void head_grad_reduce_kernel(const float* partial, int chunks, int rows, float* dw5, float* db5,
-                                        float* loss_out) {
-    const int k = threadIdx.x;  // 0 .. 257
-    if (k >= kMlpHidden + 2) return;
+                                        float* loss_hist, const uint64_t* counters) {
+    const int k = blockIdx.x, lane = threadIdx.x;  // k: 0 .. 257
     float s = 0.f;
-    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * (kMlpHidden + 2) + k];
+    for (int c = lane; c < chunks; c += 64) s += partial[(long long)c * (kMlpHidden + 2) + k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane != 0) return;
     if (k < kMlpHidden)
         dw5[k] = s;
     else if (k == kMlpHidden)
         db5[0] = s;
-    else if (loss_out)
-        *loss_out = s / (float)rows;
+    else
+        loss_hist[counters[0] % kLossChunk] = s / (float)rows;
 }
 
-// Adam on fp32 master weights; refreshes the bf16 copies of the GEMM weights
+// Adam on fp32 master weights, step t = counters[1] + 1 (bias corrections 1 - b^t); refreshes
+// the bf16 copies of the GEMM weights
 __global__ void adam_kernel(float* w, const float* grad, float* m1, float* m2, uint16_t* wbf, int n, int n_bf,
-                            float lr, float b1, float b2, float eps, float c1, float c2) {
+                            float lr, float b1, float b2, float eps, const uint64_t* counters) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const float t = (float)(counters[1] + 1);
+    const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
     const float g = grad[i];
     const float a = b1 * m1[i] + (1.f - b1) * g;
     const float v = b2 * m2[i] + (1.f - b2) * g * g;
@@ -245,16 +305,23 @@ __global__ void adam_kernel(float* w, const float* grad, float* m1, float* m2, u
     if (i < n_bf) wbf[i] = f2bf(nw);
 }
 
+// end of a training iteration: next iteration, one more Adam step
+__global__ void step_kernel(uint64_t* counters) {
+    counters[0] += 1;
+    counters[1] += 1;
+}
+
 __global__ void to_bf16_kernel(const float* w, uint16_t* wbf, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) wbf[i] = f2bf(w[i]);
 }
 
-// batch indices: idx[r] = floor(u * n) from Philox(seed, stream) at call first + r
-__global__ void sample_kernel(uint64_t seed, uint64_t stream, uint64_t first, int rows, uint64_t n, uint32_t* idx) {
+// batch indices: idx[r] = floor(u * n) from Philox(seed, stream) at call counters[0] * rows + r
+__global__ void sample_kernel(uint64_t seed, uint64_t stream, const uint64_t* counters, int rows, uint64_t n,
+                              uint32_t* idx) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
-    const double u = philox_uniform(seed, stream, first + r);
+    const double u = philox_uniform(seed, stream, counters[0] * (uint64_t)rows + r);
     uint64_t v = (uint64_t)(u * (double)n);
     idx[r] = (uint32_t)(v < n ? v : n - 1);
 }
@@ -332,6 +399,7 @@ Mlp::Mlp(Context* c, int max_rows) : ctx(c), cap(max_rows) {
         alloc(&loss_rows, cap);
         alloc(&idx, cap);
         alloc(&loss_hist, kLossChunk);
+        alloc(&counters, 2);
         alloc(&partial, (size_t)kMaxSplits * (kMlpHidden + 1) * kMlpHidden);
         alloc(&head_partial, (size_t)((cap + kHeadChunk - 1) / kHeadChunk) * (kMlpHidden + 2));
         ALEATOR_HIP_CHECK(hipMemcpyAsync(kvec, kv.data(), kv.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -349,6 +417,7 @@ Mlp::Mlp(Context* c, int max_rows) : ctx(c), cap(max_rows) {
 
 Mlp::~Mlp() {
     (void)hipStreamSynchronize(ctx->stream);
+    drop_graph();
     for (void* p : buffers) (void)hipFree(p);
 }
 
@@ -358,7 +427,7 @@ void Mlp::set_params(const float* host) {
                        kMlpOffsets[8]);
     ALEATOR_HIP_CHECK(hipMemsetAsync(m1, 0, kMlpParams * sizeof(float), ctx->stream));
     ALEATOR_HIP_CHECK(hipMemsetAsync(m2, 0, kMlpParams * sizeof(float), ctx->stream));
-    adam_t = 0;
+    ALEATOR_HIP_CHECK(hipMemsetAsync(counters, 0, 2 * sizeof(uint64_t), ctx->stream));
     ALEATOR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
@@ -394,15 +463,15 @@ void Mlp::forward(const double* frac, const uint32_t* rows_idx, int rows) {
     }
 }
 
-void Mlp::backward(const double* target, const uint32_t* rows_idx, int rows, float* loss_slot) {
+void Mlp::backward(const double* target, const uint32_t* rows_idx, int rows) {
     hipStream_t s = ctx->stream;
     hipLaunchKernelGGL(head_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, h[3], z[3], w + kMlpOffsets[8],
                        w + kMlpOffsets[9], target, rows_idx, rows, y, dy, dz[3], loss_rows, 1);
     const int chunks = (rows + kHeadChunk - 1) / kHeadChunk;
     hipLaunchKernelGGL(head_grad_partial_kernel, dim3(chunks), dim3(kMlpHidden), 0, s, h[3], dy, loss_rows, rows,
                        head_partial);
-    hipLaunchKernelGGL(head_grad_reduce_kernel, dim3(1), dim3(kMlpHidden + 64), 0, s, head_partial, chunks, rows,
-                       grad + kMlpOffsets[8], grad + kMlpOffsets[9], loss_slot);
+    hipLaunchKernelGGL(head_grad_reduce_kernel, dim3(kMlpHidden + 2), dim3(64), 0, s, head_partial, chunks, rows,
+                       grad + kMlpOffsets[8], grad + kMlpOffsets[9], loss_hist, counters);
     const int splits = std::max(1, std::min(kMaxSplits, rows / 256));
     for (int l = 3; l >= 0; --l) {
         // [dW_l; db_l] = [H_{l-1}^T; 1^T] dZ_l   ([kin + 1 x rows] x [rows x 256]), split over rows;
@@ -451,17 +520,54 @@ void Mlp::backward(const double* target, const uint32_t* rows_idx, int rows, flo
     }
 }
 
-void Mlp::adam(float lr) {
-    ++adam_t;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.f - std::pow(b1, (float)adam_t), c2 = 1.f - std::pow(b2, (float)adam_t);
-    hipLaunchKernelGGL(adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, ctx->stream, w, grad, m1, m2, wbf,
-                       kMlpParams, kMlpOffsets[8], lr, b1, b2, 1e-8f, c1, c2);
+void Mlp::iteration(const double* frac, const double* target, uint64_t n_samples, int batch, float lr,
+                    uint64_t seed, uint64_t stream) {
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(sample_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, seed, stream, counters, batch,
+                       n_samples, idx);
+    forward(frac, idx, batch);
+    backward(target, idx, batch);
+    hipLaunchKernelGGL(adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, w, grad, m1, m2, wbf, kMlpParams,
+                       kMlpOffsets[8], lr, 0.9f, 0.999f, 1e-8f, counters);
+    hipLaunchKernelGGL(step_kernel, dim3(1), dim3(1), 0, s, counters);
 }
 
-void Mlp::sample(uint64_t seed, uint64_t stream, uint64_t first, int rows, uint64_t n) {
-    hipLaunchKernelGGL(sample_kernel, dim3((rows + 255) / 256), dim3(256), 0, ctx->stream, seed, stream, first, rows, n,
-                       idx);
+void Mlp::drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+}
+
+void Mlp::train(const double* frac, const double* target, uint64_t n_samples, int batch, int n_iters, float lr,
+                uint64_t seed, uint64_t stream, float* loss_trace) {
+    hipStream_t s = ctx->stream;
+    ALEATOR_HIP_CHECK(hipMemsetAsync(counters, 0, sizeof(uint64_t), s));  // iteration 0 of this call
+    const GraphKey key{frac, target, n_samples, batch, lr, seed, stream};
+    if (use_graph && n_iters > 1 && !(graph_exec && key == graph_key)) {
+        drop_graph();
+        hipGraph_t graph = nullptr;
+        ALEATOR_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        iteration(frac, target, n_samples, batch, lr, seed, stream);
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        ALEATOR_HIP_CHECK(e);
+        const hipError_t ei = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        ALEATOR_HIP_CHECK(ei);
+        graph_key = key;
+    }
+    const bool replay = use_graph && n_iters > 1;
+    for (int it0 = 0; it0 < n_iters; it0 += kLossChunk) {
+        const int cnt = std::min(kLossChunk, n_iters - it0);
+        for (int i = 0; i < cnt; ++i) {
+            if (replay)
+                ALEATOR_HIP_CHECK(hipGraphLaunch(graph_exec, s));
+            else
+                iteration(frac, target, n_samples, batch, lr, seed, stream);
+        }
+        ALEATOR_HIP_CHECK(hipGetLastError());
+        if (loss_trace)
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(loss_trace + it0, loss_hist, cnt * sizeof(float), hipMemcpyDeviceToHost, s));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+    }
 }
 
 void Mlp::predict(const double* frac, int rows, float* out) {
@@ -565,8 +671,9 @@ aleator_status aleator_gpu_mlp_loss_grad(aleator_mlp* m, const double* frac, con
         require(m && frac && target, ALEATOR_INVALID_ARGUMENT, "null argument");
         Mlp& net = m->net;
         require(n >= 1 && n <= net.cap, ALEATOR_INVALID_ARGUMENT, "mlp: rows must be in [1, max_rows]");
+        ALEATOR_HIP_CHECK(hipMemsetAsync(net.counters, 0, sizeof(uint64_t), net.ctx->stream));  // loss slot 0
         net.forward(frac, nullptr, n);
-        net.backward(target, nullptr, n, net.loss_hist);
+        net.backward(target, nullptr, n);
         check_launch();
         float l = 0.0f;
         ALEATOR_HIP_CHECK(hipMemcpyAsync(&l, net.loss_hist, sizeof(float), hipMemcpyDeviceToHost, net.ctx->stream));
@@ -586,21 +693,15 @@ aleator_status aleator_gpu_mlp_train(aleator_mlp* m, const double* frac, const d
         require(batch >= 1 && batch <= net.cap, ALEATOR_INVALID_ARGUMENT, "mlp: batch must be in [1, max_rows]");
         require(n_iters >= 0, ALEATOR_INVALID_ARGUMENT, "mlp: negative iteration count");
         require(lr > 0.0f && std::isfinite(lr), ALEATOR_INVALID_ARGUMENT, "mlp: learning rate must be positive");
-        for (int it0 = 0; it0 < n_iters; it0 += kLossChunk) {
-            const int cnt = std::min(kLossChunk, n_iters - it0);
-            for (int i = 0; i < cnt; ++i) {
-                const uint64_t it = (uint64_t)(it0 + i);
-                net.sample(seed, stream, it * (uint64_t)batch, batch, (uint64_t)n_samples);
-                net.forward(frac, net.idx, batch);
-                net.backward(target, net.idx, batch, net.loss_hist + i);
-                net.adam(lr);
-            }
-            check_launch();
-            if (loss_trace)
-                ALEATOR_HIP_CHECK(hipMemcpyAsync(loss_trace + it0, net.loss_hist, cnt * sizeof(float),
-                                                 hipMemcpyDeviceToHost, net.ctx->stream));
-            ALEATOR_HIP_CHECK(hipStreamSynchronize(net.ctx->stream));
-        }
+        net.train(frac, target, (uint64_t)n_samples, batch, n_iters, lr, seed, stream, loss_trace);
+    });
+}
+
+aleator_status aleator_gpu_mlp_use_graph(aleator_mlp* m, int32_t enable) {
+    return guarded([&] {
+        require(m != nullptr, ALEATOR_INVALID_ARGUMENT, "null argument");
+        m->net.use_graph = enable != 0;
+        if (!m->net.use_graph) m->net.drop_graph();
     });
 }
 

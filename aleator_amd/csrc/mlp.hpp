@@ -42,7 +42,21 @@ struct Mlp {
     float* loss_hist = nullptr;  // per-iteration loss of the current train call chunk
     float* partial = nullptr;    // split-K partial weight gradients [kMaxSplits][257][256]
     float* head_partial = nullptr;  // output-layer partial sums [row chunk][258]
-    int adam_t = 0;
+    uint64_t* counters = nullptr;   // device: [0] iteration of the current train call, [1] Adam steps taken
+    // a training iteration captured as a hipGraph (kernels only), replayed for every iteration
+    // of train calls with the same arguments
+    struct GraphKey {
+        const double* frac;
+        const double* target;
+        uint64_t n;
+        int batch;
+        float lr;
+        uint64_t seed, stream;
+        bool operator==(const GraphKey&) const = default;
+    };
+    bool use_graph = true;
+    hipGraphExec_t graph_exec = nullptr;
+    GraphKey graph_key{};
 
     Mlp(Context* c, int max_rows);
     ~Mlp();
@@ -50,10 +64,15 @@ struct Mlp {
     void get(float* host, const float* src) const;
     // forward of rows (gathered through rows_idx when not null) up to the last hidden layer
     void forward(const double* frac, const uint32_t* rows_idx, int rows);
-    // head, loss and gradients of the last forward; loss_slot receives the mean loss (device)
-    void backward(const double* target, const uint32_t* rows_idx, int rows, float* loss_slot);
-    void adam(float lr);
-    void sample(uint64_t seed, uint64_t stream, uint64_t first, int rows, uint64_t n);
+    // head, loss and gradients of the last forward; the mean loss goes to
+    // loss_hist[counters[0] mod kLossChunk]
+    void backward(const double* target, const uint32_t* rows_idx, int rows);
+    // one training iteration: sample a minibatch, forward, backward, Adam, advance the counters
+    void iteration(const double* frac, const double* target, uint64_t n_samples, int batch, float lr, uint64_t seed,
+                   uint64_t stream);
+    void train(const double* frac, const double* target, uint64_t n_samples, int batch, int n_iters, float lr,
+               uint64_t seed, uint64_t stream, float* loss_trace);
+    void drop_graph();
     void predict(const double* frac, int rows, float* out);
 };
 

@@ -373,6 +373,9 @@ aleator_status aleator_gpu_mlp_loss_grad(aleator_mlp* mlp, const double* frac_de
 aleator_status aleator_gpu_mlp_train(aleator_mlp* mlp, const double* frac_device, const double* target_device,
                                      int64_t n_samples, int32_t batch, int32_t n_iters, float lr, uint64_t seed,
                                      uint64_t stream, float* loss_trace);
+/* Whether train replays one captured hipGraph of an iteration's kernels (default 1) or launches
+ * them one by one (0); both give the same results bit for bit. */
+aleator_status aleator_gpu_mlp_use_graph(aleator_mlp* mlp, int32_t enable);
 
 /* ---- host utilities of the path (no device work) ------------------------ */
 /* Philox4x32Rng(seed, stream).nextUniform() at call indices first .. first+n-1

@@ -100,6 +100,25 @@ def test_training_fits_a_smooth_periodic_target(ctx):
     assert err < 0.1
 
 
+def test_graph_replay_equals_eager_launches(ctx):
+    """Training from a replayed hipGraph gives the eager launches' results bit for bit, over two
+    train calls with other device work (a prediction) between them."""
+    print("mlp training: graph replay vs eager launches", flush=True)
+    frac, target = rows(5000, seed=12)
+    d_f = aa.DeviceBuffer.from_array(frac)
+    d_t = aa.DeviceBuffer.from_array(target)
+    runs = []
+    for graph in (True, False):
+        net = aa.Mlp(ctx, max_rows=512, seed=4)
+        net.use_graph(graph)
+        l1 = net.train_device(int(d_f), int(d_t), len(frac), 512, 25, 1e-3, seed=5, stream=1)
+        pred = net.predict(frac[:100])
+        l2 = net.train_device(int(d_f), int(d_t), len(frac), 512, 30, 1e-3, seed=6, stream=1)
+        runs.append((l1, pred, l2, net.params))
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+
+
 def test_errors(ctx):
     net = aa.Mlp(ctx, max_rows=16, seed=0)
     frac, target = rows(32)
