@@ -19,7 +19,8 @@ import os
 import numpy as np
 
 __all__ = ["Context", "Framework", "EwaldState", "grid_point_counts", "synth_framework", "library_path",
-           "FP64", "MIXED", "TRUNCATED", "SHIFTED", "LINEAR_FORCE_SHIFTED", "NotImplementedInReference"]
+           "FP64", "MIXED", "TRUNCATED", "SHIFTED", "LINEAR_FORCE_SHIFTED", "NotImplementedInReference",
+           "read_cif", "parse_cif", "Structure", "CifDocument", "CifParseError"]
 
 FP64, MIXED = 0, 1
 TRUNCATED, SHIFTED, LINEAR_FORCE_SHIFTED = 0, 1, 2
@@ -39,6 +40,10 @@ class NotImplementedInReference(RuntimeError):
 
 class DeviceError(RuntimeError):
     """HIP runtime failure."""
+
+
+class CifParseError(RuntimeError):
+    """io::CifParseError (src/io/cif_error.hpp:8-16)."""
 
 
 _dp = C.POINTER(C.c_double)
@@ -95,7 +100,7 @@ def _check(status: int):
         return
     msg = _load().aleator_gpu_last_error().decode(errors="replace")
     raise {1: ValueError, 2: IndexError, 3: RuntimeError, 4: NotImplementedInReference,
-           5: DeviceError}.get(status, RuntimeError)(msg)
+           5: DeviceError, 6: CifParseError}.get(status, RuntimeError)(msg)
 
 
 def _p(a, t=_dp):
@@ -170,6 +175,163 @@ def grid_cache_load(directory: str, key: GridCacheKey):
     _check(_load().aleator_grid_cache_load(directory.encode(), C.byref(key.key), _p(n, _i32p), _p(sp, _u32p),
                                            C.byref(nt), _p(out), C.c_uint64(out.size), C.byref(hit)))
     return tuple(int(v) for v in n), sp[:nt.value].copy(), out
+
+
+# ---------------------------------------------------------------- CIF structure files
+class Structure:
+    """io::StructureData (io/structure_io.hpp:18-37): lattice rows, Cartesian positions, masses,
+    charges, species indices into species_symbols, occupancies and source labels, per atom."""
+
+    def __init__(self, handle):
+        lib = _load()
+        try:
+            n, ns = C.c_uint64(), C.c_uint32()
+            _check(lib.aleator_structure_size(handle, C.byref(n), C.byref(ns)))
+            lat = _Lattice()
+            _check(lib.aleator_structure_lattice(handle, C.byref(lat)))
+            self.lattice = np.array(lat.m[:]).reshape(3, 3)
+            k = n.value
+            self.x, self.y, self.z, self.mass, self.charge, self.occupancy = (np.zeros(k) for _ in range(6))
+            self.species = np.zeros(k, dtype=np.uint32)
+            _check(lib.aleator_structure_atoms(handle, _p(self.x), _p(self.y), _p(self.z), _p(self.mass),
+                                               _p(self.charge), _p(self.species, _u32p), _p(self.occupancy)))
+            out = C.c_char_p()
+            self.species_symbols = []
+            for i in range(ns.value):
+                _check(lib.aleator_structure_species_symbol(handle, C.c_uint32(i), C.byref(out)))
+                self.species_symbols.append(out.value.decode())
+            self.source_labels = []
+            for i in range(k):
+                _check(lib.aleator_structure_source_label(handle, C.c_uint64(i), C.byref(out)))
+                self.source_labels.append(out.value.decode())
+        finally:
+            lib.aleator_structure_destroy(handle)
+
+    def __len__(self):
+        return len(self.x)
+
+    def positions(self) -> np.ndarray:
+        return np.stack([self.x, self.y, self.z], axis=1)
+
+
+def read_cif(path) -> Structure:
+    """io::readCif (io/structure_io.cc:173-250); CifParseError on malformed input."""
+    h = C.c_void_p()
+    _check(_load().aleator_read_cif(os.fsencode(path), C.byref(h)))
+    return Structure(h)
+
+
+def parse_cif(text: str) -> Structure:
+    """read_cif from CIF text in memory."""
+    b = text.encode() if isinstance(text, str) else bytes(text)
+    h = C.c_void_p()
+    _check(_load().aleator_parse_cif(b, C.c_uint64(len(b)), C.byref(h)))
+    return Structure(h)
+
+
+class CifDocument:
+    """io::CifDocument (io/cif_document.hpp:26-42) of parseCifDocument(text): block_name, tags
+    (lowercased tag -> raw value) and loops (list of (columns, rows))."""
+
+    def __init__(self, text: str):
+        lib = _load()
+        b = text.encode() if isinstance(text, str) else bytes(text)
+        h = C.c_void_p()
+        _check(lib.aleator_cif_document_parse(b, C.c_uint64(len(b)), C.byref(h)))
+        self._h = h
+        try:
+            out = C.c_char_p()
+            _check(lib.aleator_cif_document_block_name(h, C.byref(out)))
+            self.block_name = out.value.decode()
+            nt = C.c_uint64()
+            _check(lib.aleator_cif_document_num_tags(h, C.byref(nt)))
+            self.tags = {}
+            val = C.c_char_p()
+            for i in range(nt.value):
+                _check(lib.aleator_cif_document_tag(h, C.c_uint64(i), C.byref(out), C.byref(val)))
+                self.tags[out.value.decode()] = val.value.decode()
+            nl = C.c_uint64()
+            _check(lib.aleator_cif_document_num_loops(h, C.byref(nl)))
+            self.loops = []
+            for li in range(nl.value):
+                nc, nr = C.c_uint64(), C.c_uint64()
+                _check(lib.aleator_cif_document_loop_shape(h, C.c_uint64(li), C.byref(nc), C.byref(nr)))
+                cols = []
+                for c in range(nc.value):
+                    _check(lib.aleator_cif_document_loop_column(h, C.c_uint64(li), C.c_uint64(c), C.byref(out)))
+                    cols.append(out.value.decode())
+                rows = []
+                for r in range(nr.value):
+                    row = []
+                    for c in range(nc.value):
+                        _check(lib.aleator_cif_document_loop_value(h, C.c_uint64(li), C.c_uint64(r), C.c_uint64(c),
+                                                                   C.byref(out)))
+                        row.append(out.value.decode())
+                    rows.append(row)
+                self.loops.append((cols, rows))
+        except Exception:
+            self.close()
+            raise
+
+    def find_tag(self, tag: str):
+        """CifDocument::findTag: the raw value, or None."""
+        out = C.c_char_p()
+        _check(_load().aleator_cif_document_find_tag(self._h, tag.encode(), C.byref(out)))
+        return None if out.value is None else out.value.decode()
+
+    def find_loop(self, columns):
+        """CifDocument::findLoopWithColumns: index into loops of the first loop with every column, or None."""
+        arr = (C.c_char_p * max(len(columns), 1))(*[c.encode() for c in columns])
+        idx = C.c_int64()
+        _check(_load().aleator_cif_document_find_loop(self._h, arr, C.c_uint32(len(columns)), C.byref(idx)))
+        return None if idx.value < 0 else idx.value
+
+    def close(self):
+        if self._h:
+            _load().aleator_cif_document_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def try_parse_cif_number(token: str):
+    """tryParseCifNumber (cif_document.cc:166-183): float, or None for '?', '.' or non-numbers."""
+    v, has = C.c_double(), C.c_int32()
+    _check(_load().aleator_cif_try_parse_number(token.encode(), C.byref(v), C.byref(has)))
+    return v.value if has.value else None
+
+
+def parse_cif_number(token: str) -> float:
+    """parseCifNumber (cif_document.cc:158-164): CifParseError when there is no number."""
+    v = C.c_double()
+    _check(_load().aleator_cif_parse_number(token.encode(), C.byref(v)))
+    return v.value
+
+
+def parse_symmetry_operation(text: str):
+    """parseSymmetryOperation (symmetry_operation.cc:118-150): (matrix 3x3, translation 3)."""
+    m, t = np.zeros(9), np.zeros(3)
+    _check(_load().aleator_parse_symmetry_operation(text.encode(), _p(m), _p(t)))
+    return m.reshape(3, 3), t
+
+
+def element_from_label(label: str):
+    """elementFromLabel (element_table.cc:81-99): canonical symbol, or None."""
+    buf = C.create_string_buffer(4)
+    found = C.c_int32()
+    _check(_load().aleator_element_from_label(label.encode(), buf, C.byref(found)))
+    return buf.value.decode() if found.value else None
+
+
+def atomic_mass(symbol: str):
+    """atomicMass (element_table.cc:67-79): amu, or None for an unknown (case-exact) symbol."""
+    m, found = C.c_double(), C.c_int32()
+    _check(_load().aleator_atomic_mass(symbol.encode(), C.byref(m), C.byref(found)))
+    return m.value if found.value else None
 
 
 def philox_uniforms(seed: int, stream: int, first_call_index: int, n: int) -> np.ndarray:

@@ -44,7 +44,8 @@ typedef enum {
     ALEATOR_OUT_OF_RANGE = 2,     /* std::out_of_range */
     ALEATOR_RUNTIME_ERROR = 3,    /* std::runtime_error */
     ALEATOR_NOT_IMPLEMENTED = 4,  /* aleator::NotImplemented */
-    ALEATOR_DEVICE_ERROR = 5      /* HIP runtime failure (no reference counterpart) */
+    ALEATOR_DEVICE_ERROR = 5,     /* HIP runtime failure (no reference counterpart) */
+    ALEATOR_PARSE_ERROR = 6       /* io::CifParseError (io/cif_error.hpp:8-16) */
 } aleator_status;
 
 /* LennardJonesTruncation (pairwise/lennard_jones.hpp:21-38) */
@@ -403,6 +404,70 @@ aleator_status aleator_lj_tail_insertion_delta(const aleator_lj_params* lj, cons
                                                const uint32_t* framework_species, uint64_t n_framework,
                                                const aleator_molecule_site* sites, int32_t n_sites,
                                                double* delta);
+
+/* ---- structure files: CIF reader (host only) ---------------------------- */
+/* io::readCif (io/structure_io.hpp:39-50, structure_io.cc:173-250): cell from _cell_length_*
+ * and _cell_angle_* (a along x, b in the xy plane), symmetry operations from the first loop
+ * with _symmetry_equiv_pos_as_xyz, else _space_group_symop_operation_xyz, else identity, and
+ * the _atom_site loop expanded through every operation: images wrapped into [0,1), an image
+ * within 1e-3 A of one already kept for that site dropped.  Atoms are emitted site by site in
+ * file order, each site's images in operation order; species are numbered by first
+ * appearance of each element; positions are Cartesian, masses from the element table, charge
+ * 0 and occupancy 1 when absent.  Malformed input, or a file that cannot be opened, returns
+ * ALEATOR_PARSE_ERROR. */
+typedef struct aleator_structure aleator_structure;
+aleator_status aleator_read_cif(const char* path, aleator_structure** out);
+/* The same from CIF text in memory. */
+aleator_status aleator_parse_cif(const char* text, uint64_t length, aleator_structure** out);
+aleator_status aleator_structure_destroy(aleator_structure* structure);
+aleator_status aleator_structure_size(const aleator_structure* structure, uint64_t* n_atoms,
+                                      uint32_t* n_species);
+aleator_status aleator_structure_lattice(const aleator_structure* structure, aleator_lattice* lattice);
+/* Copies the per-atom arrays (n_atoms entries each); any pointer may be NULL to skip it
+ * (StructureData::particles, ::occupancy; structure_io.hpp:18-37). */
+aleator_status aleator_structure_atoms(const aleator_structure* structure, double* x, double* y,
+                                       double* z, double* mass, double* charge, uint32_t* species,
+                                       double* occupancy);
+/* StructureData::speciesSymbols[species] and ::sourceLabel[atom]; the strings live as long
+ * as the structure. */
+aleator_status aleator_structure_species_symbol(const aleator_structure* structure, uint32_t species,
+                                                const char** symbol);
+aleator_status aleator_structure_source_label(const aleator_structure* structure, uint64_t atom,
+                                              const char** label);
+
+/* io::parseCifDocument (io/cif_document.hpp:26-50, cif_document.cc:185-350): the first data
+ * block's single tags (lowercased) and loop_ tables, values as tokenized (quotes removed). */
+typedef struct aleator_cif_document aleator_cif_document;
+aleator_status aleator_cif_document_parse(const char* text, uint64_t length, aleator_cif_document** out);
+aleator_status aleator_cif_document_destroy(aleator_cif_document* doc);
+aleator_status aleator_cif_document_block_name(const aleator_cif_document* doc, const char** name);
+/* CifDocument::findTag: *value is NULL when the tag is absent (the lookup is case-blind). */
+aleator_status aleator_cif_document_find_tag(const aleator_cif_document* doc, const char* tag,
+                                             const char** value);
+/* The single tags in ascending (byte) order of their lowercased names. */
+aleator_status aleator_cif_document_num_tags(const aleator_cif_document* doc, uint64_t* n_tags);
+aleator_status aleator_cif_document_tag(const aleator_cif_document* doc, uint64_t index, const char** tag,
+                                        const char** value);
+aleator_status aleator_cif_document_num_loops(const aleator_cif_document* doc, uint64_t* n_loops);
+/* CifDocument::findLoopWithColumns: index of the first loop holding every tag (lowercase), or -1. */
+aleator_status aleator_cif_document_find_loop(const aleator_cif_document* doc, const char* const* tags,
+                                              uint32_t n_tags, int64_t* loop);
+aleator_status aleator_cif_document_loop_shape(const aleator_cif_document* doc, uint64_t loop,
+                                               uint64_t* n_columns, uint64_t* n_rows);
+aleator_status aleator_cif_document_loop_column(const aleator_cif_document* doc, uint64_t loop,
+                                                uint64_t column, const char** tag);
+aleator_status aleator_cif_document_loop_value(const aleator_cif_document* doc, uint64_t loop,
+                                               uint64_t row, uint64_t column, const char** value);
+/* tryParseCifNumber / parseCifNumber (cif_document.hpp:52-61): "?" and "." are absent, a
+ * "(esd)" suffix is dropped.  The second returns ALEATOR_PARSE_ERROR where the first reports
+ * no value. */
+aleator_status aleator_cif_try_parse_number(const char* token, double* value, int32_t* has_value);
+aleator_status aleator_cif_parse_number(const char* token, double* value);
+/* parseSymmetryOperation (io/symmetry_operation.hpp:20-28): x' = M x + t, M row-major. */
+aleator_status aleator_parse_symmetry_operation(const char* text, double matrix[9], double translation[3]);
+/* elementFromLabel / atomicMass (io/element_table.hpp:9-21); symbol is NUL-terminated. */
+aleator_status aleator_element_from_label(const char* label, char symbol[4], int32_t* found);
+aleator_status aleator_atomic_mass(const char* symbol, double* mass, int32_t* found);
 
 /* ---- synthetic frameworks (no reference counterpart) ------------------ */
 /* Uniform fractional placement from Philox4x32Rng(seed, stream), rejecting a candidate

@@ -27,6 +27,11 @@
 #include "forcefield/electrostatics/ewald.hpp"
 #include "forcefield/electrostatics/ewald_incremental_state.hpp"
 #include "forcefield/pairwise/lennard_jones.hpp"
+#include "io/cif_document.hpp"
+#include "io/cif_error.hpp"
+#include "io/element_table.hpp"
+#include "io/structure_io.hpp"
+#include "io/symmetry_operation.hpp"
 
 using namespace aleator;
 
@@ -458,6 +463,109 @@ int ref_pr_fugacity(double tc, double pc, double omega, double t, double p, doub
     return guarded([&] {
         engines::PengRobinson pr({tc, pc, omega});
         *out = pr.fugacityPascal(t, p);
+    });
+}
+
+// ---- CIF reader (io/) ---------------------------------------------------
+// Status convention of the CIF wrappers: 0 ok, 1 io::CifParseError, 2 any other exception.
+} // extern "C"
+
+namespace {
+template <typename F>
+int cif_guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const io::CifParseError&) {
+        return 1;
+    } catch (const std::exception&) {
+        return 2;
+    }
+}
+void put_text(const std::string& s, char* buf, uint64_t cap, uint64_t* len) {
+    *len = s.size();
+    if (buf && cap > s.size()) std::memcpy(buf, s.c_str(), s.size() + 1);
+}
+} // namespace
+
+extern "C" {
+
+// io::readCif: lattice (9), per-atom arrays (when n <= cap), species symbols and source
+// labels each joined with '\n'.
+int ref_read_cif(const char* path, double* lattice9, uint64_t* n_atoms, uint64_t cap, double* x, double* y,
+                 double* z, double* mass, double* charge, double* occupancy, uint32_t* species, char* symbols,
+                 uint64_t symbols_cap, uint64_t* symbols_len, char* labels, uint64_t labels_cap,
+                 uint64_t* labels_len) {
+    return cif_guarded([&] {
+        const io::StructureData s = io::readCif(path);
+        const auto& m = s.lattice.matrix();
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) lattice9[3 * i + j] = m[i][j];
+        const std::size_t n = s.particles.size();
+        *n_atoms = n;
+        if (n <= cap) {
+            for (std::size_t i = 0; i < n; ++i) {
+                x[i] = s.particles.x[i];
+                y[i] = s.particles.y[i];
+                z[i] = s.particles.z[i];
+                mass[i] = s.particles.mass[i];
+                charge[i] = s.particles.charge[i];
+                species[i] = s.particles.species[i];
+                occupancy[i] = s.occupancy[i];
+            }
+        }
+        std::string sym, lab;
+        for (std::size_t i = 0; i < s.speciesSymbols.size(); ++i) sym += (i ? "\n" : "") + s.speciesSymbols[i];
+        for (std::size_t i = 0; i < s.sourceLabel.size(); ++i) lab += (i ? "\n" : "") + s.sourceLabel[i];
+        put_text(sym, symbols, symbols_cap, symbols_len);
+        put_text(lab, labels, labels_cap, labels_len);
+    });
+}
+
+// io::parseCifDocument, flattened to text: "block\x1f<name>", then one "tag\x1f<t>\x1f<v>" per
+// single tag in sorted order, then per loop "loop\x1f<ncols>\x1f<nrows>", its columns and its
+// values, every record ended by '\x1e'.
+int ref_cif_document(const char* text, uint64_t length, char* out, uint64_t cap, uint64_t* len) {
+    return cif_guarded([&] {
+        const io::CifDocument d = io::parseCifDocument(std::string(text, length));
+        std::string r = "block\x1f" + d.blockName + "\x1e";
+        std::vector<std::pair<std::string, std::string>> tags(d.tags.begin(), d.tags.end());
+        std::sort(tags.begin(), tags.end());
+        for (const auto& [t, v] : tags) r += "tag\x1f" + t + "\x1f" + v + "\x1e";
+        for (const auto& l : d.loops) {
+            r += "loop\x1f" + std::to_string(l.columns.size()) + "\x1f" + std::to_string(l.rows.size()) + "\x1e";
+            for (const auto& c : l.columns) r += c + "\x1e";
+            for (const auto& row : l.rows)
+                for (const auto& v : row) r += v + "\x1e";
+        }
+        put_text(r, out, cap, len);
+    });
+}
+
+int ref_cif_try_number(const char* token, double* value, int* has_value) {
+    return cif_guarded([&] {
+        const auto v = io::tryParseCifNumber(token);
+        *has_value = v.has_value();
+        *value = v.value_or(0.0);
+    });
+}
+
+int ref_symmetry_operation(const char* text, double* matrix9, double* translation) {
+    return cif_guarded([&] {
+        const io::SymmetryOperation op = io::parseSymmetryOperation(text);
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) matrix9[3 * i + j] = op.matrix[i][j];
+            translation[i] = op.translation[i];
+        }
+    });
+}
+
+int ref_element_from_label(const char* label, char* symbol4, int* found) {
+    return cif_guarded([&] {
+        const auto e = io::elementFromLabel(label);
+        *found = e.has_value();
+        std::memset(symbol4, 0, 4);
+        if (e) std::memcpy(symbol4, e->c_str(), std::min<std::size_t>(e->size(), 3));
     });
 }
 

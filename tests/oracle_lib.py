@@ -545,3 +545,67 @@ def mlp_he_init(seed: int) -> np.ndarray:
         a = np.sqrt(3.0 / fan_in) if name == "w5" else np.sqrt(6.0 / fan_in)
         p[o:o + n] = (2.0 * u[o:o + n] - 1.0) * a
     return p.astype(np.float32)
+
+
+# ---------------------------------------------------------------- CIF reader of the reference (oracle/_ref)
+def ref_read_cif(path):
+    """io::readCif of the compiled reference: None on io::CifParseError, else a dict with the
+    lattice, per-atom arrays, species symbols and source labels."""
+    path = os.fsencode(str(path))
+    lat = np.zeros(9)
+    n = C.c_uint64()
+    slen, llen = C.c_uint64(), C.c_uint64()
+    st = ref().ref_read_cif(path, ptr(lat), C.byref(n), C.c_uint64(0), None, None, None, None, None, None, None,
+                            None, C.c_uint64(0), C.byref(slen), None, C.c_uint64(0), C.byref(llen))
+    if st == 1:
+        return None
+    if st != 0:
+        raise RuntimeError("reference readCif raised a non-CIF exception")
+    k = n.value
+    arrs = [np.zeros(k) for _ in range(6)]
+    sp = np.zeros(k, dtype=np.uint32)
+    sym = C.create_string_buffer(slen.value + 1)
+    lab = C.create_string_buffer(llen.value + 1)
+    st = ref().ref_read_cif(path, ptr(lat), C.byref(n), C.c_uint64(k), *[ptr(a) for a in arrs], ptr(sp, u32p), sym,
+                            C.c_uint64(slen.value + 1), C.byref(slen), lab, C.c_uint64(llen.value + 1), C.byref(llen))
+    assert st == 0
+    names = ("x", "y", "z", "mass", "charge", "occupancy")
+    out = dict(zip(names, arrs))
+    out.update(lattice=lat.reshape(3, 3), species=sp,
+               species_symbols=sym.value.decode().split("\n") if slen.value else [],
+               source_labels=lab.value.decode().split("\n") if llen.value else [])
+    return out
+
+
+def ref_cif_document(text: str):
+    """io::parseCifDocument of the compiled reference, flattened (see ref_shim.cc): None on
+    io::CifParseError."""
+    b = text.encode()
+    n = C.c_uint64()
+    st = ref().ref_cif_document(b, C.c_uint64(len(b)), None, C.c_uint64(0), C.byref(n))
+    if st == 1:
+        return None
+    assert st == 0
+    buf = C.create_string_buffer(n.value + 1)
+    ref().ref_cif_document(b, C.c_uint64(len(b)), buf, C.c_uint64(n.value + 1), C.byref(n))
+    return buf.raw[:n.value].decode()
+
+
+def ref_try_cif_number(token: str):
+    v, has = C.c_double(), C.c_int()
+    assert ref().ref_cif_try_number(token.encode(), C.byref(v), C.byref(has)) == 0
+    return v.value if has.value else None
+
+
+def ref_symmetry_operation(text: str):
+    """(matrix, translation), or None on io::CifParseError."""
+    m, t = np.zeros(9), np.zeros(3)
+    st = ref().ref_symmetry_operation(text.encode(), ptr(m), ptr(t))
+    return None if st == 1 else (m.reshape(3, 3), t)
+
+
+def ref_element_from_label(label: str):
+    buf = C.create_string_buffer(4)
+    found = C.c_int()
+    assert ref().ref_element_from_label(label.encode(), buf, C.byref(found)) == 0
+    return buf.value.decode() if found.value else None
