@@ -42,6 +42,10 @@ struct NotImplemented : std::logic_error {
 struct DeviceError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+/// Stand-in for aleator::io::CifParseError (io/cif_error.hpp:8-16).
+struct CifParseError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 /// Maps a status of the C ABI onto the reference's exception types.
 inline void check(aleator_status s) {
@@ -53,6 +57,7 @@ inline void check(aleator_status s) {
         case ALEATOR_OUT_OF_RANGE: throw std::out_of_range(msg);
         case ALEATOR_NOT_IMPLEMENTED: throw NotImplemented(msg);
         case ALEATOR_DEVICE_ERROR: throw DeviceError(msg);
+        case ALEATOR_PARSE_ERROR: throw CifParseError(msg);
         default: throw std::runtime_error(msg);
     }
 }
@@ -436,6 +441,75 @@ double widomInsertionHenryCoefficient(const Framework& fw, const std::vector<Sit
     check(aleator_gpu_widom(fw.handle(), s.data(), (int32_t)s.size(), seed, stream, firstCallIndex, numTrials,
                             temperature, precision, du ? du->data() : nullptr, &sum));
     return sum / ((double)numTrials * temperature);
+}
+
+/// core::Lattice as returned by the reader: rows are the lattice vectors (lattice.hpp:21-30).
+struct Lattice {
+    std::array<std::array<double, 3>, 3> m{};
+    [[nodiscard]] const std::array<std::array<double, 3>, 3>& matrix() const noexcept { return m; }
+};
+
+/// The core::ParticleData members a structure file fills (particle_data.hpp:20-99).
+struct ParticleData {
+    std::vector<double> x, y, z, mass, charge;
+    std::vector<std::uint32_t> species;
+    [[nodiscard]] std::size_t size() const noexcept { return x.size(); }
+};
+
+/// io::StructureData (io/structure_io.hpp:18-37).
+struct StructureData {
+    ParticleData particles;
+    Lattice lattice;
+    std::vector<std::string> speciesSymbols;
+    std::vector<double> occupancy;
+    std::vector<std::string> sourceLabel;
+};
+
+namespace detail {
+inline StructureData take_structure(aleator_structure* h) {
+    struct Guard {
+        aleator_structure* h;
+        ~Guard() { aleator_structure_destroy(h); }
+    } guard{h};
+    StructureData s;
+    std::uint64_t n = 0;
+    std::uint32_t ns = 0;
+    check(aleator_structure_size(h, &n, &ns));
+    aleator_lattice lat{};
+    check(aleator_structure_lattice(h, &lat));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) s.lattice.m[i][j] = lat.m[i][j];
+    auto& p = s.particles;
+    for (auto* v : {&p.x, &p.y, &p.z, &p.mass, &p.charge, &s.occupancy}) v->resize(n);
+    p.species.resize(n);
+    check(aleator_structure_atoms(h, p.x.data(), p.y.data(), p.z.data(), p.mass.data(), p.charge.data(),
+                                  p.species.data(), s.occupancy.data()));
+    const char* str = nullptr;
+    for (std::uint32_t i = 0; i < ns; ++i) {
+        check(aleator_structure_species_symbol(h, i, &str));
+        s.speciesSymbols.emplace_back(str);
+    }
+    for (std::uint64_t i = 0; i < n; ++i) {
+        check(aleator_structure_source_label(h, i, &str));
+        s.sourceLabel.emplace_back(str);
+    }
+    return s;
+}
+}  // namespace detail
+
+/// io::readCif (io/structure_io.hpp:39-50): throws CifParseError on malformed input or a file
+/// that cannot be opened.
+inline StructureData readCif(const std::filesystem::path& file) {
+    aleator_structure* h = nullptr;
+    check(aleator_read_cif(file.string().c_str(), &h));
+    return detail::take_structure(h);
+}
+
+/// readCif from CIF text in memory.
+inline StructureData parseCif(const std::string& text) {
+    aleator_structure* h = nullptr;
+    check(aleator_parse_cif(text.data(), text.size(), &h));
+    return detail::take_structure(h);
 }
 
 }  // namespace aleator::gpu

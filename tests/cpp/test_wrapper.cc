@@ -120,8 +120,35 @@ orc_lj olj(const System& s, std::vector<double>& e, std::vector<double>& g) {
     return {e.data(), g.data(), (uint32_t)e.size(), s.lj.rc, s.lj.trunc, s.lj.mixing};
 }
 
+std::filesystem::path g_cif_dir;  // tests/golden/cif, found from the program's own path
+
+void cif_checks() {
+    namespace ag = aleator::gpu;
+    const ag::StructureData s = ag::readCif(g_cif_dir / "IRMOF-1.cif");
+    expect(s.particles.size() == 424 && s.occupancy.size() == 424 && s.sourceLabel.size() == 424,
+           "readCif: IRMOF-1 expands to 424 atoms");
+    expect((s.speciesSymbols == std::vector<std::string>{"Zn", "O", "C", "H"}), "readCif: species by first appearance");
+    expect(s.lattice.matrix()[0][0] == 25.832 && s.lattice.matrix()[1][1] == 25.832, "readCif: cubic cell");
+    double mass = 0.0;
+    for (double m : s.particles.mass) mass += m;
+    expect(std::abs(mass - 6158.936) < 1e-9, "readCif: framework mass");
+    expect(throws<ag::CifParseError>([] { (void)ag::readCif(g_cif_dir / "no_such_file.cif"); }),
+           "readCif: a missing file raises CifParseError");
+    expect(throws<ag::CifParseError>([] { (void)ag::parseCif("data_X\n_cell_length_a 1\n"); }),
+           "parseCif: missing cell tags raise CifParseError");
+    const ag::StructureData t = ag::parseCif(
+        "data_T\n_cell_length_a 10\n_cell_length_b 10\n_cell_length_c 10\n_cell_angle_alpha 90\n"
+        "_cell_angle_beta 90\n_cell_angle_gamma 90\nloop_\n_symmetry_equiv_pos_as_xyz\n'x,y,z'\n'-x,-y,-z'\n"
+        "loop_\n_atom_site_label\n_atom_site_fract_x\n_atom_site_fract_y\n_atom_site_fract_z\n_atom_site_charge\n"
+        "Na1 0.1 0.2 0.3 0.7\nCl1 0.4 0.5 0.6 -0.7\n");
+    expect(t.particles.size() == 4 && t.particles.charge[1] == 0.7 && t.particles.charge[3] == -0.7 &&
+               t.particles.x[1] == 9.0 && t.sourceLabel[2] == "Cl1",
+           "parseCif: charges and labels follow every symmetry image");
+}
+
 void cpu_checks() {
     namespace ag = aleator::gpu;
+    cif_checks();
     // exception mapping: each status code becomes the reference's type
     expect(throws<std::invalid_argument>([] { ag::check(ALEATOR_INVALID_ARGUMENT); }), "status 1 -> invalid_argument");
     expect(throws<std::out_of_range>([] { ag::check(ALEATOR_OUT_OF_RANGE); }), "status 2 -> out_of_range");
@@ -245,6 +272,7 @@ void gpu_checks() {
 
 int main(int argc, char** argv) {
     const std::string mode = argc > 1 ? argv[1] : "cpu";
+    g_cif_dir = std::filesystem::absolute(argv[0]).parent_path() / ".." / ".." / "golden" / "cif";
     try {
         cpu_checks();
         if (mode == "gpu") gpu_checks();

@@ -95,6 +95,12 @@ def oracle():
     return _oracle
 
 
+def pr_fugacity(tc, pc, omega, temperature, pressure) -> float:
+    """PengRobinson::fugacityPascal restated (oracle.c, peng_robinson.cc:10-110)."""
+    return oracle().orc_pr_fugacity(C.c_double(tc), C.c_double(pc), C.c_double(omega), C.c_double(temperature),
+                                    C.c_double(pressure))
+
+
 def have_ref() -> bool:
     return os.path.exists(REF_SO)
 
