@@ -101,30 +101,43 @@ template <>
 __device__ inline double erfc_over_r<double>(double alpha, double r) {
     return erfc(alpha * r) / r;
 }
-// FP32 erfc(z)/r without branches: erfc(z) = t exp(-z^2 + P(t)), t = 1/(1 + z/2), with the
-// degree-9 Chebyshev fit P of Press et al., Numerical Recipes in C (2nd ed.) section 6.2
-// ("erfcc", fractional error below 1.2e-7 for every z >= 0).  -z^2 is split as
-// -zh^2 - (z - zh)(z + zh) with zh = z rounded down to 12 significant bits, so zh^2 is exact
-// in FP32 and exp of the large part carries no argument rounding (the splitting fdlibm's
-// erf uses); the large exponent goes through the accurate expf, the small one (|.| < 2)
-// through the hardware exponential.  FP32 evaluation against the FP64 erfc: relative error
-// below 6e-7 on [0, 4].
+// FP32 erfc(z)/r, z = alpha r, from r and 1/r, without branches: erfc(z) = t exp(w),
+// w = P(t) - z^2, t = 1/(1 + z/2), with the degree-9 Chebyshev fit P of Press et al., Numerical
+// Recipes in C (2nd ed.) section 6.2 ("erfcc", fractional error below 1.2e-7 for every z >= 0).
+// exp(w) = 2^(w log2 e) goes through the hardware exp2; the roundings of z^2 (recovered exactly
+// by an FMA), of P - z^2 (recovered exactly by a two-sum) and of the product by log2 e (an FMA
+// split of log2 e into two floats) are carried as the first-order factor 1 + lo ln 2.  FP32
+// evaluation against the FP64 erfc: relative error below 5e-7 on [0, 4].
+__device__ inline float erfc_over_r_f32(float alpha, float r, float ir) {
+#pragma clang fp contract(off)  // the two-sum below needs every product and sum rounded as written
+    const float z = alpha * r;
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, z, 1.0f));
+    float p = 0.17087277f;
+    p = fmaf(t, p, -0.82215223f);
+    p = fmaf(t, p, 1.48851587f);
+    p = fmaf(t, p, -1.13520398f);
+    p = fmaf(t, p, 0.27886807f);
+    p = fmaf(t, p, -0.18628806f);
+    p = fmaf(t, p, 0.09678418f);
+    p = fmaf(t, p, 0.37409196f);
+    p = fmaf(t, p, 1.00002368f);
+    p = fmaf(t, p, -1.26551223f);
+    const float h = z * z;
+    const float l = fmaf(z, z, -h);  // z^2 = h + l exactly
+    const float s = p - h;           // two-sum: p - h = s + e exactly
+    const float bb = s - p;
+    const float e = (p - (s - bb)) + (-h - bb);
+    const float lo = e - l;  // w = s + lo
+    constexpr float kLog2e = 1.44269502162933349609375f;  // log2(e) rounded to float
+    constexpr float kLog2eLo = 1.925963e-08f;             // log2(e) - kLog2e
+    constexpr float kLn2 = 0.693147182464599609375f;
+    const float y = s * kLog2e;
+    const float ylo = fmaf(s, kLog2e, -y) + fmaf(s, kLog2eLo, lo * kLog2e);
+    return t * (__builtin_amdgcn_exp2f(y) * fmaf(ylo, kLn2, 1.0f)) * ir;
+}
 template <>
 __device__ inline float erfc_over_r<float>(float alpha, float r) {
-    const float z = alpha * r;
-    const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * z);
-    float p = 0.17087277f;
-    p = -0.82215223f + t * p;
-    p = 1.48851587f + t * p;
-    p = -1.13520398f + t * p;
-    p = 0.27886807f + t * p;
-    p = -0.18628806f + t * p;
-    p = 0.09678418f + t * p;
-    p = 0.37409196f + t * p;
-    p = 1.00002368f + t * p;
-    const float zh = __uint_as_float(__float_as_uint(z) & 0xfffff000u);
-    const float small = -(z - zh) * (z + zh) + (-1.26551223f + t * p);
-    return t * expf(-zh * zh) * __expf(small) * __builtin_amdgcn_rcpf(r);
+    return erfc_over_r_f32(alpha, r, __builtin_amdgcn_rcpf(r));
 }
 
 // ---------------------------------------------------------------- complex helpers

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "device.hpp"
@@ -64,10 +65,13 @@ template <typename R, int NG, bool COUL, int TX, int TY, int TZ, int NPT>
 __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __restrict__ descs,
                                                              const int* __restrict__ prefix, int n_desc) {
     static_assert(TX * TY * TZ == kThreads, "one thread per (x, y, z-group)");
+    constexpr bool kF32 = std::is_same_v<R, float>;
     constexpr int NZ = TZ * NPT;  // nodes per tile along z
     __shared__ Entries<R> ent;
     __shared__ double tab[4][kMaxGuests][kMaxTypes];
-    __shared__ R tabr[2][kMaxGuests][kMaxTypes];  // sigma^2 and 4 eps in the pair precision
+    // FP32 pair constants: sigma^2, 4 eps, the shift V(rc) (0 when truncated) and the slope
+    // V'(rc) (0 unless linear-force-shifted), so every truncation takes the same instructions
+    __shared__ float tabf[4][kMaxGuests][kMaxTypes];
     __shared__ int scan_tmp[kThreads / 64 + 1];
 
     // framework of this workgroup
@@ -83,8 +87,15 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
 
     const int tid = threadIdx.x;
     for (int i = tid; i < 4 * kMaxGuests * kMaxTypes; i += kThreads) {
-        (&tab[0][0][0])[i] = a.ljtab[i];
-        if (i < 2 * kMaxGuests * kMaxTypes) (&tabr[0][0][0])[i] = (R)a.ljtab[i];
+        const double v = a.ljtab[i];
+        if constexpr (kF32) {
+            const int k = i / (kMaxGuests * kMaxTypes);
+            const bool keep = k < 2 || (k == 2 && a.trunc != ALEATOR_LJ_TRUNCATED) ||
+                              (k == 3 && a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED);
+            (&tabf[0][0][0])[i] = keep ? (float)v : 0.0f;
+        } else {
+            (&tab[0][0][0])[i] = v;
+        }
     }
 
     const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
@@ -132,7 +143,45 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
 
     auto compute = [&](int count) {
         __syncthreads();
-        if (any_valid) {
+        if constexpr (kF32) {
+            // 1/r from one reciprocal square root: r = r2 / sqrt(r2), 1/r^2 = (1/r)^2
+            if (any_valid) {
+#pragma unroll 2
+                for (int e = 0; e < count; ++e) {
+                    const float ex = ent.x[e], ey = ent.y[e], ez = ent.z[e];
+                    const int t = ent.t[e];
+                    float s2c[NG > 0 ? NG : 1], e4c[NG > 0 ? NG : 1], shc[NG > 0 ? NG : 1], slc[NG > 0 ? NG : 1];
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        s2c[g] = tabf[0][g][t];
+                        e4c[g] = tabf[1][g][t];
+                        shc[g] = tabf[2][g][t];
+                        slc[g] = tabf[3][g][t];
+                    }
+#pragma unroll
+                    for (int p = 0; p < NPT; ++p) {
+                        const float dx = ex - ox[p];
+                        const float dy = ey - oy[p];
+                        const float dz = ez - oz[p];
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        const float ir = __builtin_amdgcn_rsqf(r2);
+                        const float r = r2 * ir;
+                        if (NG > 0 && r2 <= rc2) {
+                            const float ir2 = ir * ir;
+                            const float dr = r - rc;
+#pragma unroll
+                            for (int g = 0; g < NG; ++g) {
+                                const float s2 = s2c[g] * ir2;
+                                const float s6 = s2 * s2 * s2;
+                                const float v = e4c[g] * fmaf(s6, s6, -s6);
+                                acc[p][g] += (double)fmaf(-dr, slc[g], v - shc[g]);
+                            }
+                        }
+                        if (COUL && r2 <= rcr2) accq[p] += (double)(ent.q[e] * erfc_over_r_f32(alpha, r, ir));
+                    }
+                }
+            }
+        } else if (any_valid) {
 #pragma unroll 2
             for (int e = 0; e < count; ++e) {
                 const R ex = ent.x[e], ey = ent.y[e], ez = ent.z[e];
@@ -140,8 +189,8 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
                 R s2c[NG > 0 ? NG : 1], e4c[NG > 0 ? NG : 1];
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
-                    s2c[g] = tabr[0][g][t];
-                    e4c[g] = tabr[1][g][t];
+                    s2c[g] = (R)tab[0][g][t];
+                    e4c[g] = (R)tab[1][g][t];
                 }
 #pragma unroll
                 for (int p = 0; p < NPT; ++p) {
