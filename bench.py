@@ -216,6 +216,33 @@ def cpu_baseline_c1(w, sample_nodes=1200, seed=3):
                       f"{dt:.1f} s; CPU {cpu_model()}, {os.cpu_count()} logical cores on host"}
 
 
+LEGS = ("c1", "c0", "c2", "c3", "mlp", "c4")
+
+
+def mlp_training_rate(aa, ctx, sync, batch=4096, iters=500):
+    """Adam steps of the insertion-density MLP (64 Fourier features, 4 x 256 SiLU, 1 output) on a
+    smooth periodic target over 2e5 resident rows: forward + backward + Adam per step, replayed
+    from its hipGraph.  FLOP per step = 3 x 2 x (weights of the 5 layers) x batch."""
+    import torch
+
+    rng = np.random.default_rng(7)
+    frac = rng.random((200_000, 3))
+    target = np.sin(2 * np.pi * frac[:, 0]) * np.cos(2 * np.pi * frac[:, 1]) + 0.5 * np.cos(4 * np.pi * frac[:, 2])
+    d_f = torch.from_numpy(frac).cuda()
+    d_t = torch.from_numpy(target).cuda()
+    net = aa.Mlp(ctx, max_rows=batch, seed=1)
+    net.train_device(d_f.data_ptr(), d_t.data_ptr(), len(frac), batch, 20, 1e-3, seed=1, stream=0)  # capture + warm
+    sync()
+    t0 = time.perf_counter()
+    losses = net.train_device(d_f.data_ptr(), d_t.data_ptr(), len(frac), batch, iters, 1e-3, seed=2, stream=0)
+    sync()
+    dt = (time.perf_counter() - t0) / iters
+    weights = 64 * 256 + 3 * 256 * 256 + 256
+    flop = 3 * 2 * weights * batch
+    return {"mlp_ms_per_adam_step_batch4096": dt * 1e3, "mlp_tflops": flop / dt / 1e12,
+            "mlp_final_loss": float(np.mean(losses[-20:]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,8 +255,15 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM byte counters)")
     ap.add_argument("--c4-plain-steps", type=int, default=1_000_000)
+    ap.add_argument("--legs", default="all",
+                    help="comma-separated extra legs to run after the headline: " + ",".join(LEGS) + " (or all)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     args = ap.parse_args()
+    legs = set() if args.headline_only else (set(LEGS) if args.legs == "all" else set(args.legs.split(",")))
+    if legs - set(LEGS):
+        ap.error(f"unknown legs {sorted(legs - set(LEGS))}")
+    if args.no_c4:
+        legs.discard("c4")
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the hot path would shard by grid slab, trial range or chain, with no exchange "
@@ -280,10 +314,11 @@ def main():
             dst.copy_(src)
         sync()
         extra["copy_calibration_bytes_each_way"] = src.numel() * 4
-    if not args.headline_only:
+    if "c1" in legs:
         t = timed(lambda: fw.grid_device([4, 5], w["counts"], out.data_ptr(), coulomb=True, precision=prec,
                                          grid_lattice=w["prim"]), 5, 1, sync)
         extra["c1_grid_per_call_entry_point_pts_per_s"] = 3 * nodes / t
+    if "c0" in legs:
         c0 = c0_workload(aa)
         fw0 = aa.Framework(ctx, c0["m"], c0["x"], c0["y"], c0["z"], c0["sp"], c0["eps"], c0["sig"], c0["rc"])
         n0 = int(np.prod(c0["counts"]))
@@ -304,9 +339,11 @@ def main():
         t = timed(lambda: fw.widom_device(co2, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
                                           precision=prec), 3, 1, sync)
         extra[f"c1_widom_co2_{n_trials}_{args.precision}_inserts_per_s"] = n_trials / t
+        del du
 
-        from aleator_amd import workloads as wl
+    from aleator_amd import workloads as wl
 
+    if "c2" in legs:
         # c2: 256 frameworks, one batched (graph-captured, autotuned) launch per step, FP32 pair terms
         try:
             fws2, eps2, sig2 = wl.c2(256)
@@ -328,6 +365,7 @@ def main():
         except Exception as e:  # report, do not hide the headline
             extra["c2_error"] = repr(e)
 
+    if "c3" in legs:
         # c3: 64 CO2 GCMC chains on the charged framework, one persistent launch
         try:
             chains = wl.c3_chains()
@@ -351,12 +389,18 @@ def main():
         except Exception as e:
             extra["c3_error"] = repr(e)
 
+    if "mlp" in legs:
+        try:
+            extra.update(mlp_training_rate(aa, ctx, sync))
+        except Exception as e:
+            extra["mlp_error"] = repr(e)
+
+    if "c4" in legs:
         # c4: learned insertion density vs plain uniform insertion on the c3 chains
-        if not args.no_c4:
-            try:
-                extra.update(c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=args.c4_plain_steps))
-            except Exception as e:
-                extra["c4_error"] = repr(e)
+        try:
+            extra.update(c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=args.c4_plain_steps))
+        except Exception as e:
+            extra["c4_error"] = repr(e)
 
     cpu = None
     if not args.no_cpu_baseline and not args.headline_only:
