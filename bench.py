@@ -152,16 +152,17 @@ def c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=1_000_000, learned_max
 
 
 def measured_traffic(tile_shape):
-    """HBM bytes per launch of the headline tile kernel, from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE runs scaled by a device copy of known size), when it was taken with
-    the tile shape this run uses; else None."""
+    """HBM bytes per launch of the headline tile kernel with the tile shape this run uses, from the
+    committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE runs scaled by a device copy of
+    known size); None when the summary has no figure for that shape."""
     import re
     try:
         with open(os.path.join(ROOT, "profiles", "grid_tile_kernel_c1_co2_3tables.md")) as f:
-            m = re.search(r"traffic_bytes_per_launch: (\d+) \(tile shape (\d+)\)", f.read())
+            lines = re.findall(r"traffic_bytes_per_launch: (\d+) \(tile shape (\d+)\)", f.read())
     except OSError:
         return None
-    return float(m.group(1)) if m and int(m.group(2)) == tile_shape else None
+    by_shape = {int(k): float(v) for v, k in lines}
+    return by_shape.get(tile_shape)
 
 
 def cpu_model():
