@@ -196,10 +196,11 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
                             }
                         }
                         if (COUL && r2 <= rcr2) {
-                            const float u = fminf(r * erfc_scale, (float)kErfcIntervals - 1e-3f);
-                            const float fl = floorf(u);
-                            const float f = u - fl;
-                            const float4 c = erfc_tab[(int)fl];
+                            // r <= rc(1 + 2^-22) here, so 0 <= u < kErfcIntervals and the piece
+                            // index (int)u = floor(u) stays inside the table
+                            const float u = r * erfc_scale;
+                            const float f = __builtin_amdgcn_fractf(u);
+                            const float4 c = erfc_tab[(int)u];
                             const float erfcz = fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
                             accq[p] += (double)(ent.q[e] * (erfcz * ir));
                         }
