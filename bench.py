@@ -217,7 +217,95 @@ def cpu_baseline_c1(w, sample_nodes=1200, seed=3):
                       f"{dt:.1f} s; CPU {cpu_model()}, {os.cpu_count()} logical cores on host"}
 
 
-LEGS = ("c1", "c0", "c2", "c3", "mlp", "c4")
+LEGS = ("c1", "c0", "c2", "c3", "mlp", "irmof1", "c4")
+
+
+def irmof1_gcmc_rates(aa, ctx, sync):
+    """GCMC on the reference's real IRMOF-1 systems (CIFs under tests/golden/cif/), device chains
+    against the reference engine on one host core:
+      methane, UFF, rc 12 A shifted, 298 K, 0.1 and 10 bar (Peng-Robinson), 424 atoms;
+      CO2, DDEC charges, 2x2x2 (848 atoms), Ewald alpha 0.25 kmax 6, rc 12.8 A, 1e6 Pa, the
+      reference's high-occupancy benchmark point (bench_gcmc_co2_irmof1.cc:116-160).
+    Per-step latency of one chain, aggregate chain-steps/s of many, and the reference CLI's
+    <N> (seed 2026, 30000 + 90000 steps at 0.1 bar) computed on the device."""
+    cif = os.path.join(ROOT, "tests", "golden", "cif")
+    uff = {"Zn": (62.3992, 2.46155), "O": (48.1581, 3.03315), "C": (47.8562, 3.47299), "H": (7.64893, 2.84642)}
+    out = {}
+
+    def timed_run(gc, steps):
+        sync()
+        t0 = time.perf_counter()
+        gc.run(steps)
+        sync()
+        return time.perf_counter() - t0
+
+    s = aa.read_cif(os.path.join(cif, "IRMOF-1.cif"))
+    eps = [uff[e][0] for e in s.species_symbols] + [158.5]
+    sig = [uff[e][1] for e in s.species_symbols] + [3.72]
+    fw = aa.Framework(ctx, s.lattice, s.x, s.y, s.z, s.species, eps, sig, 12.0, truncation=aa.SHIFTED)
+    ch4 = [(0.0, 0.0, 0.0, len(s.species_symbols), 0.0)]
+    for bar in (0.1, 10.0):
+        f = aa.peng_robinson_fugacity(190.6, 4.599e6, 0.011, 298.0, bar * 1e5)
+        for n in (1, 256):
+            gc = aa.Gcmc(fw, ch4, [(f, 298.0, 2026, c) for c in range(n)], max_molecules=1024)
+            gc.run(2000)
+            dt = timed_run(gc, 20000)
+            key = f"irmof1_ch4_{bar:g}bar_{n}chains"
+            out[key + "_us_per_step"] = dt / 20000 * 1e6
+            out[key + "_chain_steps_per_s"] = n * 20000 / dt
+            gc.close()
+    f = aa.peng_robinson_fugacity(190.6, 4.599e6, 0.011, 298.0, 0.1e5)
+    gc = aa.Gcmc(fw, ch4, [(f, 298.0, 2026, 0)], max_molecules=1024)
+    gc.run(30000)
+    gc.reset_stats()
+    gc.run(90000)
+    st = gc.stats()[0]
+    out["irmof1_ch4_cli_run_mean_N"] = st["sum_molecules"] / st["steps"]  # the reference CLI prints 0.231456
+    gc.close()
+
+    d = aa.read_cif(os.path.join(cif, "IRMOF-1_primitive_DDEC.cif"))
+    tot = 0.0
+    for v in d.charge:
+        tot += float(v)
+    q = d.charge - tot / len(d.charge)
+    lat, x, y, z, qq, sp = aa.replicate_supercell(d.lattice, d.x, d.y, d.z, q, d.species, (2, 2, 2))
+    ns = len(d.species_symbols)
+    eps = [uff[e][0] for e in d.species_symbols] + [27.0, 79.0]
+    sig = [uff[e][1] for e in d.species_symbols] + [2.8, 3.05]
+    fw2 = aa.Framework(ctx, lat, x, y, z, sp, eps, sig, 12.8, truncation=aa.SHIFTED, charge=qq, ewald=(0.25, 12.8, 6))
+    co2 = [(0.0, 0.0, -1.16, ns + 1, -0.35), (0.0, 0.0, 0.0, ns, 0.70), (0.0, 0.0, 1.16, ns + 1, -0.35)]
+    f = aa.peng_robinson_fugacity(304.1282, 7377300.0, 0.22394, 298.0, 1.0e6)
+    for n in (1, 64):
+        gc = aa.Gcmc(fw2, co2, [(f, 298.0, 2026, c) for c in range(n)], max_molecules=1024)
+        gc.run(2000)
+        dt = timed_run(gc, 2000)
+        key = f"irmof1_co2_1e6Pa_{n}chains"
+        out[key + "_us_per_step"] = dt / 2000 * 1e6
+        out[key + "_chain_steps_per_s"] = n * 2000 / dt
+        out[key + "_molecules_after_4000_steps"] = float(np.mean([c["n_molecules"] for c in gc.stats()]))
+        gc.close()
+
+    # the reference engine (compiled from its sources into oracle/_ref), one host core
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    if ol.have_ref():
+        import irmof1_systems as sy
+        m = sy.methane(0)
+        t0 = time.perf_counter()
+        ol.ref_gcmc_run(m.lj, m.fw, m.sites, m.fugacity, 298.0, 2026, 0, 20000)
+        out["irmof1_cpu_reference_ch4_0.1bar_us_per_step"] = (time.perf_counter() - t0) / 20000 * 1e6
+        m = sy.methane(3)
+        t0 = time.perf_counter()
+        ol.ref_gcmc_run(m.lj, m.fw, m.sites, m.fugacity, 298.0, 2026, 3, 20000)
+        out["irmof1_cpu_reference_ch4_10bar_us_per_step"] = (time.perf_counter() - t0) / 20000 * 1e6
+        c = sy.co2(0)
+        times = []
+        for steps in (2000, 2500):
+            t0 = time.perf_counter()
+            ol.ref_gcmc_run(c.lj, c.fw, c.sites, f, 298.0, 2026, 0, steps, ewald=c.ewald)
+            times.append(time.perf_counter() - t0)
+        out["irmof1_cpu_reference_co2_1e6Pa_us_per_step_after_2000"] = (times[1] - times[0]) / 500 * 1e6
+    return out
 
 
 def mlp_training_rate(aa, ctx, sync, batch=4096, iters=500):
@@ -395,6 +483,12 @@ def main():
             extra.update(mlp_training_rate(aa, ctx, sync))
         except Exception as e:
             extra["mlp_error"] = repr(e)
+
+    if "irmof1" in legs:
+        try:
+            extra.update(irmof1_gcmc_rates(aa, ctx, sync))
+        except Exception as e:
+            extra["irmof1_error"] = repr(e)
 
     if "c4" in legs:
         # c4: learned insertion density vs plain uniform insertion on the c3 chains
