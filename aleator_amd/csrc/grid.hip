@@ -70,9 +70,10 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
     constexpr int NZ = TZ * NPT;  // nodes per tile along z
     __shared__ Entries<R> ent;
     __shared__ double tab[4][kMaxGuests][kMaxTypes];
-    // FP32 pair constants: sigma^2, 4 eps, the shift V(rc) (0 when truncated) and the slope
-    // V'(rc) (0 unless linear-force-shifted), so every truncation takes the same instructions
-    __shared__ float tabf[4][kMaxGuests][kMaxTypes];
+    // FP32 pair constants per (guest, type): A = 4 eps sigma^12, B = 4 eps sigma^6, the shift
+    // V(rc) (0 when truncated) and the slope V'(rc) (0 unless linear-force-shifted), so every
+    // truncation takes the same instructions: V = A/r^12 - B/r^6 - shift - (r - rc) slope
+    __shared__ float4 tabf[kMaxGuests][kMaxTypes];
     __shared__ float4 erfc_tab[(std::is_same_v<R, float> && COUL) ? kErfcIntervals : 1];
     __shared__ int scan_tmp[kThreads / 64 + 1];
 
@@ -88,16 +89,17 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
     const int tile = b - prefix[lo];
 
     const int tid = threadIdx.x;
-    for (int i = tid; i < 4 * kMaxGuests * kMaxTypes; i += kThreads) {
-        const double v = a.ljtab[i];
-        if constexpr (kF32) {
-            const int k = i / (kMaxGuests * kMaxTypes);
-            const bool keep = k < 2 || (k == 2 && a.trunc != ALEATOR_LJ_TRUNCATED) ||
-                              (k == 3 && a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED);
-            (&tabf[0][0][0])[i] = keep ? (float)v : 0.0f;
-        } else {
-            (&tab[0][0][0])[i] = v;
+    if constexpr (kF32) {
+        constexpr int kPairs = kMaxGuests * kMaxTypes;
+        for (int i = tid; i < kPairs; i += kThreads) {
+            const double s2 = a.ljtab[i], e4 = a.ljtab[kPairs + i];
+            const double s6 = s2 * s2 * s2;
+            const double shift = a.trunc != ALEATOR_LJ_TRUNCATED ? a.ljtab[2 * kPairs + i] : 0.0;
+            const double slope = a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED ? a.ljtab[3 * kPairs + i] : 0.0;
+            (&tabf[0][0])[i] = make_float4((float)(e4 * s6 * s6), (float)(e4 * s6), (float)shift, (float)slope);
         }
+    } else {
+        for (int i = tid; i < 4 * kMaxGuests * kMaxTypes; i += kThreads) (&tab[0][0][0])[i] = a.ljtab[i];
     }
 
     // erfc(alpha r) for the FP32 pair loop: piece k covers z = alpha r in [k h, (k+1) h],
@@ -168,14 +170,9 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
                 for (int e = 0; e < count; ++e) {
                     const float ex = ent.x[e], ey = ent.y[e], ez = ent.z[e];
                     const int t = ent.t[e];
-                    float s2c[NG > 0 ? NG : 1], e4c[NG > 0 ? NG : 1], shc[NG > 0 ? NG : 1], slc[NG > 0 ? NG : 1];
+                    float4 cg[NG > 0 ? NG : 1];
 #pragma unroll
-                    for (int g = 0; g < NG; ++g) {
-                        s2c[g] = tabf[0][g][t];
-                        e4c[g] = tabf[1][g][t];
-                        shc[g] = tabf[2][g][t];
-                        slc[g] = tabf[3][g][t];
-                    }
+                    for (int g = 0; g < NG; ++g) cg[g] = tabf[g][t];
 #pragma unroll
                     for (int p = 0; p < NPT; ++p) {
                         const float dx = ex - ox[p];
@@ -186,13 +183,13 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
                         const float r = r2 * ir;
                         if (NG > 0 && r2 <= rc2) {
                             const float ir2 = ir * ir;
+                            const float ir6 = ir2 * ir2 * ir2;
+                            const float ir12 = ir6 * ir6;
                             const float dr = r - rc;
 #pragma unroll
                             for (int g = 0; g < NG; ++g) {
-                                const float s2 = s2c[g] * ir2;
-                                const float s6 = s2 * s2 * s2;
-                                const float v = e4c[g] * fmaf(s6, s6, -s6);
-                                acc[p][g] += (double)fmaf(-dr, slc[g], v - shc[g]);
+                                const float w = fmaf(cg[g].y, ir6, fmaf(dr, cg[g].w, cg[g].z));
+                                acc[p][g] += (double)fmaf(cg[g].x, ir12, -w);
                             }
                         }
                         if (COUL && r2 <= rcr2) {
