@@ -129,10 +129,12 @@ void reciprocal_delta(Framework& fw, double* d_sre, double* d_sim, uint64_t n_pr
     const size_t lds = (size_t)4 * 3 * (2 * K + 1) * (ns > 0 ? ns : 1) * sizeof(cplx);
     const double scale = 2.0 * kCoulomb * (2.0 * kPi / std::abs(fw.lat.volume));
     const unsigned blocks = (unsigned)((n_prop + 3) / 4);
+    ctx.time_begin();
     hipLaunchKernelGGL(recip_delta_kernel, dim3(blocks), dim3(256), lds, ctx.stream, d_sites, n_prop,
                        n_rem, n_add, reinterpret_cast<const int4*>(fw.d_kn), fw.d_kw, d_sre, d_sim,
                        fw.nk_half, K, scale, d_out, commit ? 1 : 0);
     ALEATOR_HIP_CHECK(hipGetLastError());
+    ctx.time_end();
 }
 
 }  // namespace aleator_gpu

@@ -241,7 +241,40 @@ def host_threads() -> int:
         return os.cpu_count() or 1
 
 
-LEGS = ("c1", "c0", "c2", "c3", "mlp", "irmof1", "c4")
+LEGS = ("c1", "c0", "c2", "c3", "ewald", "mlp", "irmof1", "c4")
+
+
+def ewald_proposal_rate(aa, ctx, fw, sync, n_prop=100_000):
+    """EwaldIncrementalState proposals against the c1 framework's S(k) (half set 1102 k), batched
+    one wave per proposal: CO2 insertions (3 sites) and moves (3 removed + 3 added).  Algorithmic
+    work per proposal: sites x (Nk/2) x 16 FLOP (FP64); S(k) state read: (Nk/2) x 40 B
+    (k index 16 B, weight 8 B, S 16 B), from L2."""
+    rng = np.random.default_rng(11)
+    nk = fw.num_k // 2
+    st = aa.EwaldState(fw)
+    co2 = np.array([(0.0, 0.0, -1.16, -0.35), (0.0, 0.0, 0.0, 0.70), (0.0, 0.0, 1.16, -0.35)])
+
+    def molecules(n):
+        c = rng.random((n, 1, 3)) @ np.asarray(fw.lattice)
+        out = np.repeat(co2[None], n, axis=0)
+        out[:, :, :3] += c
+        return out
+
+    res = {}
+    for name, removed, added in (("insert", np.zeros((n_prop, 0, 4)), molecules(n_prop)),
+                                 ("move", molecules(n_prop), molecules(n_prop))):
+        st.propose_batch(removed[:1000], added[:1000])  # warm
+        ctx.timing(True)
+        for _ in range(3):
+            st.propose_batch(removed, added)
+        ms, n = ctx.timing(False)
+        t = ms / max(n, 1) / 1e3
+        sites = removed.shape[1] + added.shape[1]
+        res[f"ewald_{name}_proposals_per_s"] = n_prop / t
+        res[f"ewald_{name}_fp64_tflops"] = n_prop * sites * nk * 16 / t / 1e12
+        res[f"ewald_{name}_state_read_gb_per_s"] = n_prop * nk * 40 / t / 1e9
+    st.close()
+    return res
 
 
 def irmof1_gcmc_rates(aa, ctx, sync):
@@ -501,6 +534,12 @@ def main():
             gc.close()
         except Exception as e:
             extra["c3_error"] = repr(e)
+
+    if "ewald" in legs:
+        try:
+            extra.update(ewald_proposal_rate(aa, ctx, fw, sync))
+        except Exception as e:
+            extra["ewald_error"] = repr(e)
 
     if "mlp" in legs:
         try:
