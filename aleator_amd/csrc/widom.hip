@@ -18,7 +18,9 @@
 // Kernel 2 (charged only; one wave per trial): per-axis phase factors of the sites in
 // LDS, then the half k-set sum of recip.hpp.  Both write per-workgroup partial sums of
 // exp(-dU/T), reduced in a fixed order by kernel 3.
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "device.hpp"
@@ -235,12 +237,304 @@ __global__ __launch_bounds__(256) void widom_recip_kernel(const WidomArgs a) {
     if (threadIdx.x == 0) a.block_sums[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// ---------------------------------------------------------------- cell-sorted direct sum
+// Trials are grouped by the cell of their reference point in a Cartesian grid of cubes over
+// the lattice's bounding box (about one pass of a workgroup's trials per occupied cube).  One workgroup per cell builds,
+// as the grid tile kernel does, an LDS list of the framework images within
+// R = max cutoff + cell half-diagonal + molecule radius of the cell centre (atom order, block
+// scan), and each thread sums TPT trials' sites against it.  Every image within the cutoff of
+// a site lies in the list, and at most one image of an atom does (cutoff <= L_perp/2), so the
+// sum is the reference's minimum-image sum, taken in atom order per trial.
+constexpr int kWCap = 1024;   // LDS list capacity (entries)
+constexpr int kWChunk = 128;  // atoms tested per pass (8 images each <= kWCap)
+
+struct CellArgs {
+    int nc[3];         // cubes per Cartesian axis
+    double lo[3];      // bounding-box corner
+    double edge;       // cube edge
+    double rlist2;     // R^2
+    int single_image;  // 2R < min L_perp: only the nearest image of an atom can be within R
+    const uint32_t* start;  // [n_cells + 1]
+    const uint32_t* order;  // trial indices grouped by cell
+};
+
+__device__ inline uint32_t trial_cell(const WidomArgs& a, const CellArgs& c, uint64_t t) {
+#pragma clang fp contract(off)
+    const uint64_t call = a.first + t * (uint64_t)a.draws;
+    const double u0 = philox_uniform(a.seed, a.stream, call);
+    const double u1 = philox_uniform(a.seed, a.stream, call + 1);
+    const double u2 = philox_uniform(a.seed, a.stream, call + 2);
+    int i[3];
+    for (int j = 0; j < 3; ++j) {
+        const double x = a.m[j] * u0 + a.m[3 + j] * u1 + a.m[6 + j] * u2;  // the trial's reference point
+        i[j] = max(0, min((int)floor((x - c.lo[j]) / c.edge), c.nc[j] - 1));
+    }
+    return (uint32_t)((i[2] * c.nc[1] + i[1]) * c.nc[0] + i[0]);
+}
+
+__global__ void widom_key_kernel(const WidomArgs a, const CellArgs c, uint32_t* key, uint32_t* count) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_trials) return;
+    const uint32_t k = trial_cell(a, c, t);
+    key[t] = k;
+    atomicAdd(&count[k], 1u);
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// start[k] = sum of count[<k]; cursor = start; start[n] = total.  One workgroup of 1024.
+__global__ __launch_bounds__(1024) void scan_counts_kernel(const uint32_t* count, uint32_t n, uint32_t* start,
+                                                           uint32_t* cursor) {
+    __shared__ int tmp[17];
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t lo = min(threadIdx.x * per, n), hi = min(lo + per, n);
+    int s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += (int)count[i];
+    int total;
+    int run = block_exclusive_scan(s, tmp, &total);
+    for (uint32_t i = lo; i < hi; ++i) {
+        start[i] = (uint32_t)run;
+        cursor[i] = (uint32_t)run;
+        run += (int)count[i];
+    }
+    if (threadIdx.x == 0) start[n] = (uint32_t)total;
+}
+
+__global__ void scatter_kernel(const uint32_t* key, uint64_t n, uint32_t* cursor, uint32_t* order) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    order[atomicAdd(&cursor[key[t]], 1u)] = (uint32_t)t;
+}
+
+template <typename R>
+struct WEntries {
+    R x[kWCap], y[kWCap], z[kWCap], q[kWCap];
+    int t[kWCap];
+};
+
+template <typename R, int NS, bool CHARGED, int TPT>
+__global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a, const CellArgs c) {
+    constexpr bool kF32 = std::is_same_v<R, float>;
+    __shared__ WEntries<R> ent;
+    __shared__ double tab[4][NS][kMaxTypes];
+    __shared__ float4 tabf[NS][kMaxTypes];  // A = 4 eps s^12, B = 4 eps s^6, shift, slope
+    __shared__ float4 erfc_tab[(kF32 && CHARGED) ? kWThreads : 1];
+    __shared__ int scan_tmp[kWThreads / 64 + 1];
+    const int tid = threadIdx.x;
+    const int cell = blockIdx.x;
+    const uint32_t lo = c.start[cell], hi = c.start[cell + 1];
+    if (lo == hi) return;  // uniform over the workgroup
+
+    if constexpr (kF32) {
+        for (int i = tid; i < NS * kMaxTypes; i += kWThreads) {
+            const int s = i / kMaxTypes, ty = i % kMaxTypes;
+            const double s2 = a.ljtab[(0 * kMaxSites + s) * kMaxTypes + ty];
+            const double e4 = a.ljtab[(1 * kMaxSites + s) * kMaxTypes + ty];
+            const double s6 = s2 * s2 * s2;
+            const double shift = a.trunc != ALEATOR_LJ_TRUNCATED ? a.ljtab[(2 * kMaxSites + s) * kMaxTypes + ty] : 0.0;
+            const double slope =
+                a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED ? a.ljtab[(3 * kMaxSites + s) * kMaxTypes + ty] : 0.0;
+            tabf[s][ty] = make_float4((float)(e4 * s6 * s6), (float)(e4 * s6), (float)shift, (float)slope);
+        }
+    } else {
+        for (int i = tid; i < 4 * NS * kMaxTypes; i += kWThreads) {
+            const int k = i / (NS * kMaxTypes), rem = i % (NS * kMaxTypes);
+            (&tab[0][0][0])[i] = a.ljtab[(k * kMaxSites + rem / kMaxTypes) * kMaxTypes + rem % kMaxTypes];
+        }
+    }
+    // erfc(alpha r) pieces for the FP32 path, as in the grid tile kernel
+    float erfc_scale = 0.0f;
+    if constexpr (kF32 && CHARGED) {
+        const double h = a.alpha * sqrt(a.rc2_real) / (kWThreads - 1);
+        const double z0 = tid * h, z1 = z0 + h;
+        const double f0 = erfc(z0), f1 = erfc(z1);
+        const double d0 = -1.1283791670955126 * exp(-z0 * z0) * h, d1 = -1.1283791670955126 * exp(-z1 * z1) * h;
+        erfc_tab[tid] = make_float4((float)f0, (float)d0, (float)(3.0 * (f1 - f0) - (2.0 * d0 + d1)),
+                                    (float)(2.0 * (f0 - f1) + (d0 + d1)));
+        erfc_scale = (float)(a.alpha / h);
+    }
+
+    // cube centre, Cartesian and fractional
+    const int i0 = cell % c.nc[0], i1 = (cell / c.nc[0]) % c.nc[1], i2 = cell / (c.nc[0] * c.nc[1]);
+    const double cc[3] = {c.lo[0] + (i0 + 0.5) * c.edge, c.lo[1] + (i1 + 0.5) * c.edge, c.lo[2] + (i2 + 0.5) * c.edge};
+    double cf[3];
+    to_frac(a, cc, cf);
+
+    const R rc2 = (R)a.rc2_lj, rcr2 = (R)a.rc2_real, rc = (R)a.rc_lj, alpha = (R)a.alpha;
+    for (uint32_t pass = lo; pass < hi; pass += kWThreads * TPT) {
+        uint64_t tr[TPT];
+        bool ok[TPT];
+        R ox[TPT][NS], oy[TPT][NS], oz[TPT][NS];
+        double lj[TPT], real[TPT];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            const uint32_t idx = pass + k * kWThreads + tid;
+            ok[k] = idx < hi;
+            tr[k] = ok[k] ? c.order[idx] : 0;
+            lj[k] = 0.0;
+            real[k] = 0.0;
+            double cart[kMaxSites][3];
+            if (ok[k]) trial_sites(a, tr[k], cart);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                ox[k][s] = ok[k] ? (R)(cart[s][0] - cc[0]) : (R)1e6;
+                oy[k][s] = ok[k] ? (R)(cart[s][1] - cc[1]) : (R)1e6;
+                oz[k][s] = ok[k] ? (R)(cart[s][2] - cc[2]) : (R)1e6;
+            }
+        }
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) any |= ok[k];
+
+        auto compute = [&](int count) {
+            __syncthreads();
+            if (any) {
+                for (int e = 0; e < count; ++e) {
+                    const R ex = ent.x[e], ey = ent.y[e], ez = ent.z[e];
+                    const int ty = ent.t[e];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if constexpr (kF32) {
+                            const float4 cg = tabf[s][ty];
+#pragma unroll
+                            for (int k = 0; k < TPT; ++k) {
+                                const float dx = ex - ox[k][s], dy = ey - oy[k][s], dz = ez - oz[k][s];
+                                const float r2 = dx * dx + dy * dy + dz * dz;
+                                const float ir = __builtin_amdgcn_rsqf(r2);
+                                const float r = r2 * ir;
+                                if (r2 <= rc2) {
+                                    const float ir2 = ir * ir;
+                                    const float ir6 = ir2 * ir2 * ir2;
+                                    const float w = fmaf(cg.y, ir6, fmaf(r - rc, cg.w, cg.z));
+                                    lj[k] += (double)fmaf(cg.x, ir6 * ir6, -w);
+                                }
+                                if (CHARGED && r2 <= rcr2) {
+                                    const float u = r * erfc_scale;
+                                    const float f = __builtin_amdgcn_fractf(u);
+                                    const float4 cf4 = erfc_tab[(int)u];
+                                    const float ez4 = fmaf(fmaf(fmaf(cf4.w, f, cf4.z), f, cf4.y), f, cf4.x);
+                                    real[k] += (double)(((float)a.charge[s] * ent.q[e]) * (ez4 * ir));
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < TPT; ++k) {
+                                const double dx = ex - ox[k][s], dy = ey - oy[k][s], dz = ez - oz[k][s];
+                                const double r2 = dx * dx + dy * dy + dz * dz;
+                                if (r2 <= rc2) {
+                                    const double ir2 = 1.0 / r2;
+                                    const double s2 = tab[0][s][ty] * ir2;
+                                    const double s6 = s2 * s2 * s2;
+                                    double v = tab[1][s][ty] * (s6 * s6 - s6);
+                                    if (a.trunc == ALEATOR_LJ_SHIFTED) {
+                                        v -= tab[2][s][ty];
+                                    } else if (a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED) {
+                                        v = v - tab[2][s][ty] - (sqrt(r2) - rc) * tab[3][s][ty];
+                                    }
+                                    lj[k] += v;
+                                }
+                                if (CHARGED && r2 <= rcr2) {
+                                    const double r = sqrt(r2);
+                                    real[k] += (a.charge[s] * ent.q[e]) * erfc_over_r<double>(alpha, r);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        };
+
+        int pending = 0;
+        for (int base = 0; base < a.natoms; base += kWChunk) {
+            const int at = base + tid;
+            int cnt = 0;
+            unsigned mask = 0;
+            double h0 = 0, h1 = 0, h2 = 0;
+            if (tid < kWChunk && at < a.natoms) {
+                h0 = a.fx[at] - cf[0];
+                h1 = a.fy[at] - cf[1];
+                h2 = a.fz[at] - cf[2];
+                h0 -= floor(h0);
+                h1 -= floor(h1);
+                h2 -= floor(h2);
+                for (int sh = 0; sh < 8; ++sh) {
+                    if (c.single_image && sh != ((h0 >= 0.5 ? 1 : 0) | (h1 >= 0.5 ? 2 : 0) | (h2 >= 0.5 ? 4 : 0)))
+                        continue;
+                    const double g0 = h0 - (sh & 1), g1 = h1 - ((sh >> 1) & 1), g2 = h2 - (sh >> 2);
+                    const double x = g0 * a.m[0] + g1 * a.m[3] + g2 * a.m[6];
+                    const double y = g0 * a.m[1] + g1 * a.m[4] + g2 * a.m[7];
+                    const double z = g0 * a.m[2] + g1 * a.m[5] + g2 * a.m[8];
+                    if (x * x + y * y + z * z <= c.rlist2) {
+                        mask |= 1u << sh;
+                        ++cnt;
+                    }
+                }
+            }
+            int total;
+            const int off = block_exclusive_scan(cnt, scan_tmp, &total);
+            if (pending + total > kWCap) {
+                compute(pending);
+                pending = 0;
+            }
+            if (mask) {
+                int w = pending + off;
+                const double qa = a.q[at];
+                const int ta = a.type[at];
+                for (int sh = 0; sh < 8; ++sh) {
+                    if (!(mask & (1u << sh))) continue;
+                    const double g0 = h0 - (sh & 1), g1 = h1 - ((sh >> 1) & 1), g2 = h2 - (sh >> 2);
+                    ent.x[w] = (R)(g0 * a.m[0] + g1 * a.m[3] + g2 * a.m[6]);
+                    ent.y[w] = (R)(g0 * a.m[1] + g1 * a.m[4] + g2 * a.m[7]);
+                    ent.z[w] = (R)(g0 * a.m[2] + g1 * a.m[5] + g2 * a.m[8]);
+                    ent.q[w] = (R)qa;
+                    ent.t[w] = ta;
+                    ++w;
+                }
+            }
+            pending += total;
+        }
+        compute(pending);
+#pragma unroll
+        for (int k = 0; k < TPT; ++k)
+            if (ok[k]) a.du[tr[k]] = CHARGED ? lj[k] + a.kc * real[k] : lj[k];
+    }
+}
+
+// Per workgroup sum of exp(-dU/T) over 256 consecutive trials (uncharged: dU is complete).
+__global__ __launch_bounds__(kWThreads) void widom_boltz_kernel(const WidomArgs a) {
+    __shared__ double red[kWThreads / 64];
+    const uint64_t t = (uint64_t)blockIdx.x * kWThreads + threadIdx.x;
+    const double b = t < a.n_trials ? exp(-a.du[t] / a.temperature) : 0.0;
+    const double s = block_sum(b, red);
+    if (threadIdx.x == 0) a.block_sums[blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(1024) void sum_kernel(const double* parts, uint64_t n, double* out) {
     __shared__ double red[16];
     double s = 0.0;
     for (uint64_t i = threadIdx.x; i < n; i += 1024) s += parts[i];
     s = block_sum(s, red);
     if (threadIdx.x == 0) *out = s;
+}
+
+// Trials per thread of the cube kernel: 1, or (for large batches) 4 for one site and 2 otherwise.
+inline int max_trials_per_thread(int ns) { return ns == 1 ? 4 : 2; }
+
+template <typename R, bool CH>
+void launch_cells(const WidomArgs& a, const CellArgs& c, unsigned n_cells, bool wide, hipStream_t s) {
+    switch (a.nsites * 2 + (wide ? 1 : 0)) {
+#define CASE(N, T)                                                                                           \
+    case 2 * N + (T > 1 ? 1 : 0):                                                                            \
+        hipLaunchKernelGGL((widom_cell_kernel<R, N, CH, T>), dim3(n_cells), dim3(kWThreads), 0, s, a, c); \
+        return;
+        CASE(1, 1) CASE(1, 4) CASE(2, 1) CASE(2, 2) CASE(3, 1) CASE(3, 2) CASE(4, 1) CASE(4, 2) CASE(5, 1) CASE(5, 2)
+#undef CASE
+    }
+    fail(ALEATOR_INVALID_ARGUMENT, "widom: 1 to 5 sites supported");
 }
 
 template <typename R, bool CH>
@@ -320,12 +614,59 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
             tab[(2 * kMaxSites + s) * kMaxTypes + t] = v;
             tab[(3 * kMaxSites + s) * kMaxTypes + t] = dv;
         }
+    // Cube grid for the cell-sorted direct sum: about n/1024 trials per occupied cube (so that
+    // the cubes keep every CU busy), at least 64 and at most one pass of a workgroup (256
+    // threads x trials per thread); list radius R = max cutoff + cube half-diagonal + molecule
+    // radius.  It needs R < min L_perp (the 8-image list then holds every image within
+    // R); the all-atom kernel runs instead when that fails, or when the sphere of radius R holds
+    // more than 70 % of the cell's volume (the list would save too little to pay for the sort).
+    CellArgs cg{};
+    unsigned n_cells = 0;
+    bool wide_cells = false;
+    {
+        double rmol = 0.0;
+        for (int s = 0; s < ns; ++s)
+            rmol = std::max(rmol, std::sqrt(a.local[s][0] * a.local[s][0] + a.local[s][1] * a.local[s][1] +
+                                            a.local[s][2] * a.local[s][2]));
+        const double rmax = std::max(fw.rc, charged ? fw.rc_real : 0.0);
+        const double lmin = std::min({fw.lat.lperp[0], fw.lat.lperp[1], fw.lat.lperp[2]});
+        const double vol = std::abs(fw.lat.volume);
+        double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int corner = 0; corner < 8; ++corner)
+            for (int j = 0; j < 3; ++j) {
+                double x = 0.0;
+                for (int i = 0; i < 3; ++i)
+                    if (corner & (1 << i)) x += fw.lat.m[i][j];
+                lo[j] = std::min(lo[j], x);
+                hi[j] = std::max(hi[j], x);
+            }
+        const double per_cell =
+            std::clamp((double)req.n_trials / 1024.0, 64.0, (double)kWThreads * max_trials_per_thread(ns));
+        wide_cells = per_cell > kWThreads;
+        const double edge = std::cbrt(vol / std::max(1.0, (double)req.n_trials / per_cell));
+        const double rl = rmax + 0.5 * std::sqrt(3.0) * edge + rmol;
+        long long count = 1;
+        for (int j = 0; j < 3; ++j) {
+            cg.nc[j] = std::max(1, (int)std::ceil((hi[j] - lo[j]) / edge));
+            cg.lo[j] = lo[j];
+            count *= cg.nc[j];
+        }
+        cg.edge = edge;
+        if (req.n_trials <= 0x7fffffffull && count <= (1 << 22) && rl < lmin &&
+            4.0 / 3.0 * kPi * rl * rl * rl <= 0.7 * vol) {
+            cg.rlist2 = rl * rl;
+            cg.single_image = 2.0 * rl < lmin ? 1 : 0;
+            n_cells = (unsigned)count;
+        }
+    }
     const unsigned pair_blocks = (unsigned)((req.n_trials + kWThreads - 1) / kWThreads);
     const unsigned rec_blocks = (unsigned)((req.n_trials + 3) / 4);
     const size_t tab_bytes = (tab.size() * sizeof(double) + 255) & ~size_t(255);
     const size_t parts = charged ? rec_blocks : pair_blocks;
     size_t du_bytes = req.du_device ? 0 : ((req.n_trials * sizeof(double) + 255) & ~size_t(255));
-    char* base = static_cast<char*>(ctx.ensure_scratch(tab_bytes + parts * sizeof(double) + du_bytes));
+    const size_t sort_bytes =
+        n_cells ? ((2 * req.n_trials + 3 * (size_t)n_cells + 1) * sizeof(uint32_t) + 255) & ~size_t(255) : 0;
+    char* base = static_cast<char*>(ctx.ensure_scratch(tab_bytes + parts * sizeof(double) + du_bytes + sort_bytes));
     ALEATOR_HIP_CHECK(hipMemcpyAsync(base, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
                                      ctx.stream));
     a.ljtab = reinterpret_cast<const double*>(base);
@@ -335,7 +676,33 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
     const bool fp64 = req.precision == ALEATOR_PRECISION_FP64;
     if (!fp64 && req.precision != ALEATOR_PRECISION_MIXED) fail(ALEATOR_INVALID_ARGUMENT, "widom: unknown precision");
     ctx.time_begin();
-    if (charged) {
+    if (n_cells) {
+        // group the trials by cell: counts, exclusive scan, scatter (order within a cell is
+        // arbitrary and does not matter: every trial is summed on its own, into du[t])
+        uint32_t* key = reinterpret_cast<uint32_t*>(base + tab_bytes + parts * sizeof(double) + du_bytes);
+        uint32_t* order = key + req.n_trials;
+        uint32_t* count = order + req.n_trials;
+        uint32_t* start = count + n_cells;
+        uint32_t* cursor = start + n_cells + 1;
+        cg.start = start;
+        cg.order = order;
+        const unsigned tb = (unsigned)((req.n_trials + 255) / 256);
+        hipLaunchKernelGGL(fill_u32_kernel, dim3((n_cells + 255) / 256), dim3(256), 0, ctx.stream, count,
+                           (uint64_t)n_cells);
+        hipLaunchKernelGGL(widom_key_kernel, dim3(tb), dim3(256), 0, ctx.stream, a, cg, key, count);
+        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, ctx.stream, count, n_cells, start, cursor);
+        hipLaunchKernelGGL(scatter_kernel, dim3(tb), dim3(256), 0, ctx.stream, key, req.n_trials, cursor, order);
+        if (charged) {
+            if (fp64) launch_cells<double, true>(a, cg, n_cells, wide_cells, ctx.stream);
+            else launch_cells<float, true>(a, cg, n_cells, wide_cells, ctx.stream);
+            const size_t lds = (size_t)4 * 3 * (2 * fw.kmax + 1) * ns * sizeof(cplx);
+            hipLaunchKernelGGL(widom_recip_kernel, dim3(rec_blocks), dim3(256), lds, ctx.stream, a);
+        } else {
+            if (fp64) launch_cells<double, false>(a, cg, n_cells, wide_cells, ctx.stream);
+            else launch_cells<float, false>(a, cg, n_cells, wide_cells, ctx.stream);
+            hipLaunchKernelGGL(widom_boltz_kernel, dim3(pair_blocks), dim3(kWThreads), 0, ctx.stream, a);
+        }
+    } else if (charged) {
         if (fp64) launch_pair<double, true>(a, pair_blocks, ctx.stream);
         else launch_pair<float, true>(a, pair_blocks, ctx.stream);
         const size_t lds = (size_t)4 * 3 * (2 * fw.kmax + 1) * ns * sizeof(cplx);

@@ -77,3 +77,47 @@ def test_widom_rejects_bad_arguments(ctx):
         dev.widom(CH4, 10, -1.0)
     with pytest.raises(IndexError):
         dev.widom([(0, 0, 0, 99, 0.0)], 10, 298.0)
+
+
+def test_widom_cell_path_matches_the_oracle(ctx):
+    """Large batches group trials by cube and sum against per-cube LDS lists; small ones scan
+    every atom.  On c0 a 1e7-trial batch takes the first path and a 200-trial batch the second:
+    both give the oracle's dU for the first 200 trials, and MIXED tracks FP64 over all 1e7."""
+    print("widom c0: 200 and 1e7 trials, FP64 and MIXED", flush=True)
+    fw, lj = ol.c0_system()
+    dev = device_framework(ctx, fw, lj)
+    _, du_ref = ol.widom(lj, fw, CH4, seed=3, stream=2, n_trials=200, temperature=298.0)
+    _, small = dev.widom(CH4, 200, 298.0, seed=3, stream=2, precision=aa.FP64)
+    _, big = dev.widom(CH4, 10_000_000, 298.0, seed=3, stream=2, precision=aa.FP64)
+    assert rel_err(small, du_ref).max() <= 1e-10
+    assert rel_err(big[:200], du_ref).max() <= 1e-10
+    _, big32 = dev.widom(CH4, 10_000_000, 298.0, seed=3, stream=2, precision=aa.MIXED)
+    sel = np.abs(big) > 1.0
+    err = np.abs(big32 - big)[sel] / np.abs(big[sel])
+    print(f"  MIXED vs FP64 over 1e7 trials: median {np.median(err):.2e}, within 1e-5 {np.mean(err <= 1e-5):.4f}")
+    assert np.median(err) <= 1e-6
+
+
+def test_widom_cell_path_charged_c1(ctx):
+    """Charged CO2 on the c1 supercell (4800 atoms, Ewald alpha 0.265, kmax 8): a 1e5-trial batch
+    runs the cube path; its first 40 trials equal the oracle's dU."""
+    print("widom c1 CO2: 1e5 trials against the oracle's first 40", flush=True)
+    fw, lj = ol.c1_system()
+    sc = ol.replicate(fw, ol.min_supercell(fw.m, 12.0))
+    st = ol.EwaldState(sc, 0.265, 8)
+    dev = device_framework(ctx, sc, lj, ewald=(0.265, 12.0, 8))
+    _, du_ref = ol.widom(lj, sc, ol.co2_sites(), seed=4, stream=0, n_trials=40, temperature=298.0, ewald=st,
+                         rc_real=12.0)
+    _, du = dev.widom(ol.co2_sites(), 100_000, 298.0, seed=4, stream=0, precision=aa.FP64)
+    assert rel_err(du[:40], du_ref).max() <= 1e-10
+
+
+def test_widom_all_atom_path_for_large_molecules(ctx):
+    """A molecule wider than the cell leaves no cell grid whose list radius fits inside the cell,
+    so the all-atom kernel runs; it matches the oracle too."""
+    fw, lj = ol.c0_system()
+    dev = device_framework(ctx, fw, lj)
+    wide = [(0.0, 0.0, -13.5, 4, 0.0), (0.0, 0.0, 13.5, 4, 0.0)]  # radius 13.5 > L - rc = 13.03
+    _, du_ref = ol.widom(lj, fw, wide, seed=5, stream=1, n_trials=100, temperature=298.0)
+    _, du = dev.widom(wide, 100, 298.0, seed=5, stream=1, precision=aa.FP64)
+    assert rel_err(du, du_ref).max() <= 1e-10
