@@ -480,11 +480,17 @@ def main():
                 t = timed(lambda: fw0.widom_device(ch4, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
                                                    precision=p), 5, 1, sync)
                 extra[f"c0_widom_ch4_{n_trials}_{name}_inserts_per_s"] = n_trials / t
+        # algorithmic work per insertion: CH4 on c0, n = 216.1 atoms within rc x 20 FLOP (LJ)
+        rate = extra["c0_widom_ch4_10000000_mixed_inserts_per_s"]
+        extra["c0_widom_ch4_1e7_mixed_tflops"] = rate * 216.1 * F_LJ / 1e12
         co2 = [(0.0, 0.0, -1.16, 5, -0.35), (0.0, 0.0, 0.0, 4, 0.70), (0.0, 0.0, 1.16, 5, -0.35)]
-        n_trials = 100_000
-        t = timed(lambda: fw.widom_device(co2, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
-                                          precision=prec), 3, 1, sync)
-        extra[f"c1_widom_co2_{n_trials}_{args.precision}_inserts_per_s"] = n_trials / t
+        # CO2 on c1: 3 sites x (444.3 x (20 + 30) FLOP direct + 1102 half-set k x 16 FLOP FP64)
+        f_ins = 3 * (nbar * (F_LJ + F_REAL) + (fw.num_k // 2) * 16)
+        for n_trials in (100_000, 1_000_000):
+            t = timed(lambda: fw.widom_device(co2, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
+                                              precision=prec), 3, 1, sync)
+            extra[f"c1_widom_co2_{n_trials}_{args.precision}_inserts_per_s"] = n_trials / t
+            extra[f"c1_widom_co2_{n_trials}_{args.precision}_tflops"] = n_trials / t * f_ins / 1e12
         del du
 
     from aleator_amd import workloads as wl
