@@ -425,25 +425,28 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                 }
             }
         }
-        // other guests
-        for (int g = tid; g < n_mol; g += NT) {
-            if (g == mi) continue;
-            for (int u = 0; u < ns; ++u) {
-                const double* gp = pos + ((size_t)g * ns + u) * 3;
+        // other guests: every (site row, guest site) pair in one flattened loop over the
+        // workgroup, so that a few dozen molecules still occupy every thread
+        {
+            const int gsites = n_mol * ns;
+            const int total = nsite * gsites;
+            for (int i = tid; i < total; i += NT) {
+                const int row = i / gsites, gs = i - row * gsites;
+                const int g = gs / ns, u = gs - g * ns;
+                if (g == mi) continue;
+                const double* gp = pos + (size_t)gs * 3;
                 const double gc[3] = {gp[0], gp[1], gp[2]};
                 double gf[3];
                 frac_of(a, gc, gf);
-                for (int row = 0; row < nsite; ++row) {
-                    const bool is_old = row < n_rem;
-                    const int s = is_old ? row : row - n_rem;
-                    const int idx = s * kMaxSites + u;
-                    const double r2 = dist2(a, gf[0] - site4[4 * row], gf[1] - site4[4 * row + 1],
-                                            gf[2] - site4[4 * row + 2]);
-                    if (r2 <= a.rc2_lj) e4[is_old ? 0 : 1] += lj_term(a, a.ljgg, kMaxSites * kMaxSites, idx, r2);
-                    if (a.charged && r2 <= a.rc2_real) {
-                        const double r = sqrt(r2);
-                        e4[is_old ? 2 : 3] += a.charge[s] * a.charge[u] * (erfc(a.alpha * r) / r);
-                    }
+                const bool is_old = row < n_rem;
+                const int s = is_old ? row : row - n_rem;
+                const int idx = s * kMaxSites + u;
+                const double r2 =
+                    dist2(a, gf[0] - site4[4 * row], gf[1] - site4[4 * row + 1], gf[2] - site4[4 * row + 2]);
+                if (r2 <= a.rc2_lj) e4[is_old ? 0 : 1] += lj_term(a, a.ljgg, kMaxSites * kMaxSites, idx, r2);
+                if (a.charged && r2 <= a.rc2_real) {
+                    const double r = sqrt(r2);
+                    e4[is_old ? 2 : 3] += a.charge[s] * a.charge[u] * (erfc(a.alpha * r) / r);
                 }
             }
         }
