@@ -10,7 +10,8 @@
  *   removeMolecule / removeParticleIndices  :814-848 (swap-with-last particle removal)
  *   acceptance ratios        gcmc_acceptance.hpp:28-101
  * Guest-guest dispersion is a full scan in particle-index order, which is what the reference's
- * guest cell list reproduces bit for bit (monte_carlo_engine.cc:316-374).  No energy grid.
+ * guest cell list reproduces bit for bit (monte_carlo_engine.cc:316-374).  With an energy grid
+ * (orc_gcmc_set_grid) the framework part is the grid's interpolation instead.
  * Nested-argument draws are taken left to right (see oracle.c, Widom).
  *
  * Pinned bitwise against the reference engine compiled in oracle/_ref
@@ -98,22 +99,78 @@ void orc_gcmc_free(orc_gcmc* g) {
     free(g->sig);
     free(g->prop_cdf);
     free(g->prop_fac);
+    free(g->grid_tables);
+    free(g->grid_species);
     if (g->charged) orc_ewald_state_free(&g->ew);
     memset(g, 0, sizeof(*g));
 }
 
 static double uni(orc_gcmc* g) { return orc_rng_next_uniform(&g->rng); }
 
-/* moleculeDispersionEnergy without a grid (monte_carlo_engine.cc:341-374) */
+/* computeParticleEnergyOverCandidates over the guests, ascending index (lennard_jones.cc:184-216) */
+static double guest_lj(orc_gcmc* g, size_t index, const size_t* excl, size_t ne) {
+    double energy = 0.0;
+    const double pi[3] = {g->x[index], g->y[index], g->z[index]};
+    for (size_t j = g->n_fw; j < g->n; ++j) {
+        if (j == index) continue;
+        int skip = 0;
+        for (size_t k = 0; k < ne && !skip; ++k) skip = excl[k] == j;
+        if (skip) continue;
+        double eps, sig, d[3];
+        orc_lj_mixed(&g->lj, g->sp[index], g->sp[j], &eps, &sig);
+        const double pj[3] = {g->x[j], g->y[j], g->z[j]};
+        orc_minimum_image(&g->lat, pi, pj, d);
+        energy += orc_lj_pair_energy(&g->lj, sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]), eps, sig);
+    }
+    return energy;
+}
+
+/* moleculeDispersionEnergy (monte_carlo_engine.cc:341-374): per site, the framework part
+ * (direct scan, or the grid's interpolation) and then the guests */
 static double dispersion(orc_gcmc* g, const size_t* sites, size_t ns, const size_t* also, size_t na) {
     size_t excl[64];
     size_t ne = 0;
     for (size_t i = 0; i < ns; ++i) excl[ne++] = sites[i];
     for (size_t i = 0; i < na; ++i) excl[ne++] = also[i];
-    orc_particles p = view(g);
     double e = 0.0;
+    if (g->grid_tables) {
+        const size_t cells = (size_t)g->grid_n[0] * g->grid_n[1] * g->grid_n[2];
+        for (size_t s = 0; s < ns; ++s) {
+            const size_t i = sites[s];
+            int32_t k = 0;
+            while (g->grid_species[k] != g->sp[i]) ++k; /* present: checked by orc_gcmc_set_grid */
+            const double pos[3] = {g->x[i], g->y[i], g->z[i]};
+            e += orc_grid_interpolate(&g->lat, g->grid_n, g->grid_tables + (size_t)k * cells, pos);
+            e += guest_lj(g, i, excl, ne);
+        }
+        return e;
+    }
+    orc_particles p = view(g);
     for (size_t s = 0; s < ns; ++s) e += orc_lj_particle_energy(&g->lj, sites[s], &p, &g->lat, excl, ne);
     return e;
+}
+
+int orc_gcmc_set_grid(orc_gcmc* g, const int32_t n[3], const uint32_t* species, int32_t n_tables,
+                      const double* tables) {
+    free(g->grid_tables);
+    free(g->grid_species);
+    g->grid_tables = NULL;
+    g->grid_species = NULL;
+    g->grid_n_tables = 0;
+    if (!tables) return 0;
+    for (size_t s = 0; s < g->ns; ++s) {
+        int found = 0;
+        for (int32_t k = 0; k < n_tables; ++k) found |= species[k] == g->sites[s].lj_species;
+        if (!found) return -1;
+    }
+    const size_t cells = (size_t)n[0] * n[1] * n[2];
+    g->grid_tables = (double*)malloc((size_t)n_tables * cells * sizeof(double));
+    g->grid_species = (uint32_t*)malloc((size_t)n_tables * sizeof(uint32_t));
+    memcpy(g->grid_tables, tables, (size_t)n_tables * cells * sizeof(double));
+    memcpy(g->grid_species, species, (size_t)n_tables * sizeof(uint32_t));
+    g->grid_n_tables = n_tables;
+    for (int a = 0; a < 3; ++a) g->grid_n[a] = n[a];
+    return 0;
 }
 
 /* moleculeElectrostaticRealSpaceEnergy (monte_carlo_engine.cc:299-314) */

@@ -202,6 +202,12 @@ typedef struct {
     int prop_nb[3], prop_reps[3];
     size_t prop_nbins;
     double prop_inv[3][3];
+    /* framework energy grid (monte_carlo_engine.cc:341-374 with frameworkEnergyGrid_): NULL =
+     * direct scan.  grid_tables[k] is the [n2][n1][n0] table of species grid_species[k]. */
+    double* grid_tables;
+    uint32_t* grid_species;
+    int32_t grid_n_tables;
+    int32_t grid_n[3];
 } orc_gcmc;
 
 int orc_gcmc_init(orc_gcmc* g, const orc_lj* lj, const orc_particles* fw, const orc_lattice* lat,
@@ -216,6 +222,11 @@ void orc_gcmc_run(orc_gcmc* g, uint64_t steps, double* n_trace);
 int orc_gcmc_set_proposal(orc_gcmc* g, const double* w, const int nb[3], const int reps[3], double mix);
 double orc_gcmc_electrostatic_energy(const orc_gcmc* g);
 void orc_gcmc_free(orc_gcmc* g);
+/* Framework LJ from interpolated tables (copied), as the reference engine given a
+ * FrameworkEnergyGrid; tables == NULL restores the direct scan.  -1 if a site species has no
+ * table (FrameworkEnergyGrid::interpolate's std::out_of_range). */
+int orc_gcmc_set_grid(orc_gcmc* g, const int32_t n[3], const uint32_t* species, int32_t n_tables,
+                      const double* tables);
 
 /* ---- Synthetic frameworks (no reference counterpart) ----------------- */
 /* Uniform fractional placement from Philox(seed, stream), rejecting candidates

@@ -459,6 +459,48 @@ int ref_gcmc_run(uint32_t ns, const double* eps, const double* sig, double rc, i
     });
 }
 
+// ref_gcmc_run with the reference's FrameworkEnergyGrid (loadOrBuildFrameworkEnergyGrid in
+// cache_dir, so ref_grid_build on the same directory returns the very tables the engine used).
+int ref_gcmc_run_grid(uint32_t ns, const double* eps, const double* sig, double rc, int trunc, int mixing,
+                 const double* m, uint64_t n, const double* x, const double* y, const double* z,
+                 const double* q, const uint32_t* species, int n_sites, const double* site_xyz,
+                 const uint32_t* site_species, const double* site_q, double fugacity_pa,
+                 double temperature, double max_translation, double max_rotation, int charged,
+                 double alpha, double rc_real, int kmax, uint64_t seed, uint64_t stream,
+                 uint64_t steps, uint64_t cap, double* ox, double* oy, double* oz, double* oq,
+                 uint64_t* n_particles, uint64_t* n_molecules, double* elec_energy, double spacing,
+                      const char* cache_dir, const uint32_t* guests, int n_guest) {
+    return guarded([&] {
+        const auto lj = makeLj(ns, eps, sig, rc, trunc, mixing);
+        auto rng = std::make_unique<core::Philox4x32Rng>();
+        rng->seed(seed, stream);
+        engines::MoleculeSpecies sp;
+        for (int s = 0; s < n_sites; ++s)
+            sp.sites.push_back({site_xyz[3 * s], site_xyz[3 * s + 1], site_xyz[3 * s + 2],
+                                site_species[s], site_q[s], 1.0});
+        std::shared_ptr<const forcefield::Ewald> ew;
+        if (charged) ew = std::make_shared<forcefield::Ewald>(alpha, rc_real, kmax);
+        std::vector<uint32_t> gids(guests, guests + n_guest);
+        engines::FrameworkEnergyGridCacheRequest req{cache_dir, gids, spacing};
+        const auto grid = std::make_shared<const engines::FrameworkEnergyGrid>(engines::loadOrBuildFrameworkEnergyGrid(
+            *lj, makeParticles(n, x, y, z, q, species), makeLattice(m), req, nullptr));
+        engines::MonteCarloEngine eng(makeParticles(n, x, y, z, q, species), makeLattice(m), lj,
+                                      std::move(rng), temperature, sp, fugacity_pa,
+                                      max_translation, max_rotation, ew, grid);
+        eng.run(steps);
+        const auto& p = eng.particles();
+        *n_particles = p.size();
+        *n_molecules = eng.moleculeCount();
+        *elec_energy = eng.electrostaticEnergyForTesting();
+        for (std::size_t i = 0; i < p.size() && i < cap; ++i) {
+            ox[i] = p.x[i];
+            oy[i] = p.y[i];
+            oz[i] = p.z[i];
+            oq[i] = p.charge[i];
+        }
+    });
+}
+
 int ref_pr_fugacity(double tc, double pc, double omega, double t, double p, double* out) {
     return guarded([&] {
         engines::PengRobinson pr({tc, pc, omega});

@@ -410,7 +410,8 @@ class OrcGcmc(C.Structure):
                 ("moves_since_resync", C.c_size_t), ("rng", OrcRng),
                 ("attempted", C.c_uint64 * 4), ("accepted", C.c_uint64 * 4),
                 ("prop_cdf", dp), ("prop_fac", dp), ("prop_nb", C.c_int * 3), ("prop_reps", C.c_int * 3),
-                ("prop_nbins", C.c_size_t), ("prop_inv", C.c_double * 9)]
+                ("prop_nbins", C.c_size_t), ("prop_inv", C.c_double * 9),
+                ("grid_tables", dp), ("grid_species", u32p), ("grid_n_tables", C.c_int32), ("grid_n", C.c_int32 * 3)]
 
 
 class Gcmc:
@@ -449,6 +450,18 @@ class Gcmc:
                                             C.c_double(mix))
         if rc != 0:
             raise ValueError("invalid proposal")
+
+    def set_grid(self, n_points, species, tables):
+        """Framework LJ from these interpolated tables ([n_tables][n2][n1][n0], species ids), as
+        the reference engine given a FrameworkEnergyGrid; tables None = direct scan."""
+        if tables is None:
+            assert oracle().orc_gcmc_set_grid(C.byref(self.g), None, None, 0, None) == 0
+            return
+        n = np.ascontiguousarray(n_points, dtype=np.int32)
+        sp = np.ascontiguousarray(species, dtype=np.uint32)
+        t = np.ascontiguousarray(tables, dtype=np.float64)
+        if oracle().orc_gcmc_set_grid(C.byref(self.g), ptr(n, i32p), ptr(sp, u32p), C.c_int32(len(sp)), ptr(t)) != 0:
+            raise IndexError("no grid table for a site species")
 
     def run(self, steps, trace=False):
         tr = np.zeros(steps) if trace else None
@@ -615,3 +628,30 @@ def ref_element_from_label(label: str):
     found = C.c_int()
     assert ref().ref_element_from_label(label.encode(), buf, C.byref(found)) == 0
     return buf.value.decode() if found.value else None
+
+
+def ref_gcmc_run_grid(lj: LJ, fw: Framework, sites, fugacity_pa, temperature, seed, stream, steps, spacing, guests,
+                      cache_dir, ewald=None, max_translation=1.5, max_rotation=0.5):
+    """The reference engine with its FrameworkEnergyGrid (built through loadOrBuildFrameworkEnergyGrid
+    in cache_dir): (molecule count, (x, y, z, q) of all particles, committed Ewald energy)."""
+    sx = np.array([s[:3] for s in sites], dtype=np.float64).ravel()
+    ss = np.array([s[3] for s in sites], dtype=np.uint32)
+    sq = np.array([s[4] for s in sites], dtype=np.float64)
+    g = np.ascontiguousarray(sorted(set(guests)), dtype=np.uint32)
+    cap = fw.n + 3 * len(sites) * 2000
+    out = [np.zeros(cap) for _ in range(4)]
+    npart, nmol, e = C.c_uint64(), C.c_uint64(), C.c_double()
+    alpha, rc_real, kmax = ewald if ewald is not None else (1.0, 1.0, 1)
+    rc = ref().ref_gcmc_run_grid(C.c_uint32(len(lj.eps)), ptr(lj.eps), ptr(lj.sigma), C.c_double(lj.cutoff),
+                                 lj.truncation, lj.mixing, ptr(np.ascontiguousarray(fw.m)), C.c_uint64(fw.n),
+                                 ptr(fw.x), ptr(fw.y), ptr(fw.z), ptr(fw.q), ptr(fw.species, u32p), len(sites), ptr(sx),
+                                 ptr(ss, u32p), ptr(sq), C.c_double(fugacity_pa), C.c_double(temperature),
+                                 C.c_double(max_translation), C.c_double(max_rotation), 1 if ewald is not None else 0,
+                                 C.c_double(alpha), C.c_double(rc_real), kmax, C.c_uint64(seed), C.c_uint64(stream),
+                                 C.c_uint64(steps), C.c_uint64(cap), ptr(out[0]), ptr(out[1]), ptr(out[2]), ptr(out[3]),
+                                 C.byref(npart), C.byref(nmol), C.byref(e), C.c_double(spacing),
+                                 os.fsencode(str(cache_dir)), ptr(g, u32p), C.c_int(len(g)))
+    if rc != 0:
+        raise RuntimeError("reference engine failed")
+    n = npart.value
+    return nmol.value, tuple(a[:n] for a in out), e.value

@@ -246,6 +246,49 @@ def test_gcmc_charged_co2_trajectory_bitwise():
     assert g.electrostatic_energy() == e_ref
 
 
+@pytest.mark.parametrize("fugacity", [1e7, 1e8])
+def test_gcmc_methane_with_energy_grid_trajectory_bitwise(fugacity, tmp_path):
+    """The engine given a FrameworkEnergyGrid (framework LJ interpolated, guests scanned): the
+    oracle with the reference's own tables follows the reference's trajectory bit for bit."""
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff, lj.truncation = 6.5, ol.SHIFTED
+    sites = [(0.0, 0.0, 0.0, 4, 0.0)]
+    n_ref, parts, _ = ol.ref_gcmc_run_grid(lj, fw, sites, fugacity, 300.0, 5, 0, 3000, 0.5, [4], tmp_path)
+    counts, tables = ref_grid(lj, fw, [4], 0.5)
+    g = ol.Gcmc(lj, fw, sites, fugacity, 300.0, 5, 0)
+    g.set_grid(counts, [4], tables)
+    g.run(3000)
+    assert g.n_molecules == n_ref and n_ref > 0
+    assert all(np.array_equal(a, b) for a, b in zip(g.particles(), parts))
+    direct = ol.Gcmc(lj, fw, sites, fugacity, 300.0, 5, 0)
+    direct.run(3000)
+    assert not all(np.array_equal(a, b) for a, b in zip(direct.particles(), parts))  # the grid changed the run
+
+
+def test_gcmc_charged_co2_with_energy_grid_trajectory_bitwise(tmp_path):
+    m = ol.cell_from_params(16.0, 17.0, 18.0, 80.0, 95.0, 100.0)
+    fw = ol.synth_framework(m, 120, seed=4, charge_sigma=0.5)
+    eps = np.array([t[0] for t in ol.C0_TYPES] + [ol.CO2_C[0], ol.CO2_O[0]])
+    sig = np.array([t[1] for t in ol.C0_TYPES] + [ol.CO2_C[1], ol.CO2_O[1]])
+    lj = ol.LJ(eps, sig, 7.5)
+    n_ref, parts, e_ref = ol.ref_gcmc_run_grid(lj, fw, ol.co2_sites(), 1e6, 298.0, 3, 1, 1000, 0.6, [4, 5], tmp_path,
+                                               ewald=(0.35, 7.5, 5))
+    counts, tables = ref_grid(lj, fw, [4, 5], 0.6)
+    g = ol.Gcmc(lj, fw, ol.co2_sites(), 1e6, 298.0, 3, 1, ewald=(0.35, 7.5, 5))
+    g.set_grid(counts, [4, 5], tables)
+    g.run(1000)
+    assert g.n_molecules == n_ref and n_ref > 0
+    assert all(np.array_equal(a, b) for a, b in zip(g.particles(), parts))
+    assert g.electrostatic_energy() == e_ref
+
+
+def test_gcmc_grid_missing_species_is_out_of_range():
+    fw, lj = ol.c0_system(40, a=14.0)
+    g = ol.Gcmc(lj, fw, [(0.0, 0.0, 0.0, 4, 0.0)], 1e7, 300.0, 5, 0)
+    with pytest.raises(IndexError):
+        g.set_grid((4, 4, 4), [3], np.zeros((1, 4, 4, 4)))
+
+
 def test_peng_robinson_bitwise():
     out = C.c_double()
     for bar in (0.1, 1.1525, 30.0):
