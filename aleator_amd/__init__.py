@@ -709,6 +709,25 @@ class Gcmc:
         _check(_load().aleator_gpu_gcmc_set_proposal(self._h, None if w is None else _p(w), nb_a, reps_a,
                                                      C.c_double(uniform_mix)))
 
+    def set_energy_grid(self, n_points, species, tables):
+        """Framework LJ from energy-grid tables (aleator_gpu_gcmc_set_energy_grid): tables is a
+        host array [len(species)][n2][n1][n0], or a device pointer (DeviceBuffer or int) to one;
+        None restores the direct framework scan.  IndexError when a site's species has no table
+        (the reference's std::out_of_range)."""
+        if tables is None:
+            _check(_load().aleator_gpu_gcmc_set_energy_grid(self._h, None, None, C.c_int32(0), None))
+            return
+        n = (C.c_int32 * 3)(*[int(v) for v in n_points])
+        sp = np.ascontiguousarray(species, dtype=np.uint32)
+        if isinstance(tables, np.ndarray):
+            t = np.ascontiguousarray(tables, dtype=np.float64)
+            if t.size != sp.size * int(np.prod([int(v) for v in n_points])):
+                raise ValueError("tables must hold len(species) * n0 * n1 * n2 values")
+            _check(_load().aleator_gpu_gcmc_set_energy_grid_host(self._h, n, _p(sp, _u32p), C.c_int32(sp.size), _p(t)))
+        else:
+            _check(_load().aleator_gpu_gcmc_set_energy_grid(self._h, n, _p(sp, _u32p), C.c_int32(sp.size),
+                                                            C.cast(C.c_void_p(int(tables)), _dp)))
+
     def positions(self, chain: int) -> np.ndarray:
         n = C.c_int32()
         _check(_load().aleator_gpu_gcmc_positions(self._h, C.c_int32(chain), None, C.c_uint64(0), C.byref(n)))

@@ -7,12 +7,13 @@
 // artifact and the printed results follow the reference (io/config.cc:116-350, cli/main.cc), so
 // a config written for the reference CLI runs here unchanged and, with the same seed, prints the
 // same <N>: the device chain takes the reference engine's draws in the reference's order.
-// Differences, each reported as an error rather than ignored:
-//   * gcmc.energy_grid_spacing_angstrom: the reference then samples interpolated grid energies;
-//     the device chain always uses exact direct sums, so such a config is rejected;
-//   * [pore] configs and `bench`: not part of this build.
+// With gcmc.energy_grid_spacing_angstrom the framework LJ energy comes from the adsorbate's grid
+// table, loaded from gcmc.energy_grid_cache_directory (the reference's cache file format, so a
+// grid the reference cached loads here and the other way round) or built on the device and
+// cached.  Not part of this build, and reported as such: [pore] configs and `bench`.
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -365,7 +366,7 @@ std::string json_escape(std::string_view s) {
 }
 
 // resolvedConfigJson (cli/main.cc:344-395), plus "backend":"hip"
-std::string resolved_json(const Config& cfg, const std::array<int, 3>& sc) {
+std::string resolved_json(const Config& cfg, const std::array<int, 3>& sc, const std::optional<bool>& grid_hit) {
     const auto& r = cfg.run;
     const auto& g = cfg.gcmc;
     std::ostringstream o;
@@ -378,7 +379,8 @@ std::string resolved_json(const Config& cfg, const std::array<int, 3>& sc) {
       << ",\"equilibration_steps\":" << g.equilibration_steps << ",\"production_steps\":" << g.production_steps
       << ",\"supercell\":[" << sc[0] << "," << sc[1] << "," << sc[2] << "],\"energy_grid_spacing_angstrom\":"
       << (g.grid_spacing ? std::to_string(*g.grid_spacing) : "null") << ",\"energy_grid_cache_directory\":\""
-      << json_escape(g.grid_cache_directory.string()) << "\",\"energy_grid_cache_status\":null"
+      << json_escape(g.grid_cache_directory.string()) << "\",\"energy_grid_cache_status\":"
+      << (grid_hit ? (*grid_hit ? "\"hit\"" : "\"will_build\"") : "null")
       << ",\"adsorbate\":{\"name\":\"" << json_escape(g.adsorbate.name) << "\",\"epsilon_kelvin\":"
       << g.adsorbate.epsilon << ",\"sigma_angstrom\":" << g.adsorbate.sigma << ",\"mass_amu\":" << g.adsorbate.mass
       << ",\"charge_e\":" << g.adsorbate.charge << ",\"uses_peng_robinson\":"
@@ -391,9 +393,9 @@ std::string resolved_json(const Config& cfg, const std::array<int, 3>& sc) {
     return o.str();
 }
 
-void print_config(const Config& cfg, const Prepared& p, bool json) {
+void print_config(const Config& cfg, const Prepared& p, const std::optional<bool>& grid_hit, bool json) {
     if (json) {
-        std::cout << resolved_json(cfg, p.supercell) << "\n";
+        std::cout << resolved_json(cfg, p.supercell, grid_hit) << "\n";
         return;
     }
     const auto& r = cfg.run;
@@ -411,8 +413,15 @@ void print_config(const Config& cfg, const Prepared& p, bool json) {
               << p.supercell[2] << (g.supercell ? " (from gcmc.supercell)" : " (auto-computed minimum)") << "\n"
               << "gcmc.equilibration_steps       = " << g.equilibration_steps << "\n"
               << "gcmc.production_steps          = " << g.production_steps << "\n"
-              << "gcmc.energy_grid_spacing_angstrom = not set (direct guest-host sums on the device)\n"
-              << "gcmc.adsorbate.name            = " << g.adsorbate.name << "\n"
+              << "gcmc.energy_grid_spacing_angstrom = "
+              << (g.grid_spacing ? std::to_string(*g.grid_spacing) : "not set (direct guest-host sums on the device)")
+              << "\n";
+    if (g.grid_spacing)
+        std::cout << "gcmc.energy_grid_cache_directory = " << g.grid_cache_directory.string() << "\n"
+                  << "gcmc.energy_grid_cache_status  = "
+                  << (grid_hit ? (*grid_hit ? "hit (will load, not rebuild)" : "miss (will build and cache)") : "unknown")
+                  << "\n";
+    std::cout << "gcmc.adsorbate.name            = " << g.adsorbate.name << "\n"
               << "gcmc.adsorbate.epsilon_kelvin  = " << g.adsorbate.epsilon << "\n"
               << "gcmc.adsorbate.sigma_angstrom  = " << g.adsorbate.sigma << "\n"
               << "gcmc.adsorbate.mass_amu        = " << g.adsorbate.mass << "\n"
@@ -426,13 +435,40 @@ void print_config(const Config& cfg, const Prepared& p, bool json) {
     std::cout << "backend                        = hip (one MI355X chain)\n";
 }
 
-void reject_grid_spacing(const Config& cfg) {
-    if (cfg.gcmc.grid_spacing)
-        throw std::runtime_error(
-            "gcmc.energy_grid_spacing_angstrom is set, but this backend always samples exact direct guest-host "
-            "sums on the device (a grid-interpolated run would follow a different trajectory); remove the key");
-}
+// The energy-grid cache request of a prepared run (frameworkEnergyGridCacheRequest,
+// cli/main.cc:278-286): the adsorbate's table at the configured spacing, keyed on the replicated
+// framework, its lattice and the force field (makeFrameworkEnergyGridCacheKey).
+struct GridKey {
+    aleator_lattice lat{};
+    aleator_particles atoms{};
+    aleator_lj_params lj{};
+    std::uint32_t guest = 0;
+    aleator_grid_cache_key key{};
+    std::string directory;
+    GridKey(const Config& cfg, const Prepared& p) : guest(p.adsorbate_species) {
+        const auto& s = p.structure;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) lat.m[i][j] = s.lattice.m[i][j];
+        atoms = {s.particles.x.data(), s.particles.y.data(), s.particles.z.data(), nullptr, s.particles.species.data(),
+                 s.particles.size()};
+        lj = {p.eps.data(), p.sig.data(), (std::uint32_t)p.eps.size(), cfg.gcmc.cutoff, ALEATOR_LJ_SHIFTED,
+              ALEATOR_MIX_LORENTZ_BERTHELOT};
+        key = {&lat, &atoms, &lj, &guest, 1, *cfg.gcmc.grid_spacing};
+        directory = cfg.gcmc.grid_cache_directory.string();
+    }
+    GridKey(const GridKey&) = delete;
+    GridKey& operator=(const GridKey&) = delete;
+};
 
+// frameworkEnergyGridCacheHitForDryRun (cli/main.cc:331-341): whether the cache holds this
+// run's grid, from the file header only; a mismatching or corrupt file is an error, as there.
+std::optional<bool> grid_cache_status(const Config& cfg, const Prepared& p) {
+    if (!cfg.gcmc.grid_spacing) return std::nullopt;
+    const GridKey k(cfg, p);
+    std::int32_t n[3], nt = 0, hit = 0;
+    ag::check(aleator_grid_cache_load(k.directory.c_str(), &k.key, n, nullptr, &nt, nullptr, 0, &hit));
+    return hit != 0;
+}
 // A device framework of the prepared system: LJ only, shifted at the cutoff, Lorentz-Berthelot
 // (the reference CLI runs without electrostatics, cli/main.cc:524-528).
 struct DeviceRun {
@@ -464,9 +500,8 @@ double cell_volume(const ag::Lattice& l) {
 int gcmc_run(const fs::path& path, bool dry_run, bool json, int max_molecules) {
     const Config cfg = load_gcmc_config(path);
     const Prepared p = prepare(cfg);
-    reject_grid_spacing(cfg);
     if (dry_run) {
-        print_config(cfg, p, json);
+        print_config(cfg, p, grid_cache_status(cfg, p), json);
         return kExitSuccess;
     }
     std::cerr << "loaded " << p.structure.particles.size() << " framework atoms (" << p.structure.speciesSymbols.size()
@@ -482,17 +517,54 @@ int gcmc_run(const fs::path& path, bool dry_run, bool json, int max_molecules) {
     const double volume = cell_volume(p.structure.lattice);
     const double mass_kg = p.mass_amu * kAmuToKg;
 
+    DeviceRun d(cfg, p);
+    // buildFrameworkEnergyGridIfConfigured (cli/main.cc:300-322): load the cache entry, or build
+    // the table on the device (FP64, capped at 1e5 K) and write it
+    std::optional<bool> grid_hit;
+    std::vector<std::uint32_t> grid_species;
+    std::vector<double> grid_tables;
+    std::int32_t grid_n[3] = {1, 1, 1};
+    if (cfg.gcmc.grid_spacing) {
+        const GridKey k(cfg, p);
+        std::cerr << "guest-host energy grid requested at " << *cfg.gcmc.grid_spacing << " Ang spacing (cache: "
+                  << k.directory << ")...\n";
+        const auto t0 = std::chrono::steady_clock::now();
+        std::int32_t nt = 0, hit = 0;
+        ag::check(aleator_grid_cache_load(k.directory.c_str(), &k.key, grid_n, nullptr, &nt, nullptr, 0, &hit));
+        if (hit) {
+            grid_species.resize(std::max(nt, 1));
+            grid_tables.resize((size_t)grid_n[0] * grid_n[1] * grid_n[2] * (size_t)nt);
+            ag::check(aleator_grid_cache_load(k.directory.c_str(), &k.key, grid_n, grid_species.data(), &nt,
+                                              grid_tables.data(), grid_tables.size(), &hit));
+            grid_species.resize(nt);
+        } else {
+            ag::check(aleator_gpu_grid_point_counts(&k.lat, *cfg.gcmc.grid_spacing, grid_n));
+            grid_species = {p.adsorbate_species};
+            grid_tables.resize((size_t)grid_n[0] * grid_n[1] * grid_n[2]);
+            ag::check(aleator_gpu_grid_build(d.fw, nullptr, grid_species.data(), 1, 0, grid_n, ALEATOR_PRECISION_FP64,
+                                             1.0e5, grid_tables.data()));
+            std::string path(4096, '\0');
+            ag::check(aleator_grid_cache_file_path(k.directory.c_str(), &k.key, path.data(), path.size()));
+            ag::check(aleator_grid_cache_serialize(path.c_str(), &k.key, grid_n, grid_species.data(), 1,
+                                                   grid_tables.data()));
+        }
+        grid_hit = hit != 0;
+        std::cerr << "energy grid " << (hit ? "loaded from cache" : "built and cached") << " in " << std::fixed
+                  << std::setprecision(3)
+                  << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " s\n"
+                  << std::defaultfloat << std::setprecision(6);
+    }
+
     fs::create_directories(cfg.run.output_directory);
     {
         const fs::path out = cfg.run.output_directory / "resolved_config.json";
         std::ofstream f(out);
         if (!f) throw std::runtime_error("could not open " + out.string() + " for writing");
-        f << resolved_json(cfg, p.supercell);
+        f << resolved_json(cfg, p.supercell, grid_hit);
         std::cerr << "wrote resolved configuration (incl. run.rng_seed=" << cfg.run.rng_seed << ") to " << out.string()
                   << "\n";
     }
 
-    DeviceRun d(cfg, p);
     const aleator_molecule_site site{0.0, 0.0, 0.0, p.adsorbate_species, cfg.gcmc.adsorbate.charge};
     const aleator_gcmc_chain chain{p.fugacity, cfg.gcmc.temperature, cfg.run.rng_seed, 0};
     aleator_gcmc* g = nullptr;
@@ -501,6 +573,9 @@ int gcmc_run(const fs::path& path, bool dry_run, bool json, int max_molecules) {
         aleator_gcmc* g;
         ~Free() { aleator_gpu_gcmc_destroy(g); }
     } free_g{g};
+    if (!grid_tables.empty())
+        ag::check(aleator_gpu_gcmc_set_energy_grid_host(g, grid_n, grid_species.data(), (std::int32_t)grid_species.size(),
+                                                        grid_tables.data()));
     aleator_gcmc_stats st{};
     auto check_chain = [&] {
         ag::check(aleator_gpu_gcmc_stats(g, &st));

@@ -281,7 +281,9 @@ __device__ void trial_positions(const GcmcArgs& a, Draws& uni, int move, int pro
 // the move's trial sites and publishes them through LDS (fractional coordinates and charges of
 // the removed rows, then the added rows); all threads then split the energy scans and the k-sum;
 // thread 0 applies the acceptance rule and publishes the decision.
-template <int NT>
+// CHARGED: the framework carries Ewald charges (CHARGED); the uncharged instances drop the S(k)
+// arrays and code.
+template <int NT, bool CHARGED>
 __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     constexpr int NW = NT / 64;
     __shared__ double s_re[kMaxKHalf], s_im[kMaxKHalf];
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     const int ns = a.ns;
     const int nkk = 2 * a.kmax + 1;
 
-    if (a.charged)
+    if (CHARGED)
         for (int t = tid; t < a.nk_half; t += NT) {
             s_re[t] = a.sre[(size_t)chain * a.nk_half + t];
             s_im[t] = a.sim[(size_t)chain * a.nk_half + t];
@@ -386,7 +388,9 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         // rows' loads are in flight together (all atoms per row without lists)
         if (tid < nsite) {
             int beg = 0, len = a.natoms;
-            if (a.bin_start) {
+            if (a.grid && !CHARGED) {
+                len = 0;  // the grid holds the whole framework part
+            } else if (a.bin_start) {
                 const double f[3] = {site4[4 * tid], site4[4 * tid + 1], site4[4 * tid + 2]};
                 const int b = framework_bin(a, f);
                 beg = a.bin_start[b];
@@ -417,9 +421,9 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                 const bool is_old = row < n_rem;
                 const int s = is_old ? row : row - n_rem;
                 const double r2 = dist2(a, at.x - site4[4 * row], at.y - site4[4 * row + 1], at.z - site4[4 * row + 2]);
-                if (r2 <= a.rc2_lj)
+                if (!a.grid && r2 <= a.rc2_lj)
                     e4[is_old ? 0 : 1] += lj_term(a, s_lj, kMaxSites * kMaxTypes, s * kMaxTypes + ty, r2);
-                if (a.charged && r2 <= a.rc2_real) {
+                if (CHARGED && r2 <= a.rc2_real) {
                     const double r = sqrt(r2);
                     e4[is_old ? 2 : 3] += a.charge[s] * at.w * (erfc(a.alpha * r) / r);
                 }
@@ -444,16 +448,24 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                 const double r2 =
                     dist2(a, gf[0] - site4[4 * row], gf[1] - site4[4 * row + 1], gf[2] - site4[4 * row + 2]);
                 if (r2 <= a.rc2_lj) e4[is_old ? 0 : 1] += lj_term(a, a.ljgg, kMaxSites * kMaxSites, idx, r2);
-                if (a.charged && r2 <= a.rc2_real) {
+                if (CHARGED && r2 <= a.rc2_real) {
                     const double r = sqrt(r2);
                     e4[is_old ? 2 : 3] += a.charge[s] * a.charge[u] * (erfc(a.alpha * r) / r);
                 }
             }
         }
+        // framework LJ from the energy grid: one site row per thread
+        if (a.grid && tid < nsite) {
+            const bool is_old = tid < n_rem;
+            const int s = is_old ? tid : tid - n_rem;
+            const double* c = is_old ? oldc[s] : newc[s];
+            e4[is_old ? 0 : 1] += grid_interpolate(a.grid_geo, a.grid + (size_t)a.grid_table[s] * a.grid_cells, c[0],
+                                                   c[1], c[2]);
+        }
         block_sum4<NW>(e4, red);
         // ---------------- reciprocal space: removed = old sites, added = new sites
         double recip = 0.0;
-        if (a.charged) {
+        if (CHARGED) {
             for (int i = tid; i < 3 * nkk * nsite; i += NT) {
                 const int s = i % nsite, mm = (i / nsite) % nkk, axis = i / (nsite * nkk);
                 double sv, cv;
@@ -481,7 +493,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         // ---------------- acceptance (thread 0), gcmc_acceptance.hpp:28-101
         if (tid == 0) {
             double self_d = 0.0, excl_d = 0.0;
-            if (a.charged) {
+            if (CHARGED) {
                 double q2 = 0.0;
                 for (int s = 0; s < ns; ++s) q2 += a.charge[s] * a.charge[s];
                 double ex_new = 0.0, ex_old = 0.0;
@@ -533,12 +545,12 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                 error = 1;
                 break;
             }
-            if (tid == 0 && a.charged) {
+            if (tid == 0 && CHARGED) {
                 self_e += pend_self;
                 excl_e += pend_excl;
             }
             // commit S(k) (the same delta, recomputed, as commit* does, :161-172)
-            if (a.charged) {
+            if (CHARGED) {
                 for (int t = tid; t < a.nk_half; t += NT) {
                     const int4 n = a.kn[t];
                     double dre = 0.0, dim = 0.0;
@@ -574,7 +586,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             __threadfence_block();
             __syncthreads();
             // maybeResyncEwald (:380-403)
-            if (a.charged && ++since_resync >= 100) {
+            if (CHARGED && ++since_resync >= 100) {
                 since_resync = 0;
                 for (int t = tid; t < a.nk_half; t += NT) {
                     s_re[t] = a.fw_sre[t];
@@ -644,7 +656,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         }
     }
     __syncthreads();
-    if (a.charged)
+    if (CHARGED)
         for (int t = tid; t < a.nk_half; t += NT) {
             a.sre[(size_t)chain * a.nk_half + t] = s_re[t];
             a.sim[(size_t)chain * a.nk_half + t] = s_im[t];
@@ -695,7 +707,12 @@ void build_framework_bins(const GcmcArgs& a, double reach, int32_t** start, doub
 
 void launch_gcmc(const GcmcArgs& a, int n_chains, hipStream_t s) {
     if (a.nk_half > kMaxKHalf) fail(ALEATOR_NOT_IMPLEMENTED, "gcmc: more than 4096 reciprocal terms");
-    hipLaunchKernelGGL(gcmc_chain_kernel<kGThreads>, dim3(n_chains), dim3(kGThreads), 0, s, a);
+    if (a.charged)
+        hipLaunchKernelGGL((gcmc_chain_kernel<kGThreads, true>), dim3(n_chains), dim3(kGThreads), 0, s, a);
+    else if (a.grid)  // per-step work of a few guests: one wave per chain, barriers near free
+        hipLaunchKernelGGL((gcmc_chain_kernel<64, false>), dim3(n_chains), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((gcmc_chain_kernel<kGThreads, false>), dim3(n_chains), dim3(kGThreads), 0, s, a);
     ALEATOR_HIP_CHECK(hipGetLastError());
 }
 

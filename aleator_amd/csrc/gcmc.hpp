@@ -4,6 +4,7 @@
 #include <cstdint>
 
 #include "internal.hpp"
+#include "interp.hpp"
 
 namespace aleator_gpu {
 
@@ -71,6 +72,14 @@ struct GcmcArgs {
     const double4* bin_pos;    // list entries: fractional x, y, z and charge of the atom
     const int32_t* bin_type;   // list entries: compact LJ type of the atom
     int fb[3];
+    // energy grid (null: framework LJ summed over the atoms).  With a grid, a site's framework
+    // LJ energy is table grid_table[s] interpolated at its Cartesian position, the framework scan
+    // keeps only real-space Coulomb, and guest-guest LJ is unchanged (moleculeDispersionEnergy
+    // with a FrameworkEnergyGrid, monte_carlo_engine.cc:341-374)
+    const double* grid;  // [tables][n2][n1][n0]
+    size_t grid_cells;
+    GridGeometry grid_geo;
+    int grid_table[kMaxSites];
 };
 
 // Builds the framework bin lists of `a` (a.fb set) into device buffers owned by the caller:

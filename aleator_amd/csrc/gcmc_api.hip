@@ -19,6 +19,7 @@
 struct aleator_gcmc {
     aleator_gpu::Framework* fw = nullptr;
     int n_chains = 0;
+    uint32_t site_species[aleator_gpu::kMaxSites] = {};  // LJ species of each site
     aleator_gpu::GcmcArgs args{};
     aleator_gpu::ChainParams* d_params = nullptr;
     aleator_gpu::ChainState* d_state = nullptr;
@@ -31,6 +32,7 @@ struct aleator_gcmc {
     int32_t* d_bin_start = nullptr;
     double4* d_bin_pos = nullptr;
     int32_t* d_bin_type = nullptr;
+    double* d_grid = nullptr;  // energy-grid tables owned by the chains (a copy of the caller's)
 };
 
 namespace aleator_gpu {
@@ -71,6 +73,58 @@ void proposal_tables(const double* w, size_t n, double mix, std::vector<double>&
         prev = cdf[b];
     }
 }
+
+// Installs energy-grid tables (device or host memory, by `kind`) as the chains' own copy.
+void set_energy_grid(aleator_gcmc* g, const int32_t n_points[3], const uint32_t* species, int32_t n_tables,
+                     const double* tables, hipMemcpyKind kind) {
+    require(g != nullptr, ALEATOR_INVALID_ARGUMENT, "null argument");
+    hipStream_t s = g->fw->ctx->stream;
+    if (!tables) {
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+        if (g->d_grid) ALEATOR_HIP_CHECK(hipFree(g->d_grid));
+        g->d_grid = nullptr;
+        g->args.grid = nullptr;
+        return;
+    }
+    require(n_points && species, ALEATOR_INVALID_ARGUMENT, "null argument");
+    require(n_tables >= 1 && n_tables <= 256, ALEATOR_INVALID_ARGUMENT, "gcmc energy grid: 1 to 256 tables");
+    size_t cells = 1;
+    for (int i = 0; i < 3; ++i) {
+        require(n_points[i] >= 1, ALEATOR_INVALID_ARGUMENT, "gcmc energy grid: point counts must be >= 1");
+        cells *= (size_t)n_points[i];
+    }
+    require(cells <= (size_t)1 << 31, ALEATOR_INVALID_ARGUMENT, "gcmc energy grid: at most 2^31 nodes per table");
+    const Framework& fw = *g->fw;
+    GcmcArgs& a = g->args;
+    // FrameworkEnergyGrid::interpolate throws std::out_of_range for a species without a table
+    // (framework_energy_grid.cc:254-257); the check is made once here for every site
+    int table[kMaxSites];
+    for (int k = 0; k < a.ns; ++k) {
+        table[k] = -1;
+        for (int t = 0; t < n_tables && table[k] < 0; ++t)
+            if (species[t] == g->site_species[k]) table[k] = t;
+        require(table[k] >= 0, ALEATOR_OUT_OF_RANGE, "FrameworkEnergyGrid: no grid table for a site's species");
+    }
+    double* d = nullptr;
+    ALEATOR_HIP_CHECK(hipMalloc(&d, (size_t)n_tables * cells * sizeof(double)));
+    try {
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(d, tables, (size_t)n_tables * cells * sizeof(double), kind, s));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipFree(d);
+        throw;
+    }
+    if (g->d_grid) ALEATOR_HIP_CHECK(hipFree(g->d_grid));
+    g->d_grid = d;
+    a.grid = d;
+    a.grid_cells = cells;
+    GridGeometry geo{};
+    grid_cofactors(fw.lat.m, geo);
+    geo.volume = fw.lat.volume;
+    for (int i = 0; i < 3; ++i) geo.n[i] = n_points[i];
+    a.grid_geo = geo;
+    for (int k = 0; k < a.ns; ++k) a.grid_table[k] = table[k];
+}
 }  // namespace
 }  // namespace aleator_gpu
 
@@ -107,6 +161,7 @@ aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_m
         auto g = std::make_unique<aleator_gcmc>();
         g->fw = fw;
         g->n_chains = n_chains;
+        for (int k = 0; k < n_sites; ++k) g->site_species[k] = sites[k].lj_species;
         GcmcArgs& a = g->args;
         hipStream_t s = fw->ctx->stream;
         a.fx = fw->d_fx;
@@ -298,6 +353,16 @@ aleator_status aleator_gpu_gcmc_set_proposal(aleator_gcmc* g, const double* w, c
             m.prop_reps[i] = reps[i];
         }
     });
+}
+
+aleator_status aleator_gpu_gcmc_set_energy_grid(aleator_gcmc* g, const int32_t n_points[3], const uint32_t* species,
+                                               int32_t n_tables, const double* tables_device) {
+    return guarded([&] { set_energy_grid(g, n_points, species, n_tables, tables_device, hipMemcpyDeviceToDevice); });
+}
+
+aleator_status aleator_gpu_gcmc_set_energy_grid_host(aleator_gcmc* g, const int32_t n_points[3],
+                                                    const uint32_t* species, int32_t n_tables, const double* tables) {
+    return guarded([&] { set_energy_grid(g, n_points, species, n_tables, tables, hipMemcpyHostToDevice); });
 }
 
 aleator_status aleator_gpu_gcmc_stats(aleator_gcmc* g, aleator_gcmc_stats* out) {

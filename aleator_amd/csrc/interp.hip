@@ -1,9 +1,6 @@
 // interp.hip -- FrameworkEnergyGrid::interpolate on the device
-// (engines/monte_carlo/framework_energy_grid.cc:251-284): fractional coordinates by
-// Lattice::cartesianToFractional (lattice.cc:167-175: dot with the cofactor rows, divided by
-// the volume), wrapped into [0, 1), then trilinear interpolation between the 8 surrounding
-// nodes (i/n) with periodic wrap, in the reference's operation order and without contraction,
-// so every output is the reference's double.  One thread per position; the table stays in
+// (engines/monte_carlo/framework_energy_grid.cc:251-284; the arithmetic is grid_interpolate in
+// interp.hpp, bitwise the reference's double).  One thread per position; the table stays in
 // HBM/L2 (8 gathers per position).
 #include <hip/hip_runtime.h>
 
@@ -11,47 +8,16 @@
 
 #include "aleator_gpu.h"
 #include "internal.hpp"
+#include "interp.hpp"
 
 namespace aleator_gpu {
 namespace {
 
-struct InterpArgs {
-    double c12[3], c20[3], c01[3];  // a1 x a2, a2 x a0, a0 x a1
-    double volume;
-    int n[3];
-};
-
-__global__ void interpolate_kernel(const InterpArgs a, const double* __restrict__ table,
+__global__ void interpolate_kernel(const GridGeometry a, const double* __restrict__ table,
                                    const double* __restrict__ pos, uint64_t count, double* __restrict__ out) {
-#pragma clang fp contract(off)
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-    double f[3] = {(x * a.c12[0] + y * a.c12[1] + z * a.c12[2]) / a.volume,
-                   (x * a.c20[0] + y * a.c20[1] + z * a.c20[2]) / a.volume,
-                   (x * a.c01[0] + y * a.c01[1] + z * a.c01[2]) / a.volume};
-    int i0[3], i1[3];
-    double t[3];
-    for (int ax = 0; ax < 3; ++ax) {
-        f[ax] = f[ax] - floor(f[ax]);
-        const int n = a.n[ax];
-        const double g = f[ax] * (double)n;
-        const int base = (int)floor(g);
-        t[ax] = g - (double)base;
-        const int wrapped = ((base % n) + n) % n;
-        i0[ax] = wrapped;
-        i1[ax] = (wrapped + 1) % n;
-    }
-    auto v = [&](int ix, int iy, int iz) {
-        return table[((size_t)iz * (size_t)a.n[1] + (size_t)iy) * (size_t)a.n[0] + (size_t)ix];
-    };
-    const double c00 = v(i0[0], i0[1], i0[2]) * (1.0 - t[0]) + v(i1[0], i0[1], i0[2]) * t[0];
-    const double c10 = v(i0[0], i1[1], i0[2]) * (1.0 - t[0]) + v(i1[0], i1[1], i0[2]) * t[0];
-    const double c01 = v(i0[0], i0[1], i1[2]) * (1.0 - t[0]) + v(i1[0], i0[1], i1[2]) * t[0];
-    const double c11 = v(i0[0], i1[1], i1[2]) * (1.0 - t[0]) + v(i1[0], i1[1], i1[2]) * t[0];
-    const double c0 = c00 * (1.0 - t[1]) + c10 * t[1];
-    const double c1 = c01 * (1.0 - t[1]) + c11 * t[1];
-    out[i] = c0 * (1.0 - t[2]) + c1 * t[2];
+    out[i] = grid_interpolate(a, table, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
 }
 
 }  // namespace
@@ -68,7 +34,7 @@ extern "C" aleator_status aleator_gpu_grid_interpolate(aleator_ctx* ctx_in, cons
         require(ctx && lattice && n_points, ALEATOR_INVALID_ARGUMENT, "null argument");
         if (n == 0) return;
         require(table_device && positions_device && out_device, ALEATOR_INVALID_ARGUMENT, "null argument");
-        InterpArgs a{};
+        GridGeometry a{};
         for (int i = 0; i < 3; ++i) {
             require(n_points[i] >= 1, ALEATOR_INVALID_ARGUMENT, "interpolate: point counts must be >= 1");
             a.n[i] = n_points[i];
@@ -76,17 +42,7 @@ extern "C" aleator_status aleator_gpu_grid_interpolate(aleator_ctx* ctx_in, cons
         const HostLattice h = make_lattice(lattice->m);
         require(std::abs(h.volume) >= 1e-10, ALEATOR_INVALID_ARGUMENT,
                 "Lattice::cartesianToFractional: degenerate lattice");
-        // cofactor rows: inv[r][j] * volume is column j of the inverse, i.e. row r of (a_{j+1} x a_{j+2})
-        const double(*m)[3] = lattice->m;
-        auto cross = [](const double* p, const double* q, double* o) {
-#pragma clang fp contract(off)
-            o[0] = p[1] * q[2] - p[2] * q[1];
-            o[1] = p[2] * q[0] - p[0] * q[2];
-            o[2] = p[0] * q[1] - p[1] * q[0];
-        };
-        cross(m[1], m[2], a.c12);
-        cross(m[2], m[0], a.c20);
-        cross(m[0], m[1], a.c01);
+        grid_cofactors(lattice->m, a);
         a.volume = h.volume;
         const uint64_t blocks = (n + 255) / 256;
         require(blocks <= 0x7fffffffULL, ALEATOR_INVALID_ARGUMENT, "interpolate: too many positions");

@@ -241,7 +241,7 @@ def host_threads() -> int:
         return os.cpu_count() or 1
 
 
-LEGS = ("c1", "c0", "c2", "c3", "ewald", "mlp", "irmof1", "c4")
+LEGS = ("c1", "c0", "c2", "c3", "ewald", "mlp", "irmof1", "gcmc_grid", "c4")
 
 
 def ewald_proposal_rate(aa, ctx, fw, sync, n_prop=100_000):
@@ -362,6 +362,83 @@ def irmof1_gcmc_rates(aa, ctx, sync):
             ol.ref_gcmc_run(c.lj, c.fw, c.sites, f, 298.0, 2026, 0, steps, ewald=c.ewald)
             times.append(time.perf_counter() - t0)
         out["irmof1_cpu_reference_co2_1e6Pa_us_per_step_after_2000"] = (times[1] - times[0]) / 500 * 1e6
+    return out
+
+
+def gcmc_grid_rates(aa, ctx, sync):
+    """GCMC with the framework LJ taken from an energy grid, in the reference's own measurement
+    (CH4 in IRMOF-1, UFF, rc 12 A shifted, 298 K, 0.1 bar Peng-Robinson, seed 7, 0.2 A grid; the
+    single cell, 129^3 nodes, and the 2x2x2 supercell, 258^3 nodes): the device build of the
+    FP64 table, device per-step time of 1 and 256 chains with and without the grid, and the
+    reference engine's per-step time on one host core with and without the grid, its grid
+    loaded from a cache file written here from the device table (the reference's own format)."""
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+
+    cif = os.path.join(ROOT, "tests", "golden", "cif")
+    uff = {"Zn": (62.3992, 2.46155), "O": (48.1581, 3.03315), "C": (47.8562, 3.47299), "H": (7.64893, 2.84642)}
+    s = aa.read_cif(os.path.join(cif, "IRMOF-1.cif"))
+    eps = [uff[e][0] for e in s.species_symbols] + [158.5]
+    sig = [uff[e][1] for e in s.species_symbols] + [3.72]
+    ch4 = len(s.species_symbols)
+    sites = [(0.0, 0.0, 0.0, ch4, 0.0)]
+    f = aa.peng_robinson_fugacity(190.6, 4.599e6, 0.011, 298.0, 0.1e5)
+    out = {}
+
+    def timed_run(gc, steps):
+        sync()
+        t0 = time.perf_counter()
+        gc.run(steps)
+        sync()
+        return time.perf_counter() - t0
+
+    for label, reps in (("1x1x1", (1, 1, 1)), ("2x2x2", (2, 2, 2))):
+        lat, x, y, z, q, sp = aa.replicate_supercell(s.lattice, s.x, s.y, s.z, s.charge, s.species, reps)
+        fw = aa.Framework(ctx, lat, x, y, z, sp, eps, sig, 12.0, truncation=aa.SHIFTED)
+        n = aa.grid_point_counts(lat, 0.2)
+        nodes = n[0] * n[1] * n[2]
+        buf = aa.DeviceBuffer(nodes)
+        fw.grid_device([ch4], n, int(buf))  # warm (tile choice, code load)
+        sync()
+        t0 = time.perf_counter()
+        fw.grid_device([ch4], n, int(buf))
+        sync()
+        key = f"gcmc_grid_irmof1_{label}"
+        out[key + "_grid_nodes"] = nodes
+        out[key + "_device_grid_build_s"] = time.perf_counter() - t0
+        for nch in (1, 256):
+            for mode in ("direct", "grid"):
+                gc = aa.Gcmc(fw, sites, [(f, 298.0, 7, c) for c in range(nch)], max_molecules=1024)
+                if mode == "grid":
+                    gc.set_energy_grid(n, [ch4], buf)
+                gc.run(2000)
+                dt = timed_run(gc, 20000)
+                out[f"{key}_{mode}_{nch}chains_us_per_step"] = dt / 20000 * 1e6
+                out[f"{key}_{mode}_{nch}chains_chain_steps_per_s"] = nch * 20000 / dt
+                gc.close()
+        if ol.have_ref():
+            ofw = ol.Framework(m=np.asarray(lat), x=x, y=y, z=z, q=np.zeros(len(x)), species=sp)
+            lj = ol.LJ(np.array(eps), np.array(sig), 12.0, ol.SHIFTED)
+            ck = aa.GridCacheKey(lat, x, y, z, sp, eps, sig, 12.0, aa.SHIFTED, [ch4], 0.2)
+            steps = (2000, 6000) if reps != (1, 1, 1) else (10000, 30000)
+            times = []
+            for st in steps:
+                t0 = time.perf_counter()
+                ol.ref_gcmc_run(lj, ofw, sites, f, 298.0, 7, 0, st)
+                times.append(time.perf_counter() - t0)
+            out[key + "_cpu_reference_direct_us_per_step"] = (times[1] - times[0]) / (steps[1] - steps[0]) * 1e6
+            if ck.hash() == ol.ref_grid_cache_hash(ofw, lj, [ch4], 0.2):  # the reference will load, not build
+                with tempfile.TemporaryDirectory() as d:
+                    aa.grid_cache_serialize(ck.file_path(d), ck, n, [ch4], buf.to_host().reshape(1, n[2], n[1], n[0]))
+                    times = []
+                    for st in (20000, 220000):  # the difference leaves out the cache load
+                        t0 = time.perf_counter()
+                        ol.ref_gcmc_run_grid(lj, ofw, sites, f, 298.0, 7, 0, st, 0.2, [ch4], d)
+                        times.append(time.perf_counter() - t0)
+                out[key + "_cpu_reference_grid_us_per_step"] = (times[1] - times[0]) / 200000 * 1e6
+        buf.free()
+        fw.close()
     return out
 
 
@@ -558,6 +635,12 @@ def main():
             extra.update(irmof1_gcmc_rates(aa, ctx, sync))
         except Exception as e:
             extra["irmof1_error"] = repr(e)
+
+    if "gcmc_grid" in legs:
+        try:
+            extra.update(gcmc_grid_rates(aa, ctx, sync))
+        except Exception as e:
+            extra["gcmc_grid_error"] = repr(e)
 
     if "c4" in legs:
         # c4: learned insertion density vs plain uniform insertion on the c3 chains

@@ -344,6 +344,24 @@ aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* gcmc);
  * orientation as before, acceptance.  w = NULL restores uniform insertion. */
 aleator_status aleator_gpu_gcmc_set_proposal(aleator_gcmc* gcmc, const double* w, const int32_t nb[3],
                                              const int32_t reps[3], double uniform_mix);
+/* The frameworkEnergyGrid argument of MonteCarloEngine's constructor
+ * (monte_carlo_engine.hpp:129-151, used at monte_carlo_engine.cc:341-374): from the next run on,
+ * a site's framework LJ energy is its species' table interpolated at the site
+ * (FrameworkEnergyGrid::interpolate, framework_energy_grid.cc:251-284) instead of the sum over
+ * framework atoms; real-space Coulomb with the framework and every guest-guest term stay exact.
+ * tables_device: FP64 [n_tables][n2][n1][n0] (aleator_gpu_grid_build_device's layout, built
+ * on the framework's own cell) for LJ species `species[t]`; it is copied, so the caller may free
+ * it after the call.  ALEATOR_OUT_OF_RANGE when a site's species has no table (the reference
+ * throws std::out_of_range at the first interpolation).  tables_device = NULL restores the
+ * direct scan. */
+aleator_status aleator_gpu_gcmc_set_energy_grid(aleator_gcmc* gcmc, const int32_t n_points[3],
+                                                const uint32_t* species, int32_t n_tables,
+                                                const double* tables_device);
+/* Same with the tables in host memory (a FrameworkEnergyGrid's, or a cache file's as
+ * aleator_grid_cache_load returns them). */
+aleator_status aleator_gpu_gcmc_set_energy_grid_host(aleator_gcmc* gcmc, const int32_t n_points[3],
+                                                     const uint32_t* species, int32_t n_tables,
+                                                     const double* tables);
 
 /* ---- learned insertion density: Fourier-feature MLP (no reference counterpart) ----
  * 64 features (sin, cos of 2 pi k.f for 32 integer vectors k, aleator_gpu_mlp_frequencies)

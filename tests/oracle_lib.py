@@ -655,3 +655,16 @@ def ref_gcmc_run_grid(lj: LJ, fw: Framework, sites, fugacity_pa, temperature, se
         raise RuntimeError("reference engine failed")
     n = npart.value
     return nmol.value, tuple(a[:n] for a in out), e.value
+
+
+def ref_grid_cache_hash(fw: Framework, lj: LJ, guests, spacing) -> int:
+    """The compiled reference's makeFrameworkEnergyGridCacheKey(...).hash() for these inputs."""
+    g = np.ascontiguousarray(guests, dtype=np.uint32)
+    m = np.ascontiguousarray(fw.m)
+    h = C.c_uint64()
+    rc = ref().ref_grid_cache_hash(C.c_uint32(len(lj.eps)), ptr(lj.eps), ptr(lj.sigma), C.c_double(lj.cutoff),
+                                   lj.truncation, lj.mixing, ptr(m), C.c_uint64(fw.n), ptr(fw.x), ptr(fw.y), ptr(fw.z),
+                                   ptr(fw.species, u32p), ptr(g, u32p), len(g), C.c_double(spacing), C.byref(h))
+    if rc != 0:
+        raise RuntimeError("reference key hash failed")
+    return h.value

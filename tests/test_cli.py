@@ -7,7 +7,10 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
+
+import oracle_lib as ol
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -103,10 +106,51 @@ def test_config_errors_name_the_key(tmp_path, edit, message):
     assert re.search(r"c\.toml:\d+:\d+: \[", err) or "no entry for element" in err or "upward" in err
 
 
-def test_grid_spacing_is_rejected_not_ignored(tmp_path):
-    c = config(tmp_path, lambda t: t.replace("cutoff_angstrom = 12.0", "cutoff_angstrom = 12.0\nenergy_grid_spacing_angstrom = 0.2"))
+def grid_config(tmp_path, spacing=1.0):
+    return config(tmp_path, lambda t: t.replace(
+        "cutoff_angstrom = 12.0", f"cutoff_angstrom = 12.0\nenergy_grid_spacing_angstrom = {spacing}"))
+
+
+def example_grid_key(spacing):
+    """The cache key the reference CLI computes for the example run (cli/main.cc:278-286):
+    the IRMOF-1 structure as read, UFF parameters by element then CH4, rc 12 shifted, guest CH4."""
+    import aleator_amd as aa
+    s = aa.read_cif(IRMOF)
+    uff = {"Zn": (62.3992, 2.46155), "O": (48.1581, 3.03315), "C": (47.8562, 3.47299), "H": (7.64893, 2.84642)}
+    eps = [uff[e][0] for e in s.species_symbols] + [158.5]
+    sig = [uff[e][1] for e in s.species_symbols] + [3.72]
+    ch4 = len(s.species_symbols)
+    key = aa.GridCacheKey(s.lattice, s.x, s.y, s.z, s.species, eps, sig, 12.0, aa.SHIFTED, [ch4], spacing)
+    return key, aa.grid_point_counts(s.lattice, spacing), ch4
+
+
+def test_grid_cache_status_in_dry_run(tmp_path):
+    """frameworkEnergyGridCacheHitForDryRun (cli/main.cc:331-341): a dry run reports whether the
+    cache holds the run's grid (from the file header) and never builds one."""
+    import aleator_amd as aa
+    c = grid_config(tmp_path)
+    rc, out, err = cli("gcmc", "run", c, "--dry-run", "--json")
+    assert rc == 0, err
+    d = json.loads(out)["gcmc"]
+    assert d["energy_grid_spacing_angstrom"] == 1.0 and '"energy_grid_spacing_angstrom":1.000000' in out
+    assert d["energy_grid_cache_directory"] == str(tmp_path / "energy_grid_cache")
+    assert d["energy_grid_cache_status"] == "will_build"
+    rc, out, _ = cli("gcmc", "run", c, "--dry-run")
+    assert rc == 0 and "gcmc.energy_grid_cache_status  = miss (will build and cache)" in out
+    assert not (tmp_path / "energy_grid_cache").exists()
+    # the entry a run would write, made here from the reference's key: the dry run sees a hit
+    key, n, ch4 = example_grid_key(1.0)
+    path = key.file_path(str(tmp_path / "energy_grid_cache"))
+    aa.grid_cache_serialize(path, key, n, [ch4], np.zeros((1, n[2], n[1], n[0])))
+    rc, out, _ = cli("gcmc", "run", c, "--dry-run", "--json")
+    assert rc == 0 and json.loads(out)["gcmc"]["energy_grid_cache_status"] == "hit"
+    rc, out, _ = cli("gcmc", "run", c, "--dry-run")
+    assert "gcmc.energy_grid_cache_status  = hit (will load, not rebuild)" in out
+    # a truncated file at the key's path is an error, as there
+    with open(path, "r+b") as f:
+        f.truncate(40)
     rc, _, err = cli("gcmc", "run", c, "--dry-run")
-    assert rc == 1 and "energy_grid_spacing_angstrom" in err
+    assert rc == 1 and "error" in err
 
 
 def test_pore_and_unknown_configs(tmp_path):
@@ -161,3 +205,35 @@ def test_cli_widom_matches_the_reference_self_consistency(tmp_path):
     rel = abs(implied - kh) / kh
     print(f"K_H {kh:.6e}, GCMC-implied {implied:.6e}, relative difference {100 * rel:.3f} %")
     assert abs(rel - 0.0342) < 0.0006
+
+
+@pytest.mark.gpu
+def test_cli_run_with_energy_grid(tmp_path):
+    """gcmc.energy_grid_spacing_angstrom = 1.0: the first run builds the CH4 table on the device
+    and caches it, the second loads it; both print the same <N>, which is also what the oracle
+    (bitwise the reference engine in grid mode) gives on the cached table."""
+    import aleator_amd as aa
+    import irmof1_systems as irm
+    print("aleator-hip gcmc run with a 1.0 A grid: build, then load", flush=True)
+    c = grid_config(tmp_path)
+    rc, out, err = cli("gcmc", "run", c, "--json", timeout=300)
+    assert rc == 0, err
+    assert "energy grid built and cached" in err
+    n_built = json.loads(out)["mean_molecule_count"]
+    art = json.loads((tmp_path / "out" / "resolved_config.json").read_text())
+    assert art["gcmc"]["energy_grid_cache_status"] == "will_build"
+    rc, out, err = cli("gcmc", "run", c, "--json", timeout=300)
+    assert rc == 0 and "energy grid loaded from cache" in err
+    assert json.loads(out)["mean_molecule_count"] == n_built
+    assert json.loads((tmp_path / "out" / "resolved_config.json").read_text())["gcmc"]["energy_grid_cache_status"] == "hit"
+    key, n, ch4 = example_grid_key(1.0)
+    got = aa.grid_cache_load(str(tmp_path / "energy_grid_cache"), key)
+    assert got is not None and got[0] == n and list(got[1]) == [ch4]
+    sysm = irm.methane(0)
+    g = ol.Gcmc(sysm.lj, sysm.fw, sysm.sites, sysm.fugacity, sysm.temperature, irm.SEED, 0)
+    g.set_grid(n, [ch4], got[2])
+    g.run(sysm.equilibration)
+    trace = g.run(sysm.production, trace=True)
+    mean = irm._seq_sum(trace) / sysm.production
+    print(f"<N> grid mode: device {n_built:.6g}, oracle {mean:.6g}")
+    assert f"{n_built:.6g}" == f"{mean:.6g}"

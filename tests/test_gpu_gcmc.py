@@ -170,3 +170,84 @@ def test_proposal_errors(ctx):
     with pytest.raises(ValueError):
         off_centre.set_proposal(np.ones(2), (2, 1, 1), (1, 1, 1), 0.1)
     gc.set_proposal(None, (1, 1, 1), (1, 1, 1))
+
+
+def test_gcmc_methane_with_energy_grid_follows_oracle(ctx):
+    """Grid mode (MonteCarloEngine given a FrameworkEnergyGrid, monte_carlo_engine.cc:341-374):
+    the oracle's grid mode is bitwise the compiled reference's (test_oracle_vs_reference.py).
+    Chains 0-2 use the oracle's tables; a fourth chain object uses tables built on the device
+    (FP64, within 1e-10 of them) and must follow the same trajectory.  Switching the grid off
+    mid-run goes back to the direct scan on both sides."""
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff, lj.truncation = 6.5, ol.SHIFTED
+    sites = [(0.0, 0.0, 0.0, 4, 0.0)]
+    counts = ol.grid_counts(fw.m, 0.5)
+    tables = ol.grid_build(lj, fw, [4], counts)
+    dev = device_framework(ctx, fw, lj)
+    chains = [(1e8, 300.0, 5, 0), (3e7, 300.0, 6, 2), (1e8, 250.0, 7, 1)]
+    gc = aa.Gcmc(dev, sites, chains, max_molecules=256)
+    gc.set_energy_grid(counts, [4], tables)
+    buf = aa.DeviceBuffer(tables.size)
+    dev.grid_device([4], counts, int(buf))
+    gd = aa.Gcmc(dev, sites, chains[:1], max_molecules=256)
+    gd.set_energy_grid(counts, [4], buf)
+    buf.free()  # the chains keep their own copy
+    trace = gc.run(3000, trace=True)
+    trace_d = gd.run(3000, trace=True)
+    gc.set_energy_grid(None, None, None)
+    trace2 = gc.run(1000, trace=True)
+    for c, (f, t, seed, stream) in enumerate(chains):
+        g = ol.Gcmc(lj, fw, sites, f, t, seed, stream)
+        g.set_grid(counts, [4], tables)
+        ref_trace = g.run(3000, trace=True)
+        assert np.array_equal(trace[c], ref_trace.astype(np.int32)), f"chain {c} trajectory diverged"
+        if c == 0:
+            assert np.array_equal(trace_d[0], ref_trace.astype(np.int32)), "device-built tables diverged"
+            assert np.allclose(gd.positions(0), oracle_positions(g), rtol=0, atol=1e-9)
+        g.set_grid(None, None, None)
+        ref_trace2 = g.run(1000, trace=True)
+        assert np.array_equal(trace2[c], ref_trace2.astype(np.int32)), f"chain {c} diverged after the grid was removed"
+        assert np.allclose(gc.positions(c), oracle_positions(g), rtol=0, atol=1e-9)
+        att, acc = g.counts
+        assert gc.stats()[c]["attempted"] == att and gc.stats()[c]["accepted"] == acc
+
+
+def test_gcmc_charged_co2_with_energy_grid_follows_oracle(ctx):
+    """Grid mode with Ewald: LJ of C and O from their tables, real-space Coulomb with the
+    framework summed directly, S(k) as without a grid."""
+    m = ol.cell_from_params(16.0, 17.0, 18.0, 80.0, 95.0, 100.0)
+    fw = ol.synth_framework(m, 120, seed=4, charge_sigma=0.5)
+    eps = np.array([t[0] for t in ol.C0_TYPES] + [ol.CO2_C[0], ol.CO2_O[0]])
+    sig = np.array([t[1] for t in ol.C0_TYPES] + [ol.CO2_C[1], ol.CO2_O[1]])
+    lj = ol.LJ(eps, sig, 7.5)
+    counts = ol.grid_counts(fw.m, 0.6)
+    tables = ol.grid_build(lj, fw, [4, 5], counts)
+    dev = device_framework(ctx, fw, lj, ewald=(0.35, 7.5, 5))
+    chains = [(1e6, 298.0, 3, 1), (5e6, 298.0, 4, 2)]
+    gc = aa.Gcmc(dev, ol.co2_sites(), chains, max_molecules=256)
+    gc.set_energy_grid(counts, [4, 5], tables)
+    trace = gc.run(1500, trace=True)
+    for c, (f, t, seed, stream) in enumerate(chains):
+        g = ol.Gcmc(lj, fw, ol.co2_sites(), f, t, seed, stream, ewald=(0.35, 7.5, 5))
+        g.set_grid(counts, [4, 5], tables)
+        ref_trace = g.run(1500, trace=True)
+        assert np.array_equal(trace[c], ref_trace.astype(np.int32)), f"chain {c} trajectory diverged"
+        assert gc.stats()[c]["n_molecules"] == g.n_molecules > 0
+        assert np.allclose(gc.positions(c), oracle_positions(g), rtol=0, atol=1e-9)
+        e_dev, e_ref = gc.electrostatic_energy(c), g.electrostatic_energy()
+        assert abs(e_dev - e_ref) <= 1e-9 * abs(e_ref)
+
+
+def test_gcmc_energy_grid_errors(ctx):
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff = 6.5
+    dev = device_framework(ctx, fw, lj)
+    gc = aa.Gcmc(dev, [(0, 0, 0, 4, 0.0)], [(1e5, 300.0, 1, 0)])
+    with pytest.raises(IndexError):  # framework_energy_grid.cc:254-257: no table for species 4
+        gc.set_energy_grid((4, 4, 4), [3], np.zeros((1, 4, 4, 4)))
+    with pytest.raises(ValueError):
+        gc.set_energy_grid((4, 4, 4), [4], np.zeros((1, 4, 4, 3)))
+    with pytest.raises(ValueError):
+        gc.set_energy_grid((0, 4, 4), [4], np.zeros(0))
+    gc.run(10)
+    assert gc.stats()[0]["steps"] == 10
