@@ -45,10 +45,28 @@ constexpr int kCap = 1024;   // LDS list capacity (entries)
 constexpr int kChunk = 128;  // atoms staged per pass (8 images each <= kCap)
 constexpr int kErfcIntervals = 256;  // FP32 erfc(alpha r) table: cubic Hermite pieces over [0, alpha rc]
 
+// LDS image list: position, charge and compact LJ type of each atom image.  FP64 keeps
+// structure-of-arrays; FP32 packs (x, y, z, q) in one float4 so an entry is two LDS reads.
 template <typename R>
 struct Entries {
     R x[kCap], y[kCap], z[kCap], q[kCap];
     int t[kCap];
+    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_) {
+        x[w] = (R)x_;
+        y[w] = (R)y_;
+        z[w] = (R)z_;
+        q[w] = (R)q_;
+        t[w] = t_;
+    }
+};
+template <>
+struct Entries<float> {
+    float4 p[kCap];
+    int t[kCap];
+    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_) {
+        p[w] = make_float4((float)x_, (float)y_, (float)z_, (float)q_);
+        t[w] = t_;
+    }
 };
 
 template <typename R>
@@ -62,7 +80,9 @@ __device__ inline float rcp<float>(float v) {
     return __builtin_amdgcn_rcpf(v);
 }
 
-template <typename R, int NG, bool COUL, int TX, int TY, int TZ, int NPT>
+// LFS: some framework of the launch uses linear-force-shifted LJ (FP32 path: the slope term is
+// compiled in; the FP64 path reads the truncation per framework at run time).
+template <typename R, int NG, bool COUL, bool LFS, int TX, int TY, int TZ, int NPT>
 __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __restrict__ descs,
                                                              const int* __restrict__ prefix, int n_desc) {
     static_assert(TX * TY * TZ == kThreads, "one thread per (x, y, z-group)");
@@ -71,8 +91,8 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
     __shared__ Entries<R> ent;
     __shared__ double tab[4][kMaxGuests][kMaxTypes];
     // FP32 pair constants per (guest, type): A = 4 eps sigma^12, B = 4 eps sigma^6, the shift
-    // V(rc) (0 when truncated) and the slope V'(rc) (0 unless linear-force-shifted), so every
-    // truncation takes the same instructions: V = A/r^12 - B/r^6 - shift - (r - rc) slope
+    // V(rc) - rc V'(rc) and the slope V'(rc) (shift 0 when truncated, slope 0 unless
+    // linear-force-shifted): V = A/r^12 - (B/r^6 + shift + r slope), the slope term only with LFS
     __shared__ float4 tabf[kMaxGuests][kMaxTypes];
     __shared__ float4 erfc_tab[(std::is_same_v<R, float> && COUL) ? kErfcIntervals : 1];
     __shared__ int scan_tmp[kThreads / 64 + 1];
@@ -94,9 +114,10 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
         for (int i = tid; i < kPairs; i += kThreads) {
             const double s2 = a.ljtab[i], e4 = a.ljtab[kPairs + i];
             const double s6 = s2 * s2 * s2;
-            const double shift = a.trunc != ALEATOR_LJ_TRUNCATED ? a.ljtab[2 * kPairs + i] : 0.0;
+            const double vrc = a.trunc != ALEATOR_LJ_TRUNCATED ? a.ljtab[2 * kPairs + i] : 0.0;
             const double slope = a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED ? a.ljtab[3 * kPairs + i] : 0.0;
-            (&tabf[0][0])[i] = make_float4((float)(e4 * s6 * s6), (float)(e4 * s6), (float)shift, (float)slope);
+            (&tabf[0][0])[i] =
+                make_float4((float)(e4 * s6 * s6), (float)(e4 * s6), (float)(vrc - a.rc_lj * slope), (float)slope);
         }
     } else {
         for (int i = tid; i < 4 * kMaxGuests * kMaxTypes; i += kThreads) (&tab[0][0][0])[i] = a.ljtab[i];
@@ -145,6 +166,21 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
         valid[p] = ix < n0 && iy < n1 && iz0 + p < n2;
     }
     const bool any_valid = valid[0];
+    // node p of a thread sits p steps s (one grid step along the third lattice row) past node 0,
+    // so |e - o_p|^2 = |e - o_0|^2 - 2p (e - o_0).s + p^2 |s|^2: two instructions per extra node
+    // instead of six, and no cancellation because |s| is a grid spacing
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, s2p[NPT], m2p[NPT];
+    if constexpr (kF32) {
+        const double gx = a.mg[6] / n2, gy = a.mg[7] / n2, gz = a.mg[8] / n2;
+        sx = (float)gx;
+        sy = (float)gy;
+        sz = (float)gz;
+#pragma unroll
+        for (int p = 0; p < NPT; ++p) {
+            s2p[p] = (float)((double)(p * p) * (gx * gx + gy * gy + gz * gz));
+            m2p[p] = (float)(-2 * p);
+        }
+    }
 
     double acc[NPT][NG > 0 ? NG : 1];
     double accq[NPT];
@@ -164,45 +200,65 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
     auto compute = [&](int count) {
         __syncthreads();
         if constexpr (kF32) {
-            // 1/r from one reciprocal square root: r = r2 / sqrt(r2), 1/r^2 = (1/r)^2
-            if (any_valid) {
+            // 1/r from one reciprocal square root: r = r2 / sqrt(r2), 1/r^2 = (1/r)^2.  When the LJ
+            // and real-space cutoffs are equal (or only one term is built) one branch covers both
+            // terms; otherwise each term has its own branch.
+            auto pairs = [&](auto same_cutoffs) {
+                constexpr bool kOne = decltype(same_cutoffs)::value;
 #pragma unroll 2
                 for (int e = 0; e < count; ++e) {
-                    const float ex = ent.x[e], ey = ent.y[e], ez = ent.z[e];
+                    const float4 P = ent.p[e];
                     const int t = ent.t[e];
                     float4 cg[NG > 0 ? NG : 1];
 #pragma unroll
                     for (int g = 0; g < NG; ++g) cg[g] = tabf[g][t];
+                    const float dx = P.x - ox[0];
+                    const float dy = P.y - oy[0];
+                    const float dz = P.z - oz[0];
+                    const float r20 = dx * dx + dy * dy + dz * dz;
+                    const float tds = NPT > 1 ? dx * sx + dy * sy + dz * sz : 0.0f;
 #pragma unroll
                     for (int p = 0; p < NPT; ++p) {
-                        const float dx = ex - ox[p];
-                        const float dy = ey - oy[p];
-                        const float dz = ez - oz[p];
-                        const float r2 = dx * dx + dy * dy + dz * dz;
-                        const float ir = __builtin_amdgcn_rsqf(r2);
-                        const float r = r2 * ir;
-                        if (NG > 0 && r2 <= rc2) {
+                        const float r2 = p == 0 ? r20 : fmaf(tds, m2p[p], r20 + s2p[p]);
+                        auto lj = [&](float ir) {
                             const float ir2 = ir * ir;
                             const float ir6 = ir2 * ir2 * ir2;
                             const float ir12 = ir6 * ir6;
-                            const float dr = r - rc;
 #pragma unroll
                             for (int g = 0; g < NG; ++g) {
-                                const float w = fmaf(cg[g].y, ir6, fmaf(dr, cg[g].w, cg[g].z));
+                                float w = fmaf(cg[g].y, ir6, cg[g].z);
+                                if constexpr (LFS) w = fmaf(r2 * ir, cg[g].w, w);
                                 acc[p][g] += (double)fmaf(cg[g].x, ir12, -w);
                             }
-                        }
-                        if (COUL && r2 <= rcr2) {
+                        };
+                        auto coul = [&](float ir) {
                             // r <= rc(1 + 2^-22) here, so 0 <= u < kErfcIntervals and the piece
                             // index (int)u = floor(u) stays inside the table
-                            const float u = r * erfc_scale;
+                            const float u = (r2 * ir) * erfc_scale;
                             const float f = __builtin_amdgcn_fractf(u);
                             const float4 c = erfc_tab[(int)u];
                             const float erfcz = fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
-                            accq[p] += (double)(ent.q[e] * (erfcz * ir));
+                            accq[p] += (double)(P.w * (erfcz * ir));
+                        };
+                        if constexpr (kOne) {
+                            if (r2 <= (NG > 0 ? rc2 : rcr2)) {
+                                const float ir = __builtin_amdgcn_rsqf(r2);
+                                if constexpr (NG > 0) lj(ir);
+                                if constexpr (COUL) coul(ir);
+                            }
+                        } else {
+                            const float ir = __builtin_amdgcn_rsqf(r2);
+                            if (r2 <= rc2) lj(ir);
+                            if (r2 <= rcr2) coul(ir);
                         }
                     }
                 }
+            };
+            if (any_valid) {
+                if (NG == 0 || !COUL || rc2 == rcr2)
+                    pairs(std::true_type{});
+                else
+                    pairs(std::false_type{});
             }
         } else if (any_valid) {
 #pragma unroll 2
@@ -301,11 +357,8 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
             for (int s = 0; s < 8; ++s) {
                 if (!(mask & (1u << s))) continue;
                 const double g0 = h0 - (s & 1), g1 = h1 - ((s >> 1) & 1), g2 = h2 - (s >> 2);
-                ent.x[w] = (R)(g0 * a.mi[0] + g1 * a.mi[3] + g2 * a.mi[6]);
-                ent.y[w] = (R)(g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7]);
-                ent.z[w] = (R)(g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8]);
-                ent.q[w] = (R)qa;
-                ent.t[w] = ta;
+                ent.put(w, g0 * a.mi[0] + g1 * a.mi[3] + g2 * a.mi[6], g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7],
+                        g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8], qa, ta);
                 ++w;
             }
         }
@@ -375,13 +428,13 @@ struct TileShape {
 };
 constexpr TileShape kShapes[kNumTileShapes] = {{8, 8, 4, 1}, {8, 8, 4, 2}, {8, 8, 4, 4}, {4, 8, 8, 2}};
 
-template <typename R, int NG, bool COUL>
+template <typename R, int NG, bool COUL, bool LFS>
 void launch_shape(int shape, const GridArgs* d, const int* prefix, int n_desc, int blocks, hipStream_t s) {
     switch (shape) {
-#define SHAPE(I, X, Y, Z, P)                                                                              \
-    case I:                                                                                               \
-        hipLaunchKernelGGL((grid_tile_kernel<R, NG, COUL, X, Y, Z, P>), dim3(blocks), dim3(kThreads), 0, s, \
-                           d, prefix, n_desc);                                                            \
+#define SHAPE(I, X, Y, Z, P)                                                                                   \
+    case I:                                                                                                    \
+        hipLaunchKernelGGL((grid_tile_kernel<R, NG, COUL, LFS, X, Y, Z, P>), dim3(blocks), dim3(kThreads), 0, s, \
+                           d, prefix, n_desc);                                                                 \
         return;
         SHAPE(0, 8, 8, 4, 1)
         SHAPE(1, 8, 8, 4, 2)
@@ -392,23 +445,23 @@ void launch_shape(int shape, const GridArgs* d, const int* prefix, int n_desc, i
     fail(ALEATOR_INVALID_ARGUMENT, "grid: unknown tile shape");
 }
 
-template <typename R>
+template <typename R, bool LFS>
 void dispatch(int shape, int ng, bool coul, const GridArgs* d, const int* prefix, int n_desc, int blocks,
               hipStream_t s) {
     if (coul) {
         switch (ng) {
-            case 0: return launch_shape<R, 0, true>(shape, d, prefix, n_desc, blocks, s);
-            case 1: return launch_shape<R, 1, true>(shape, d, prefix, n_desc, blocks, s);
-            case 2: return launch_shape<R, 2, true>(shape, d, prefix, n_desc, blocks, s);
-            case 3: return launch_shape<R, 3, true>(shape, d, prefix, n_desc, blocks, s);
-            case 4: return launch_shape<R, 4, true>(shape, d, prefix, n_desc, blocks, s);
+            case 0: return launch_shape<R, 0, true, LFS>(shape, d, prefix, n_desc, blocks, s);
+            case 1: return launch_shape<R, 1, true, LFS>(shape, d, prefix, n_desc, blocks, s);
+            case 2: return launch_shape<R, 2, true, LFS>(shape, d, prefix, n_desc, blocks, s);
+            case 3: return launch_shape<R, 3, true, LFS>(shape, d, prefix, n_desc, blocks, s);
+            case 4: return launch_shape<R, 4, true, LFS>(shape, d, prefix, n_desc, blocks, s);
         }
     } else {
         switch (ng) {
-            case 1: return launch_shape<R, 1, false>(shape, d, prefix, n_desc, blocks, s);
-            case 2: return launch_shape<R, 2, false>(shape, d, prefix, n_desc, blocks, s);
-            case 3: return launch_shape<R, 3, false>(shape, d, prefix, n_desc, blocks, s);
-            case 4: return launch_shape<R, 4, false>(shape, d, prefix, n_desc, blocks, s);
+            case 1: return launch_shape<R, 1, false, LFS>(shape, d, prefix, n_desc, blocks, s);
+            case 2: return launch_shape<R, 2, false, LFS>(shape, d, prefix, n_desc, blocks, s);
+            case 3: return launch_shape<R, 3, false, LFS>(shape, d, prefix, n_desc, blocks, s);
+            case 4: return launch_shape<R, 4, false, LFS>(shape, d, prefix, n_desc, blocks, s);
         }
     }
     fail(ALEATOR_INVALID_ARGUMENT, "grid: unsupported guest count");
@@ -617,12 +670,14 @@ void GridJob::launch_pre(hipStream_t s) const {
                        args.n[2], d_t2);
 }
 
-void launch_tiles(int precision, int shape, int ng, bool coulomb, const GridArgs* d_descs, const int* d_prefix,
-                  int n_desc, int blocks, hipStream_t s) {
+void launch_tiles(int precision, int shape, int ng, bool coulomb, bool lfs, const GridArgs* d_descs,
+                  const int* d_prefix, int n_desc, int blocks, hipStream_t s) {
     if (precision == ALEATOR_PRECISION_FP64)
-        dispatch<double>(shape, ng, coulomb, d_descs, d_prefix, n_desc, blocks, s);
+        dispatch<double, false>(shape, ng, coulomb, d_descs, d_prefix, n_desc, blocks, s);
+    else if (lfs)
+        dispatch<float, true>(shape, ng, coulomb, d_descs, d_prefix, n_desc, blocks, s);
     else
-        dispatch<float>(shape, ng, coulomb, d_descs, d_prefix, n_desc, blocks, s);
+        dispatch<float, false>(shape, ng, coulomb, d_descs, d_prefix, n_desc, blocks, s);
 }
 
 // single build through the context scratch buffer (the per-call entry points)
@@ -641,7 +696,8 @@ void launch_grid(Framework& fw, const GridRequest& req) {
     ALEATOR_HIP_CHECK(hipMemcpyAsync(base + off_prefix, prefix, sizeof(prefix), hipMemcpyHostToDevice, ctx.stream));
     job.launch_pre(ctx.stream);
     ctx.time_begin();
-    launch_tiles(job.precision, job.shape_used(), job.ng, job.coulomb, reinterpret_cast<const GridArgs*>(base + off_desc),
+    launch_tiles(job.precision, job.shape_used(), job.ng, job.coulomb, job.args.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED,
+                 reinterpret_cast<const GridArgs*>(base + off_desc),
                  reinterpret_cast<const int*>(base + off_prefix), 1, job.blocks, ctx.stream);
     ctx.time_end();
     ALEATOR_HIP_CHECK(hipGetLastError());

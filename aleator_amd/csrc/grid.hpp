@@ -62,7 +62,8 @@ struct GridJob {
 };
 
 int tile_blocks(const int32_t n[3], int shape);
-void launch_tiles(int precision, int shape, int ng, bool coulomb, const GridArgs* d_descs, const int* d_prefix,
-                  int n_desc, int blocks, hipStream_t s);
+// lfs: some framework of the launch uses linear-force-shifted LJ
+void launch_tiles(int precision, int shape, int ng, bool coulomb, bool lfs, const GridArgs* d_descs,
+                  const int* d_prefix, int n_desc, int blocks, hipStream_t s);
 
 }  // namespace aleator_gpu

@@ -58,7 +58,9 @@ void plan_launch(aleator_grid_plan* p) {
     hipStream_t s = p->ctx->stream;
     for (const auto& j : p->jobs) j.launch_pre(s);
     const GridJob& j0 = p->jobs.front();
-    launch_tiles(j0.precision, p->shape, j0.ng, j0.coulomb, p->d_descs, p->d_prefix, (int)p->jobs.size(),
+    bool lfs = false;
+    for (const auto& j : p->jobs) lfs = lfs || j.args.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED;
+    launch_tiles(j0.precision, p->shape, j0.ng, j0.coulomb, lfs, p->d_descs, p->d_prefix, (int)p->jobs.size(),
                  p->total_blocks, s);
 }
 
