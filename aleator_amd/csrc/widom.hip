@@ -313,14 +313,32 @@ template <typename R>
 struct WEntries {
     R x[kWCap], y[kWCap], z[kWCap], q[kWCap];
     int t[kWCap];
+    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_) {
+        x[w] = (R)x_;
+        y[w] = (R)y_;
+        z[w] = (R)z_;
+        q[w] = (R)q_;
+        t[w] = t_;
+    }
+};
+// FP32: (x, y, z, q) packed in one float4, so an entry is two LDS reads
+template <>
+struct WEntries<float> {
+    float4 p[kWCap];
+    int t[kWCap];
+    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_) {
+        p[w] = make_float4((float)x_, (float)y_, (float)z_, (float)q_);
+        t[w] = t_;
+    }
 };
 
-template <typename R, int NS, bool CHARGED, int TPT>
+// LFS: linear-force-shifted LJ (FP32 path: the slope term is compiled in only then).
+template <typename R, int NS, bool CHARGED, bool LFS, int TPT>
 __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a, const CellArgs c) {
     constexpr bool kF32 = std::is_same_v<R, float>;
     __shared__ WEntries<R> ent;
     __shared__ double tab[4][NS][kMaxTypes];
-    __shared__ float4 tabf[NS][kMaxTypes];  // A = 4 eps s^12, B = 4 eps s^6, shift, slope
+    __shared__ float4 tabf[NS][kMaxTypes];  // A = 4 eps s^12, B = 4 eps s^6, V(rc) - rc slope, slope
     __shared__ float4 erfc_tab[(kF32 && CHARGED) ? kWThreads : 1];
     __shared__ int scan_tmp[kWThreads / 64 + 1];
     const int tid = threadIdx.x;
@@ -334,10 +352,11 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
             const double s2 = a.ljtab[(0 * kMaxSites + s) * kMaxTypes + ty];
             const double e4 = a.ljtab[(1 * kMaxSites + s) * kMaxTypes + ty];
             const double s6 = s2 * s2 * s2;
-            const double shift = a.trunc != ALEATOR_LJ_TRUNCATED ? a.ljtab[(2 * kMaxSites + s) * kMaxTypes + ty] : 0.0;
+            const double vrc = a.trunc != ALEATOR_LJ_TRUNCATED ? a.ljtab[(2 * kMaxSites + s) * kMaxTypes + ty] : 0.0;
             const double slope =
                 a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED ? a.ljtab[(3 * kMaxSites + s) * kMaxTypes + ty] : 0.0;
-            tabf[s][ty] = make_float4((float)(e4 * s6 * s6), (float)(e4 * s6), (float)shift, (float)slope);
+            tabf[s][ty] =
+                make_float4((float)(e4 * s6 * s6), (float)(e4 * s6), (float)(vrc - a.rc_lj * slope), (float)slope);
         }
     } else {
         for (int i = tid; i < 4 * NS * kMaxTypes; i += kWThreads) {
@@ -391,55 +410,82 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
 
         auto compute = [&](int count) {
             __syncthreads();
-            if (any) {
+            if constexpr (kF32) {
+                // one branch covers both terms when the LJ and real-space cutoffs agree (or the
+                // molecule is uncharged); otherwise each term has its own
+                auto pairs = [&](auto same_cutoffs) {
+                    constexpr bool kOne = decltype(same_cutoffs)::value;
+                    for (int e = 0; e < count; ++e) {
+                        const float4 P = ent.p[e];
+                        const int ty = ent.t[e];
+#pragma unroll
+                        for (int s = 0; s < NS; ++s) {
+                            const float4 cg = tabf[s][ty];
+                            const float qq = (float)a.charge[s] * P.w;
+#pragma unroll
+                            for (int k = 0; k < TPT; ++k) {
+                                const float dx = P.x - ox[k][s], dy = P.y - oy[k][s], dz = P.z - oz[k][s];
+                                const float r2 = dx * dx + dy * dy + dz * dz;
+                                auto lj_term = [&](float ir) {
+                                    const float ir2 = ir * ir;
+                                    const float ir6 = ir2 * ir2 * ir2;
+                                    float w = fmaf(cg.y, ir6, cg.z);
+                                    if constexpr (LFS) w = fmaf(r2 * ir, cg.w, w);
+                                    lj[k] += (double)fmaf(cg.x, ir6 * ir6, -w);
+                                };
+                                auto real_term = [&](float ir) {
+                                    const float u = (r2 * ir) * erfc_scale;
+                                    const float f = __builtin_amdgcn_fractf(u);
+                                    const float4 cf4 = erfc_tab[(int)u];
+                                    const float ez4 = fmaf(fmaf(fmaf(cf4.w, f, cf4.z), f, cf4.y), f, cf4.x);
+                                    real[k] += (double)(qq * (ez4 * ir));
+                                };
+                                if constexpr (kOne) {
+                                    if (r2 <= rc2) {
+                                        const float ir = __builtin_amdgcn_rsqf(r2);
+                                        lj_term(ir);
+                                        if constexpr (CHARGED) real_term(ir);
+                                    }
+                                } else {
+                                    const float ir = __builtin_amdgcn_rsqf(r2);
+                                    if (r2 <= rc2) lj_term(ir);
+                                    if (r2 <= rcr2) real_term(ir);
+                                }
+                            }
+                        }
+                    }
+                };
+                if (any) {
+                    if (!CHARGED || rc2 == rcr2)
+                        pairs(std::true_type{});
+                    else
+                        pairs(std::false_type{});
+                }
+            } else if (any) {
                 for (int e = 0; e < count; ++e) {
                     const R ex = ent.x[e], ey = ent.y[e], ez = ent.z[e];
                     const int ty = ent.t[e];
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
-                        if constexpr (kF32) {
-                            const float4 cg = tabf[s][ty];
 #pragma unroll
-                            for (int k = 0; k < TPT; ++k) {
-                                const float dx = ex - ox[k][s], dy = ey - oy[k][s], dz = ez - oz[k][s];
-                                const float r2 = dx * dx + dy * dy + dz * dz;
-                                const float ir = __builtin_amdgcn_rsqf(r2);
-                                const float r = r2 * ir;
-                                if (r2 <= rc2) {
-                                    const float ir2 = ir * ir;
-                                    const float ir6 = ir2 * ir2 * ir2;
-                                    const float w = fmaf(cg.y, ir6, fmaf(r - rc, cg.w, cg.z));
-                                    lj[k] += (double)fmaf(cg.x, ir6 * ir6, -w);
+                        for (int k = 0; k < TPT; ++k) {
+                            const double dx = ex - ox[k][s], dy = ey - oy[k][s], dz = ez - oz[k][s];
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            if (r2 <= rc2) {
+                                const double ir2 = 1.0 / r2;
+                                const double s2 = tab[0][s][ty] * ir2;
+                                const double s6 = s2 * s2 * s2;
+                                double v = tab[1][s][ty] * (s6 * s6 - s6);
+                                if (a.trunc == ALEATOR_LJ_SHIFTED) {
+                                    v -= tab[2][s][ty];
+                                } else if (a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED) {
+                                    v = v - tab[2][s][ty] - (sqrt(r2) - rc) * tab[3][s][ty];
                                 }
-                                if (CHARGED && r2 <= rcr2) {
-                                    const float u = r * erfc_scale;
-                                    const float f = __builtin_amdgcn_fractf(u);
-                                    const float4 cf4 = erfc_tab[(int)u];
-                                    const float ez4 = fmaf(fmaf(fmaf(cf4.w, f, cf4.z), f, cf4.y), f, cf4.x);
-                                    real[k] += (double)(((float)a.charge[s] * ent.q[e]) * (ez4 * ir));
-                                }
+                                lj[k] += v;
                             }
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < TPT; ++k) {
-                                const double dx = ex - ox[k][s], dy = ey - oy[k][s], dz = ez - oz[k][s];
-                                const double r2 = dx * dx + dy * dy + dz * dz;
-                                if (r2 <= rc2) {
-                                    const double ir2 = 1.0 / r2;
-                                    const double s2 = tab[0][s][ty] * ir2;
-                                    const double s6 = s2 * s2 * s2;
-                                    double v = tab[1][s][ty] * (s6 * s6 - s6);
-                                    if (a.trunc == ALEATOR_LJ_SHIFTED) {
-                                        v -= tab[2][s][ty];
-                                    } else if (a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED) {
-                                        v = v - tab[2][s][ty] - (sqrt(r2) - rc) * tab[3][s][ty];
-                                    }
-                                    lj[k] += v;
-                                }
-                                if (CHARGED && r2 <= rcr2) {
-                                    const double r = sqrt(r2);
-                                    real[k] += (a.charge[s] * ent.q[e]) * erfc_over_r<double>(alpha, r);
-                                }
+                            if (CHARGED && r2 <= rcr2) {
+                                const double r = sqrt(r2);
+                                real[k] += (a.charge[s] * ent.q[e]) * erfc_over_r<double>(alpha, r);
                             }
                         }
                     }
@@ -487,11 +533,8 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
                 for (int sh = 0; sh < 8; ++sh) {
                     if (!(mask & (1u << sh))) continue;
                     const double g0 = h0 - (sh & 1), g1 = h1 - ((sh >> 1) & 1), g2 = h2 - (sh >> 2);
-                    ent.x[w] = (R)(g0 * a.m[0] + g1 * a.m[3] + g2 * a.m[6]);
-                    ent.y[w] = (R)(g0 * a.m[1] + g1 * a.m[4] + g2 * a.m[7]);
-                    ent.z[w] = (R)(g0 * a.m[2] + g1 * a.m[5] + g2 * a.m[8]);
-                    ent.q[w] = (R)qa;
-                    ent.t[w] = ta;
+                    ent.put(w, g0 * a.m[0] + g1 * a.m[3] + g2 * a.m[6], g0 * a.m[1] + g1 * a.m[4] + g2 * a.m[7],
+                            g0 * a.m[2] + g1 * a.m[5] + g2 * a.m[8], qa, ta);
                     ++w;
                 }
             }
@@ -524,12 +567,12 @@ __global__ __launch_bounds__(1024) void sum_kernel(const double* parts, uint64_t
 // Trials per thread of the cube kernel: 1, or (for large batches) 4 for one site and 2 otherwise.
 inline int max_trials_per_thread(int ns) { return ns == 1 ? 4 : 2; }
 
-template <typename R, bool CH>
+template <typename R, bool CH, bool LFS>
 void launch_cells(const WidomArgs& a, const CellArgs& c, unsigned n_cells, bool wide, hipStream_t s) {
     switch (a.nsites * 2 + (wide ? 1 : 0)) {
-#define CASE(N, T)                                                                                           \
-    case 2 * N + (T > 1 ? 1 : 0):                                                                            \
-        hipLaunchKernelGGL((widom_cell_kernel<R, N, CH, T>), dim3(n_cells), dim3(kWThreads), 0, s, a, c); \
+#define CASE(N, T)                                                                                                \
+    case 2 * N + (T > 1 ? 1 : 0):                                                                                 \
+        hipLaunchKernelGGL((widom_cell_kernel<R, N, CH, LFS, T>), dim3(n_cells), dim3(kWThreads), 0, s, a, c); \
         return;
         CASE(1, 1) CASE(1, 4) CASE(2, 1) CASE(2, 2) CASE(3, 1) CASE(3, 2) CASE(4, 1) CASE(4, 2) CASE(5, 1) CASE(5, 2)
 #undef CASE
@@ -692,14 +735,17 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
         hipLaunchKernelGGL(widom_key_kernel, dim3(tb), dim3(256), 0, ctx.stream, a, cg, key, count);
         hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, ctx.stream, count, n_cells, start, cursor);
         hipLaunchKernelGGL(scatter_kernel, dim3(tb), dim3(256), 0, ctx.stream, key, req.n_trials, cursor, order);
+        const bool lfs = a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED;
         if (charged) {
-            if (fp64) launch_cells<double, true>(a, cg, n_cells, wide_cells, ctx.stream);
-            else launch_cells<float, true>(a, cg, n_cells, wide_cells, ctx.stream);
+            if (fp64) launch_cells<double, true, false>(a, cg, n_cells, wide_cells, ctx.stream);
+            else if (lfs) launch_cells<float, true, true>(a, cg, n_cells, wide_cells, ctx.stream);
+            else launch_cells<float, true, false>(a, cg, n_cells, wide_cells, ctx.stream);
             const size_t lds = (size_t)4 * 3 * (2 * fw.kmax + 1) * ns * sizeof(cplx);
             hipLaunchKernelGGL(widom_recip_kernel, dim3(rec_blocks), dim3(256), lds, ctx.stream, a);
         } else {
-            if (fp64) launch_cells<double, false>(a, cg, n_cells, wide_cells, ctx.stream);
-            else launch_cells<float, false>(a, cg, n_cells, wide_cells, ctx.stream);
+            if (fp64) launch_cells<double, false, false>(a, cg, n_cells, wide_cells, ctx.stream);
+            else if (lfs) launch_cells<float, false, true>(a, cg, n_cells, wide_cells, ctx.stream);
+            else launch_cells<float, false, false>(a, cg, n_cells, wide_cells, ctx.stream);
             hipLaunchKernelGGL(widom_boltz_kernel, dim3(pair_blocks), dim3(kWThreads), 0, ctx.stream, a);
         }
     } else if (charged) {
