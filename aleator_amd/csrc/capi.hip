@@ -446,6 +446,20 @@ aleator_status aleator_gpu_framework_create(aleator_ctx* ctx, const aleator_latt
                 kn4[4 * t + 3] = 0;
             }
             fw->d_kn = device_upload(kn4.data(), kn4.size(), s);
+            std::vector<int2> seg;
+            for (int t = 0; t < fw->nk_half; ++t) {
+                const int* n3 = &kn4[4 * t];
+                const int len = seg.empty() ? 0 : (seg.back().y >> 24);
+                const int* p3 = t > 0 ? &kn4[4 * (t - 1)] : nullptr;
+                if (seg.empty() || len == kSegLen || n3[0] != p3[0] || n3[1] != p3[1] || n3[2] != p3[2] + 1) {
+                    seg.push_back(make_int2(t, (n3[0] + fw->kmax) | (n3[1] + fw->kmax) << 8 | (n3[2] + fw->kmax) << 16 |
+                                                   1 << 24));
+                } else {
+                    seg.back().y += 1 << 24;
+                }
+            }
+            fw->n_kseg = (int)seg.size();
+            fw->d_kseg = device_upload(seg.data(), seg.size(), s);
             fw->d_kw = device_upload(fw->kset.weight.data(), (size_t)fw->nk_half, s);
             ALEATOR_HIP_CHECK(hipMalloc(&fw->d_sre, std::max(1, fw->nk_half) * sizeof(double)));
             ALEATOR_HIP_CHECK(hipMalloc(&fw->d_sim, std::max(1, fw->nk_half) * sizeof(double)));
@@ -468,7 +482,7 @@ aleator_status aleator_gpu_framework_destroy(aleator_framework* fw) {
         if (!fw) return;
         (void)hipStreamSynchronize(fw->ctx->stream);
         for (void* p : {(void*)fw->d_fx, (void*)fw->d_fy, (void*)fw->d_fz, (void*)fw->d_q, (void*)fw->d_type,
-                        (void*)fw->d_kn, (void*)fw->d_kw, (void*)fw->d_sre, (void*)fw->d_sim})
+                        (void*)fw->d_kn, (void*)fw->d_kw, (void*)fw->d_kseg, (void*)fw->d_sre, (void*)fw->d_sim})
             if (p) (void)hipFree(p);
         delete fw;
     });

@@ -60,6 +60,7 @@ constexpr int kMaxTypes = 32;       // distinct framework LJ species
 constexpr int kMaxGuests = 4;       // LJ tables per grid launch
 constexpr int kMaxSites = 8;        // sites of a Widom / proposal molecule
 constexpr int kMaxK = 16;           // kmax bound for the per-axis phase tables
+constexpr int kSegLen = 8;          // k-set run length (reciprocal Widom kernel)
 constexpr double kCoulomb = 167100.9468982874;  // ewald.hpp:62
 constexpr double kPi = 3.14159265358979323846;
 
@@ -125,6 +126,10 @@ struct Framework {
     int nk_half = 0;                 // first nk/2 reference terms (lexicographically n < 0)
     int32_t* d_kn = nullptr;         // [nk_half][4] (n0,n1,n2,pad)
     double* d_kw = nullptr;          // [nk_half]
+    // the half set as runs of consecutive n2 at fixed (n0, n1), at most kSegLen terms each:
+    // (first term index, (n0+K) | (n1+K) << 8 | (n2 of the first term + K) << 16 | length << 24)
+    int2* d_kseg = nullptr;
+    int n_kseg = 0;
     double *d_sre = nullptr, *d_sim = nullptr;  // framework S(k), half set
     std::vector<double> h_sre, h_sim;           // host copy of the same
     double self_energy = 0.0;

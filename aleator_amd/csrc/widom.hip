@@ -182,59 +182,100 @@ __global__ __launch_bounds__(kWThreads) void widom_pair_kernel(const WidomArgs a
     }
 }
 
-// One wave per trial: reciprocal + self + exclusion, completing dU.
-__global__ __launch_bounds__(256) void widom_recip_kernel(const WidomArgs a) {
-    extern __shared__ cplx wr_phase[];
-    __shared__ double site4[4][kMaxSites * 4];
-    __shared__ double red[4];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ns = a.nsites;
-    const int nkk = 2 * a.kmax + 1;
-    cplx* E = wr_phase + (size_t)wave * 3 * nkk * ns;
-    const uint64_t t = (uint64_t)blockIdx.x * 4 + wave;
+// Reciprocal + self + exclusion, completing dU, for RT trials per workgroup (RT in {4, 8, 16}):
+// the RG = 256 / RT consecutive threads of a trial split the runs of the half k-set between them
+// (run r to thread r mod RG, in order) and add their partial sums by a fixed shuffle tree.  Per
+// run and site, exp(i k.r) = E0[n0] E1[n1] E2[n2] is formed once from the trial's per-axis
+// factors in LDS and then stepped along n2 by one complex multiply per term.  The k data
+// (weight, S(k)) are read once per run by each trial's thread and shared through L1 by the
+// workgroup's trials.
+template <int NS>
+__global__ __launch_bounds__(256) void widom_recip_runs_kernel(const WidomArgs a, const int2* __restrict__ kseg,
+                                                               int n_seg, int rt) {
+    extern __shared__ cplx wr_phase[];  // [rt][NS][3][2K + 1]
+    __shared__ double s_f[16][NS][3];
+    __shared__ double s_excl[16];
+    __shared__ double red[16];
+    const int rg = 256 / rt;
+    const int j = threadIdx.x / rg, g = threadIdx.x % rg;
+    const int K = a.kmax, nkk = 2 * K + 1;
+    const uint64_t t = (uint64_t)blockIdx.x * rt + j;
     const bool valid = t < a.n_trials;
-    double cart[kMaxSites][3];
-    double excl = 0.0;
-    if (valid) {
+    if (g == 0 && valid) {
+        double cart[kMaxSites][3];
         trial_sites(a, t, cart);
-        if (lane == 0) {
-            for (int s = 0; s < ns; ++s) {
-                double f[3];
-                to_frac(a, cart[s], f);
-                site4[wave][4 * s] = f[0];
-                site4[wave][4 * s + 1] = f[1];
-                site4[wave][4 * s + 2] = f[2];
-                site4[wave][4 * s + 3] = a.charge[s];
-            }
-        }
+        for (int s = 0; s < NS; ++s) to_frac(a, cart[s], s_f[j][s]);
         // intramolecular exclusion: -kC sum_{i<j} q_i q_j erf(alpha r)/r
         // (ewald_incremental_state.cc:32-43); rigid-molecule distances are far below L_perp/2
         double sum = 0.0;
-        for (int i = 0; i < ns; ++i)
-            for (int j = i + 1; j < ns; ++j) {
-                const double dx = cart[j][0] - cart[i][0], dy = cart[j][1] - cart[i][1],
-                             dz = cart[j][2] - cart[i][2];
+        for (int i = 0; i < NS; ++i)
+            for (int k = i + 1; k < NS; ++k) {
+                const double dx = cart[k][0] - cart[i][0], dy = cart[k][1] - cart[i][1], dz = cart[k][2] - cart[i][2];
                 const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 const double eor = r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r;
-                sum += a.charge[i] * a.charge[j] * eor;
+                sum += a.charge[i] * a.charge[k] * eor;
             }
-        excl = -a.kc * sum;
+        s_excl[j] = -a.kc * sum;
     }
     __syncthreads();
-    if (valid) stage_phases(lane, ns, a.kmax, site4[wave], E);
+    // per-axis factors exp(2 pi i m f), m = -K..K: thread (s, axis) of the trial takes one
+    // sincospi and the powers by repeated multiplication (negative m by conjugation)
+    if (valid && g < 3 * NS) {
+        const int s = g / 3, ax = g % 3;
+        cplx* E = wr_phase + ((size_t)(j * NS + s) * 3 + ax) * nkk;
+        double sv, cv;
+        sincospi(2.0 * s_f[j][s][ax], &sv, &cv);
+        const cplx w = {cv, sv};
+        cplx p = {1.0, 0.0};
+        E[K] = p;
+        for (int m = 1; m <= K; ++m) {
+            p = cmul(p, w);
+            E[K + m] = p;
+            E[K - m] = {p.re, -p.im};
+        }
+    }
     __syncthreads();
-    double boltz = 0.0;
+    double acc = 0.0;
     if (valid) {
-        const double acc = wave_recip_delta(lane, ns, 0, a.kmax, E, site4[wave], a.kn, a.kw,
-                                            const_cast<double*>(a.sre), const_cast<double*>(a.sim),
-                                            a.nk_half, false);
-        const double du = a.du[t] + a.recip_scale * acc + a.self_delta + excl;
-        if (lane == 0) a.du[t] = du;
-        boltz = exp(-du / a.temperature);
+        for (int sg = g; sg < n_seg; sg += rg) {
+            const int2 d = kseg[sg];
+            const int i0 = d.y & 255, i1 = (d.y >> 8) & 255, i2 = (d.y >> 16) & 255, len = d.y >> 24;
+            cplx e[NS], w[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const cplx* E = wr_phase + (size_t)(j * NS + s) * 3 * nkk;
+                e[s] = cmul(cmul(E[i0], E[nkk + i1]), E[2 * nkk + i2]);
+                w[s] = E[2 * nkk + K + 1];
+            }
+            for (int i = 0; i < len; ++i) {
+                double dre = 0.0, dim = 0.0;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    dre += a.charge[s] * e[s].re;
+                    dim += a.charge[s] * e[s].im;
+                    e[s] = cmul(e[s], w[s]);
+                }
+                const int k = d.x + i;
+                acc += a.kw[k] * (2.0 * (a.sre[k] * dre + a.sim[k] * dim) + (dre * dre + dim * dim));
+            }
+        }
     }
-    if (lane == 0) red[wave] = valid ? boltz : 0.0;
+    for (int off = rg / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (g == 0) {
+        double boltz = 0.0;
+        if (valid) {
+            const double du = a.du[t] + a.recip_scale * acc + a.self_delta + s_excl[j];
+            a.du[t] = du;
+            boltz = exp(-du / a.temperature);
+        }
+        red[j] = boltz;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) a.block_sums[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int i = 0; i < rt; ++i) sum += red[i];
+        a.block_sums[blockIdx.x] = sum;
+    }
 }
 
 // ---------------------------------------------------------------- cell-sorted direct sum
@@ -591,6 +632,27 @@ void launch_pair(const WidomArgs& a, unsigned blocks, hipStream_t s) {
     fail(ALEATOR_INVALID_ARGUMENT, "widom: 1 to 5 sites supported");
 }
 
+// Trials per workgroup of the reciprocal kernel: 16, or fewer when their phase tables would
+// pass 64 KB of LDS.
+int recip_trials_per_block(int ns, int kmax) {
+    const size_t per_trial = (size_t)ns * 3 * (2 * kmax + 1) * sizeof(cplx);
+    return per_trial * 16 <= 65536 ? 16 : per_trial * 8 <= 65536 ? 8 : 4;
+}
+
+void launch_recip(const WidomArgs& a, const Framework& fw, unsigned blocks, int rt, hipStream_t s) {
+    const size_t lds = (size_t)rt * a.nsites * 3 * (2 * a.kmax + 1) * sizeof(cplx);
+    switch (a.nsites) {
+#define CASE(N)                                                                                                 \
+    case N:                                                                                                     \
+        hipLaunchKernelGGL((widom_recip_runs_kernel<N>), dim3(blocks), dim3(256), lds, s, a, fw.d_kseg, fw.n_kseg, \
+                           rt);                                                                                 \
+        return;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
+#undef CASE
+    }
+    fail(ALEATOR_INVALID_ARGUMENT, "widom: 1 to 5 sites supported");
+}
+
 }  // namespace
 
 void launch_widom(Framework& fw, const WidomRequest& req) {
@@ -703,7 +765,8 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
         }
     }
     const unsigned pair_blocks = (unsigned)((req.n_trials + kWThreads - 1) / kWThreads);
-    const unsigned rec_blocks = (unsigned)((req.n_trials + 3) / 4);
+    const int rec_rt = recip_trials_per_block(ns, fw.kmax);
+    const unsigned rec_blocks = (unsigned)((req.n_trials + rec_rt - 1) / rec_rt);
     const size_t tab_bytes = (tab.size() * sizeof(double) + 255) & ~size_t(255);
     const size_t parts = charged ? rec_blocks : pair_blocks;
     size_t du_bytes = req.du_device ? 0 : ((req.n_trials * sizeof(double) + 255) & ~size_t(255));
@@ -740,8 +803,7 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
             if (fp64) launch_cells<double, true, false>(a, cg, n_cells, wide_cells, ctx.stream);
             else if (lfs) launch_cells<float, true, true>(a, cg, n_cells, wide_cells, ctx.stream);
             else launch_cells<float, true, false>(a, cg, n_cells, wide_cells, ctx.stream);
-            const size_t lds = (size_t)4 * 3 * (2 * fw.kmax + 1) * ns * sizeof(cplx);
-            hipLaunchKernelGGL(widom_recip_kernel, dim3(rec_blocks), dim3(256), lds, ctx.stream, a);
+            launch_recip(a, fw, rec_blocks, rec_rt, ctx.stream);
         } else {
             if (fp64) launch_cells<double, false, false>(a, cg, n_cells, wide_cells, ctx.stream);
             else if (lfs) launch_cells<float, false, true>(a, cg, n_cells, wide_cells, ctx.stream);
@@ -751,8 +813,7 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
     } else if (charged) {
         if (fp64) launch_pair<double, true>(a, pair_blocks, ctx.stream);
         else launch_pair<float, true>(a, pair_blocks, ctx.stream);
-        const size_t lds = (size_t)4 * 3 * (2 * fw.kmax + 1) * ns * sizeof(cplx);
-        hipLaunchKernelGGL(widom_recip_kernel, dim3(rec_blocks), dim3(256), lds, ctx.stream, a);
+        launch_recip(a, fw, rec_blocks, rec_rt, ctx.stream);
     } else {
         if (fp64) launch_pair<double, false>(a, pair_blocks, ctx.stream);
         else launch_pair<float, false>(a, pair_blocks, ctx.stream);
