@@ -97,6 +97,44 @@ __global__ __launch_bounds__(256) void recip_delta_kernel(const double* sites, u
     if (lane == 0) out[p] = scale * acc;
 }
 
+// Batched proposals, RT per workgroup (RT = runs_molecules_per_block): the 256 / RT consecutive
+// threads of a proposal build its phase rows (one per site and axis) and split the k-set runs,
+// adding their partial sums by a fixed shuffle tree.  No commit: S(k) is only read.
+template <int NS>
+__global__ __launch_bounds__(256) void recip_delta_runs_kernel(const double* sites, uint64_t n_prop, int n_rem,
+                                                               const int2* kseg, int n_seg, const double* kw,
+                                                               const double* sre, const double* sim, int K,
+                                                               double scale, double* out, int rt) {
+    extern __shared__ cplx rr_phase[];  // [rt][NS][3][2K + 1]
+    const int rg = 256 / rt;
+    const int j = threadIdx.x / rg, g = threadIdx.x % rg;
+    const int nkk = 2 * K + 1;
+    const uint64_t p = (uint64_t)blockIdx.x * rt + j;
+    const bool valid = p < n_prop;
+    const double* site4 = sites + (valid ? p : 0) * NS * 4;
+    cplx* E = rr_phase + (size_t)j * NS * 3 * nkk;
+    if (valid && g < 3 * NS) phase_row(site4[4 * (g / 3) + g % 3], K, E + (size_t)g * nkk);
+    __syncthreads();
+    double acc = 0.0;
+    if (valid) {
+        double q[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) q[s] = s < n_rem ? -site4[4 * s + 3] : site4[4 * s + 3];
+        acc = runs_recip_sum<NS>(E, q, K, kseg, n_seg, g, rg, kw, sre, sim);
+    }
+    for (int off = rg / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (valid && g == 0) out[p] = scale * acc;
+}
+
+template <int NS>
+void launch_runs(Framework& fw, const double* sre, const double* sim, uint64_t n_prop, int n_rem,
+                 const double* d_sites, double* d_out, int rt, double scale) {
+    const size_t lds = (size_t)rt * NS * 3 * (2 * fw.kmax + 1) * sizeof(cplx);
+    const unsigned blocks = (unsigned)((n_prop + rt - 1) / rt);
+    hipLaunchKernelGGL((recip_delta_runs_kernel<NS>), dim3(blocks), dim3(256), lds, fw.ctx->stream, d_sites, n_prop,
+                       n_rem, fw.d_kseg, fw.n_kseg, fw.d_kw, sre, sim, fw.kmax, scale, d_out, rt);
+}
+
 }  // namespace
 
 void structure_factor(Framework& fw, uint64_t n, const double* d_fx, const double* d_fy,
@@ -129,10 +167,23 @@ void reciprocal_delta(Framework& fw, double* d_sre, double* d_sim, uint64_t n_pr
     const size_t lds = (size_t)4 * 3 * (2 * K + 1) * (ns > 0 ? ns : 1) * sizeof(cplx);
     const double scale = 2.0 * kCoulomb * (2.0 * kPi / std::abs(fw.lat.volume));
     const unsigned blocks = (unsigned)((n_prop + 3) / 4);
+    // many proposals, no commit: the run kernel (several proposals per workgroup)
+    const int rt = runs_molecules_per_block(ns, K);
     ctx.time_begin();
-    hipLaunchKernelGGL(recip_delta_kernel, dim3(blocks), dim3(256), lds, ctx.stream, d_sites, n_prop,
-                       n_rem, n_add, reinterpret_cast<const int4*>(fw.d_kn), fw.d_kw, d_sre, d_sim,
-                       fw.nk_half, K, scale, d_out, commit ? 1 : 0);
+    if (!commit && ns >= 1 && ns <= 6 && rt > 0 && n_prop >= 64) {
+        switch (ns) {
+            case 1: launch_runs<1>(fw, d_sre, d_sim, n_prop, n_rem, d_sites, d_out, rt, scale); break;
+            case 2: launch_runs<2>(fw, d_sre, d_sim, n_prop, n_rem, d_sites, d_out, rt, scale); break;
+            case 3: launch_runs<3>(fw, d_sre, d_sim, n_prop, n_rem, d_sites, d_out, rt, scale); break;
+            case 4: launch_runs<4>(fw, d_sre, d_sim, n_prop, n_rem, d_sites, d_out, rt, scale); break;
+            case 5: launch_runs<5>(fw, d_sre, d_sim, n_prop, n_rem, d_sites, d_out, rt, scale); break;
+            default: launch_runs<6>(fw, d_sre, d_sim, n_prop, n_rem, d_sites, d_out, rt, scale); break;
+        }
+    } else {
+        hipLaunchKernelGGL(recip_delta_kernel, dim3(blocks), dim3(256), lds, ctx.stream, d_sites, n_prop, n_rem,
+                           n_add, reinterpret_cast<const int4*>(fw.d_kn), fw.d_kw, d_sre, d_sim, fw.nk_half, K,
+                           scale, d_out, commit ? 1 : 0);
+    }
     ALEATOR_HIP_CHECK(hipGetLastError());
     ctx.time_end();
 }

@@ -63,4 +63,64 @@ __device__ inline double wave_recip_delta(int lane, int ns, int n_rem, int K, co
     return wave_sum(acc);
 }
 
+// exp(2 pi i m f) for m = -K..K into E[0..2K], from one sincospi and its powers (negative m
+// by conjugation); the powers carry at most K roundings.
+__device__ inline void phase_row(double f, int K, cplx* E) {
+    double sv, cv;
+    sincospi(2.0 * f, &sv, &cv);
+    const cplx w = {cv, sv};
+    cplx p = {1.0, 0.0};
+    E[K] = p;
+    for (int m = 1; m <= K; ++m) {
+        p = cmul(p, w);
+        E[K + m] = p;
+        E[K - m] = {p.re, -p.im};
+    }
+}
+
+// One molecule's share of sum_k w(k) (2 Re(conj(S) dS) + |dS|^2) over the half k-set, taken
+// as runs of consecutive n2 (see Framework::d_kseg): the caller's thread g of rg takes runs
+// g, g + rg, ... in order.  E: the molecule's per-axis rows, [NS][3][2K + 1]; q: signed charges
+// (removed sites negative).  Per run and site, exp(i k.r) = E0[n0] E1[n1] E2[n2] is formed once
+// and stepped along n2 by one complex multiply per term.
+template <int NS>
+__device__ inline double runs_recip_sum(const cplx* E, const double (&q)[NS], int K, const int2* __restrict__ kseg,
+                                        int n_seg, int g, int rg, const double* __restrict__ kw,
+                                        const double* __restrict__ sre, const double* __restrict__ sim) {
+    const int nkk = 2 * K + 1;
+    double acc = 0.0;
+    for (int sg = g; sg < n_seg; sg += rg) {
+        const int2 d = kseg[sg];
+        const int i0 = d.y & 255, i1 = (d.y >> 8) & 255, i2 = (d.y >> 16) & 255, len = d.y >> 24;
+        cplx e[NS], w[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const cplx* Es = E + (size_t)s * 3 * nkk;
+            e[s] = cmul(cmul(Es[i0], Es[nkk + i1]), Es[2 * nkk + i2]);
+            w[s] = Es[2 * nkk + K + 1];
+        }
+        for (int i = 0; i < len; ++i) {
+            double dre = 0.0, dim = 0.0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                dre += q[s] * e[s].re;
+                dim += q[s] * e[s].im;
+                e[s] = cmul(e[s], w[s]);
+            }
+            const int k = d.x + i;
+            acc += kw[k] * (2.0 * (sre[k] * dre + sim[k] * dim) + (dre * dre + dim * dim));
+        }
+    }
+    return acc;
+}
+
+// Molecules per workgroup of 256 threads for the run sum: 16, 8 or 4, so that each molecule's
+// 3 * ns phase rows have a thread each and the rows fit in 64 KB of LDS.
+inline int runs_molecules_per_block(int ns, int K) {
+    const size_t per = (size_t)ns * 3 * (2 * K + 1) * sizeof(cplx);
+    for (int rt : {16, 8, 4})
+        if (256 / rt >= 3 * ns && per * rt <= 65536) return rt;
+    return 0;  // the run kernels do not apply
+}
+
 }  // namespace aleator_gpu

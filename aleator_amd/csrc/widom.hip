@@ -218,47 +218,16 @@ __global__ __launch_bounds__(256) void widom_recip_runs_kernel(const WidomArgs a
         s_excl[j] = -a.kc * sum;
     }
     __syncthreads();
-    // per-axis factors exp(2 pi i m f), m = -K..K: thread (s, axis) of the trial takes one
-    // sincospi and the powers by repeated multiplication (negative m by conjugation)
-    if (valid && g < 3 * NS) {
-        const int s = g / 3, ax = g % 3;
-        cplx* E = wr_phase + ((size_t)(j * NS + s) * 3 + ax) * nkk;
-        double sv, cv;
-        sincospi(2.0 * s_f[j][s][ax], &sv, &cv);
-        const cplx w = {cv, sv};
-        cplx p = {1.0, 0.0};
-        E[K] = p;
-        for (int m = 1; m <= K; ++m) {
-            p = cmul(p, w);
-            E[K + m] = p;
-            E[K - m] = {p.re, -p.im};
-        }
-    }
+    // per-axis factors exp(2 pi i m f), m = -K..K: one thread per (site, axis) of the trial
+    if (valid && g < 3 * NS)
+        phase_row(s_f[j][g / 3][g % 3], K, wr_phase + ((size_t)(j * NS + g / 3) * 3 + g % 3) * nkk);
     __syncthreads();
     double acc = 0.0;
     if (valid) {
-        for (int sg = g; sg < n_seg; sg += rg) {
-            const int2 d = kseg[sg];
-            const int i0 = d.y & 255, i1 = (d.y >> 8) & 255, i2 = (d.y >> 16) & 255, len = d.y >> 24;
-            cplx e[NS], w[NS];
+        double q[NS];
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const cplx* E = wr_phase + (size_t)(j * NS + s) * 3 * nkk;
-                e[s] = cmul(cmul(E[i0], E[nkk + i1]), E[2 * nkk + i2]);
-                w[s] = E[2 * nkk + K + 1];
-            }
-            for (int i = 0; i < len; ++i) {
-                double dre = 0.0, dim = 0.0;
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    dre += a.charge[s] * e[s].re;
-                    dim += a.charge[s] * e[s].im;
-                    e[s] = cmul(e[s], w[s]);
-                }
-                const int k = d.x + i;
-                acc += a.kw[k] * (2.0 * (a.sre[k] * dre + a.sim[k] * dim) + (dre * dre + dim * dim));
-            }
-        }
+        for (int s = 0; s < NS; ++s) q[s] = a.charge[s];
+        acc = runs_recip_sum<NS>(wr_phase + (size_t)j * NS * 3 * nkk, q, K, kseg, n_seg, g, rg, a.kw, a.sre, a.sim);
     }
     for (int off = rg / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if (g == 0) {
@@ -632,13 +601,6 @@ void launch_pair(const WidomArgs& a, unsigned blocks, hipStream_t s) {
     fail(ALEATOR_INVALID_ARGUMENT, "widom: 1 to 5 sites supported");
 }
 
-// Trials per workgroup of the reciprocal kernel: 16, or fewer when their phase tables would
-// pass 64 KB of LDS.
-int recip_trials_per_block(int ns, int kmax) {
-    const size_t per_trial = (size_t)ns * 3 * (2 * kmax + 1) * sizeof(cplx);
-    return per_trial * 16 <= 65536 ? 16 : per_trial * 8 <= 65536 ? 8 : 4;
-}
-
 void launch_recip(const WidomArgs& a, const Framework& fw, unsigned blocks, int rt, hipStream_t s) {
     const size_t lds = (size_t)rt * a.nsites * 3 * (2 * a.kmax + 1) * sizeof(cplx);
     switch (a.nsites) {
@@ -765,7 +727,8 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
         }
     }
     const unsigned pair_blocks = (unsigned)((req.n_trials + kWThreads - 1) / kWThreads);
-    const int rec_rt = recip_trials_per_block(ns, fw.kmax);
+    const int rec_rt = charged ? runs_molecules_per_block(ns, fw.kmax) : 1;
+    if (rec_rt == 0) fail(ALEATOR_NOT_IMPLEMENTED, "widom: phase tables exceed the LDS budget");
     const unsigned rec_blocks = (unsigned)((req.n_trials + rec_rt - 1) / rec_rt);
     const size_t tab_bytes = (tab.size() * sizeof(double) + 255) & ~size_t(255);
     const size_t parts = charged ? rec_blocks : pair_blocks;
