@@ -246,9 +246,10 @@ LEGS = ("c1", "c0", "c2", "c3", "ewald", "mlp", "irmof1", "gcmc_grid", "c4")
 
 def ewald_proposal_rate(aa, ctx, fw, sync, n_prop=100_000):
     """EwaldIncrementalState proposals against the c1 framework's S(k) (half set 1102 k), batched
-    one wave per proposal: CO2 insertions (3 sites) and moves (3 removed + 3 added).  Algorithmic
-    work per proposal: sites x (Nk/2) x 16 FLOP (FP64); S(k) state read: (Nk/2) x 40 B
-    (k index 16 B, weight 8 B, S 16 B), from L2."""
+    (16 insertions or 8 moves per workgroup over the 183 runs of consecutive n2 in the k-set):
+    CO2 insertions (3 sites) and moves (3 removed + 3 added).  Algorithmic work per proposal:
+    sites x (Nk/2) x 16 FLOP (FP64); S(k) state read per proposal: (Nk/2) x 24 B (weight 8 B,
+    S 16 B), through L1/L2."""
     rng = np.random.default_rng(11)
     nk = fw.num_k // 2
     st = aa.EwaldState(fw)
@@ -272,7 +273,7 @@ def ewald_proposal_rate(aa, ctx, fw, sync, n_prop=100_000):
         sites = removed.shape[1] + added.shape[1]
         res[f"ewald_{name}_proposals_per_s"] = n_prop / t
         res[f"ewald_{name}_fp64_tflops"] = n_prop * sites * nk * 16 / t / 1e12
-        res[f"ewald_{name}_state_read_gb_per_s"] = n_prop * nk * 40 / t / 1e9
+        res[f"ewald_{name}_state_read_gb_per_s"] = n_prop * nk * 24 / t / 1e9
     st.close()
     return res
 

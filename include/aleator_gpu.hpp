@@ -5,6 +5,8 @@
 //   engines::FrameworkEnergyGrid and its cache (engines/monte_carlo/framework_energy_grid.hpp:23-220),
 //   forcefield::EwaldIncrementalState (forcefield/electrostatics/ewald_incremental_state.hpp:60-140),
 //   MonteCarloEngine::widomInsertionHenryCoefficient (engines/monte_carlo/monte_carlo_engine.hpp:198),
+//   MonteCarloEngine's GCMC run, batched over chains, with or without a FrameworkEnergyGrid
+//   (GcmcChains; engines/monte_carlo/monte_carlo_engine.hpp:129-151),
 // with the reference's exception types for its error cases (std::invalid_argument,
 // std::out_of_range, std::runtime_error; aleator::gpu::NotImplemented stands for
 // aleator::NotImplemented, core/exceptions.hpp:13-17).
@@ -229,6 +231,12 @@ public:
 
     [[nodiscard]] double spacingAngstrom() const noexcept { return spacing_; }
     [[nodiscard]] std::array<int, 3> pointCounts() const noexcept { return {counts_[0], counts_[1], counts_[2]}; }
+    /// Species with a table, ascending (no reference counterpart).
+    [[nodiscard]] std::vector<std::uint32_t> speciesIds() const {
+        std::vector<std::uint32_t> ids;
+        for (const auto& [s, t] : tables_) ids.push_back(s);
+        return ids;
+    }
     /// Node values [nz][ny][nx] of one table (no reference counterpart; the reference keeps them private).
     [[nodiscard]] const std::vector<double>& table(std::uint32_t guestSpeciesId) const {
         const auto it = tables_.find(guestSpeciesId);
@@ -318,6 +326,95 @@ FrameworkEnergyGrid loadOrBuildFrameworkEnergyGrid(Device& dev, const LJ& forceF
     grid.serialize(FrameworkEnergyGrid::cacheFilePath(request.cacheDirectory, key), key);
     return grid;
 }
+
+/// One chain of MonteCarloEngine's GCMC ensemble (monte_carlo_engine.hpp:143-151): fugacity in
+/// Pa, temperature, and the Philox4x32Rng (seed, stream) it draws from.
+struct ChainConfig {
+    double fugacityPascal = 0.0;
+    double temperatureKelvin = 0.0;
+    std::uint64_t seed = 0, stream = 0;
+};
+
+/// Many single-species GCMC chains of MonteCarloEngine::run (monte_carlo_engine.cc:405-671) on
+/// one device framework, advanced together; with a FrameworkEnergyGrid the framework LJ is the
+/// interpolated grid, as the engine's frameworkEnergyGrid argument makes it
+/// (monte_carlo_engine.hpp:129-151, monte_carlo_engine.cc:341-374).  Each chain follows the
+/// reference engine's trajectory for its (seed, stream).
+class GcmcChains {
+public:
+    template <class Site>
+    GcmcChains(const Framework& fw, const std::vector<Site>& species, const std::vector<ChainConfig>& chains,
+               double maxTranslationAngstrom = 1.5, double maxRotationRadians = 0.5, int maxMolecules = 1024,
+               const FrameworkEnergyGrid* frameworkEnergyGrid = nullptr)
+        : n_chains_((int)chains.size()), n_sites_((int)species.size()) {
+        std::vector<aleator_molecule_site> s;
+        for (const auto& v : species) s.push_back({v.x, v.y, v.z, v.ljSpecies, v.charge});
+        std::vector<aleator_gcmc_chain> c;
+        for (const auto& v : chains) c.push_back({v.fugacityPascal, v.temperatureKelvin, v.seed, v.stream});
+        check(aleator_gpu_gcmc_create(fw.handle(), s.data(), (int32_t)s.size(), c.data(), (int32_t)c.size(),
+                                      maxTranslationAngstrom, maxRotationRadians, maxMolecules, &g_));
+        try {
+            setEnergyGrid(frameworkEnergyGrid);
+        } catch (...) {
+            aleator_gpu_gcmc_destroy(g_);
+            throw;
+        }
+    }
+    ~GcmcChains() {
+        if (g_) aleator_gpu_gcmc_destroy(g_);
+    }
+    GcmcChains(const GcmcChains&) = delete;
+    GcmcChains& operator=(const GcmcChains&) = delete;
+
+    /// Framework LJ from `grid` from the next run on (std::out_of_range if a site's species has
+    /// no table); nullptr restores the direct sum.
+    void setEnergyGrid(const FrameworkEnergyGrid* grid) {
+        if (!grid) {
+            check(aleator_gpu_gcmc_set_energy_grid(g_, nullptr, nullptr, 0, nullptr));
+            return;
+        }
+        const auto ids = grid->speciesIds();
+        if (ids.empty()) throw std::out_of_range("FrameworkEnergyGrid: no tables");
+        std::vector<double> all;
+        for (auto id : ids) all.insert(all.end(), grid->table(id).begin(), grid->table(id).end());
+        const auto n = grid->pointCounts();
+        const int32_t counts[3] = {n[0], n[1], n[2]};
+        check(aleator_gpu_gcmc_set_energy_grid_host(g_, counts, ids.data(), (int32_t)ids.size(), all.data()));
+    }
+    /// Advances every chain by numSteps (MonteCarloEngine::run).
+    void run(std::uint64_t numSteps) { check(aleator_gpu_gcmc_run(g_, numSteps, nullptr)); }
+    /// Per chain: molecules now, steps and summed molecule count since the last reset, and the
+    /// attempted / accepted counts per move kind.
+    [[nodiscard]] std::vector<aleator_gcmc_stats> statistics() const {
+        std::vector<aleator_gcmc_stats> st(n_chains_);
+        check(aleator_gpu_gcmc_stats(g_, st.data()));
+        for (const auto& c : st)
+            if (c.error) throw std::runtime_error("GcmcChains: a chain exceeded its molecule capacity");
+        return st;
+    }
+    void resetStatistics() { check(aleator_gpu_gcmc_reset_stats(g_)); }
+    /// Site positions of one chain, molecule by molecule in the engine's order.
+    [[nodiscard]] std::vector<std::array<double, 3>> sitePositions(int chain) const {
+        int32_t n = 0;
+        check(aleator_gpu_gcmc_positions(g_, chain, nullptr, 0, &n));
+        std::vector<double> xyz((size_t)n * n_sites_ * 3);
+        check(aleator_gpu_gcmc_positions(g_, chain, xyz.data(), xyz.size(), &n));
+        std::vector<std::array<double, 3>> out(xyz.size() / 3);
+        for (size_t i = 0; i < out.size(); ++i) out[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        return out;
+    }
+    /// EwaldIncrementalState::committedEnergy of the chain (0 for an uncharged framework).
+    [[nodiscard]] double electrostaticEnergy(int chain) const {
+        double e = 0.0;
+        check(aleator_gpu_gcmc_electrostatic_energy(g_, chain, &e));
+        return e;
+    }
+    [[nodiscard]] aleator_gcmc* handle() const noexcept { return g_; }
+
+private:
+    aleator_gcmc* g_ = nullptr;
+    int n_chains_, n_sites_;
+};
 
 /// forcefield::EwaldIncrementalState (ewald_incremental_state.hpp:60-140): S(k) of the
 /// particles resident on the device; proposals never mutate, commits add the same delta.

@@ -252,6 +252,51 @@ void gpu_checks() {
         expect(raw.size() == oraw.size() && worst <= 1e-9, "rawStateForTesting equals the oracle's layout and values");
         orc_ewald_state_free(&ost);
     }
+    // ---- GcmcChains with a FrameworkEnergyGrid against the oracle's grid mode
+    {
+        System s = make_system(14.0, 40, false, 6.5);
+        s.lj.trunc = ORC_SHIFTED;
+        ag::Framework fw(dev, s.lj, s.p, s.lat);
+        ag::FrameworkEnergyGrid grid(dev, s.lj, s.p, s.lat, {4}, 0.5);
+        const std::vector<MoleculeSite> ch4 = {{0, 0, 0, 4, 0.0, 16.04}};
+        const std::vector<ag::ChainConfig> cfg = {{1e8, 300.0, 5, 0}, {3e7, 300.0, 6, 2}};
+        ag::GcmcChains chains(fw, ch4, cfg, 1.5, 0.5, 256, &grid);
+        chains.run(2000);
+        const auto st = chains.statistics();
+        orc_lattice L = olat(s.lat);
+        std::vector<double> e, g;
+        const orc_lj lj = olj(s, e, g);
+        orc_particles p = opart(s);
+        const orc_mol_site site = {0, 0, 0, 4, 0.0};
+        const auto n = grid.pointCounts();
+        const int32_t counts[3] = {n[0], n[1], n[2]};
+        const uint32_t species = 4;
+        bool same = true;
+        for (size_t c = 0; c < cfg.size(); ++c) {
+            orc_gcmc og;
+            orc_gcmc_init(&og, &lj, &p, &L, &site, 1, cfg[c].fugacityPascal, cfg[c].temperatureKelvin, 1.5, 0.5, 1.0,
+                          1.0, 1, 0, cfg[c].seed, cfg[c].stream);
+            orc_gcmc_set_grid(&og, counts, &species, 1, grid.table(4).data());
+            std::vector<double> trace(2000);
+            orc_gcmc_run(&og, 2000, trace.data());
+            double sum = 0.0;
+            for (double v : trace) sum += v;
+            const auto pos = chains.sitePositions((int)c);
+            same &= st[c].n_molecules == (int)og.n_mol && pos.size() == og.n_mol && st[c].sum_molecules == sum &&
+                    og.n_mol > 0;
+            for (size_t m = 0; m < og.n_mol && same; ++m) {
+                const size_t i = og.mol[m];
+                same &= std::abs(pos[m][0] - og.x[i]) <= 1e-9 && std::abs(pos[m][1] - og.y[i]) <= 1e-9 &&
+                        std::abs(pos[m][2] - og.z[i]) <= 1e-9;
+            }
+            for (int k = 0; k < 4; ++k) same &= st[c].attempted[k] == og.attempted[k] && st[c].accepted[k] == og.accepted[k];
+            orc_gcmc_free(&og);
+        }
+        expect(same, "GcmcChains with a FrameworkEnergyGrid follows the oracle's grid-mode trajectory");
+        ag::FrameworkEnergyGrid other(dev, s.lj, s.p, s.lat, {5}, 0.5);
+        expect(throws<std::out_of_range>([&] { ag::GcmcChains(fw, ch4, cfg, 1.5, 0.5, 256, &other); }),
+               "GcmcChains: grid without the site's species -> out_of_range");
+    }
     // ---- Widom K_H against the oracle
     {
         System s = make_system(25.83, 424, false, 12.8);
