@@ -319,6 +319,52 @@ __global__ void scatter_kernel(const uint32_t* key, uint64_t n, uint32_t* cursor
     order[atomicAdd(&cursor[key[t]], 1u)] = (uint32_t)t;
 }
 
+// Counting sort of the trials by cube with no global atomics (cube counts up to kSortMaxCells):
+// block b takes trials [b per, (b + 1) per) and histograms their cubes in LDS; a column scan
+// over blocks gives each (block, cube) its offset among the cube's trials; the block then places
+// its trials, ranked inside (block, cube) by LDS atomics.
+constexpr int kSortThreads = 1024;
+constexpr uint32_t kSortMaxCells = 12288;  // 48 KB of LDS counters
+
+__global__ __launch_bounds__(kSortThreads) void cube_hist_kernel(const WidomArgs a, const CellArgs c, uint64_t per,
+                                                                 uint32_t n_cells, uint32_t* hist) {
+    extern __shared__ uint32_t hcount[];
+    for (uint32_t i = threadIdx.x; i < n_cells; i += kSortThreads) hcount[i] = 0u;
+    __syncthreads();
+    const uint64_t lo = (uint64_t)blockIdx.x * per, hi = min(lo + per, a.n_trials);
+    for (uint64_t t = lo + threadIdx.x; t < hi; t += kSortThreads) atomicAdd(&hcount[trial_cell(a, c, t)], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n_cells; i += kSortThreads) hist[(size_t)blockIdx.x * n_cells + i] = hcount[i];
+}
+
+// hist[b][cube] <- trials of the cube in blocks before b; count[cube] <- its total
+__global__ void cube_colscan_kernel(uint32_t* hist, uint32_t n_blocks, uint32_t n_cells, uint32_t* count) {
+    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= n_cells) return;
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < n_blocks; ++b) {
+        const size_t i = (size_t)b * n_cells + cell;
+        const uint32_t v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+    count[cell] = run;
+}
+
+__global__ __launch_bounds__(kSortThreads) void cube_place_kernel(const WidomArgs a, const CellArgs c, uint64_t per,
+                                                                  uint32_t n_cells, const uint32_t* hist,
+                                                                  const uint32_t* start, uint32_t* order) {
+    extern __shared__ uint32_t hrank[];
+    for (uint32_t i = threadIdx.x; i < n_cells; i += kSortThreads) hrank[i] = 0u;
+    __syncthreads();
+    const uint32_t* base = hist + (size_t)blockIdx.x * n_cells;
+    const uint64_t lo = (uint64_t)blockIdx.x * per, hi = min(lo + per, a.n_trials);
+    for (uint64_t t = lo + threadIdx.x; t < hi; t += kSortThreads) {
+        const uint32_t k = trial_cell(a, c, t);
+        order[start[k] + base[k] + atomicAdd(&hrank[k], 1u)] = (uint32_t)t;
+    }
+}
+
 template <typename R>
 struct WEntries {
     R x[kWCap], y[kWCap], z[kWCap], q[kWCap];
@@ -733,8 +779,13 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
     const size_t tab_bytes = (tab.size() * sizeof(double) + 255) & ~size_t(255);
     const size_t parts = charged ? rec_blocks : pair_blocks;
     size_t du_bytes = req.du_device ? 0 : ((req.n_trials * sizeof(double) + 255) & ~size_t(255));
+    // LDS-histogram sort: about 4096 trials per block, at most 256 blocks
+    const bool lds_sort = n_cells > 0 && n_cells <= kSortMaxCells;
+    const uint32_t sort_blocks = lds_sort ? (uint32_t)std::min<uint64_t>(256, (req.n_trials + 4095) / 4096) : 0;
     const size_t sort_bytes =
-        n_cells ? ((2 * req.n_trials + 3 * (size_t)n_cells + 1) * sizeof(uint32_t) + 255) & ~size_t(255) : 0;
+        n_cells ? ((2 * req.n_trials + 3 * (size_t)n_cells + 1 + (size_t)sort_blocks * n_cells) * sizeof(uint32_t) +
+                   255) & ~size_t(255)
+                : 0;
     char* base = static_cast<char*>(ctx.ensure_scratch(tab_bytes + parts * sizeof(double) + du_bytes + sort_bytes));
     ALEATOR_HIP_CHECK(hipMemcpyAsync(base, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
                                      ctx.stream));
@@ -756,11 +807,24 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
         cg.start = start;
         cg.order = order;
         const unsigned tb = (unsigned)((req.n_trials + 255) / 256);
-        hipLaunchKernelGGL(fill_u32_kernel, dim3((n_cells + 255) / 256), dim3(256), 0, ctx.stream, count,
-                           (uint64_t)n_cells);
-        hipLaunchKernelGGL(widom_key_kernel, dim3(tb), dim3(256), 0, ctx.stream, a, cg, key, count);
-        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, ctx.stream, count, n_cells, start, cursor);
-        hipLaunchKernelGGL(scatter_kernel, dim3(tb), dim3(256), 0, ctx.stream, key, req.n_trials, cursor, order);
+        if (lds_sort) {
+            uint32_t* hist = cursor + n_cells;
+            const uint64_t per = (req.n_trials + sort_blocks - 1) / sort_blocks;
+            const size_t lds = (size_t)n_cells * sizeof(uint32_t);
+            hipLaunchKernelGGL(cube_hist_kernel, dim3(sort_blocks), dim3(kSortThreads), lds, ctx.stream, a, cg, per,
+                               n_cells, hist);
+            hipLaunchKernelGGL(cube_colscan_kernel, dim3((n_cells + 255) / 256), dim3(256), 0, ctx.stream, hist,
+                               sort_blocks, n_cells, count);
+            hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, ctx.stream, count, n_cells, start, cursor);
+            hipLaunchKernelGGL(cube_place_kernel, dim3(sort_blocks), dim3(kSortThreads), lds, ctx.stream, a, cg, per,
+                               n_cells, hist, start, order);
+        } else {
+            hipLaunchKernelGGL(fill_u32_kernel, dim3((n_cells + 255) / 256), dim3(256), 0, ctx.stream, count,
+                               (uint64_t)n_cells);
+            hipLaunchKernelGGL(widom_key_kernel, dim3(tb), dim3(256), 0, ctx.stream, a, cg, key, count);
+            hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, ctx.stream, count, n_cells, start, cursor);
+            hipLaunchKernelGGL(scatter_kernel, dim3(tb), dim3(256), 0, ctx.stream, key, req.n_trials, cursor, order);
+        }
         const bool lfs = a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED;
         if (charged) {
             if (fp64) launch_cells<double, true, false>(a, cg, n_cells, wide_cells, ctx.stream);

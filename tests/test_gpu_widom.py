@@ -98,6 +98,26 @@ def test_widom_cell_path_matches_the_oracle(ctx):
     assert np.median(err) <= 1e-6
 
 
+def test_widom_cube_sort_paths_agree_bitwise(ctx):
+    """The trials are grouped by cube by per-block LDS histograms when the cube grid has at most
+    12288 cubes (1e7 trials on c0: 22^3) and by global atomics above that (2e7 trials: 27^3).
+    A trial's dU is its atom-order sum over the images within the cutoff whatever cube holds it;
+    only the rounding of the cube-relative coordinates differs (amplified near the r^-12 wall),
+    so the first 1e7 trials of both batches agree within the FP64 parity bar (a misplaced trial
+    would differ by orders of magnitude), and both equal the oracle's first trials."""
+    print("widom c0: 1e7 and 2e7 trials, FP64", flush=True)
+    fw, lj = ol.c0_system()
+    dev = device_framework(ctx, fw, lj)
+    _, a = dev.widom(CH4, 10_000_000, 298.0, seed=5, stream=1, precision=aa.FP64)
+    _, b = dev.widom(CH4, 20_000_000, 298.0, seed=5, stream=1, precision=aa.FP64)
+    err = rel_err(b[:10_000_000], a)
+    print(f"  1e7 vs 2e7 batches, first 1e7 trials: max rel {err.max():.3e}, bitwise equal {np.mean(err == 0):.4f}")
+    assert err.max() <= 1e-10
+    _, du_ref = ol.widom(lj, fw, CH4, seed=5, stream=1, n_trials=100, temperature=298.0)
+    assert rel_err(a[:100], du_ref).max() <= 1e-10
+    assert rel_err(b[:100], du_ref).max() <= 1e-10
+
+
 def test_widom_cell_path_charged_c1(ctx):
     """Charged CO2 on the c1 supercell (4800 atoms, Ewald alpha 0.265, kmax 8): a 1e5-trial batch
     runs the cube path; its first 40 trials equal the oracle's dU."""
