@@ -51,7 +51,7 @@ template <typename R>
 struct Entries {
     R x[kCap], y[kCap], z[kCap], q[kCap];
     int t[kCap];
-    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_) {
+    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_, bool) {
         x[w] = (R)x_;
         y[w] = (R)y_;
         z[w] = (R)z_;
@@ -63,8 +63,9 @@ template <>
 struct Entries<float> {
     float4 p[kCap];
     int t[kCap];
-    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_) {
-        p[w] = make_float4((float)x_, (float)y_, (float)z_, (float)q_);
+    // with_charge false: the fourth word carries the type bits instead of the charge
+    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_, bool with_charge) {
+        p[w] = make_float4((float)x_, (float)y_, (float)z_, with_charge ? (float)q_ : __int_as_float(t_));
         t[w] = t_;
     }
 };
@@ -208,7 +209,8 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
 #pragma unroll 2
                 for (int e = 0; e < count; ++e) {
                     const float4 P = ent.p[e];
-                    const int t = ent.t[e];
+                    // without a Coulomb table the entry's fourth word holds its type (one LDS read)
+                    const int t = COUL ? ent.t[e] : __float_as_int(P.w);
                     float4 cg[NG > 0 ? NG : 1];
 #pragma unroll
                     for (int g = 0; g < NG; ++g) cg[g] = tabf[g][t];
@@ -220,14 +222,14 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
 #pragma unroll
                     for (int p = 0; p < NPT; ++p) {
                         const float r2 = p == 0 ? r20 : fmaf(tds, m2p[p], r20 + s2p[p]);
-                        auto lj = [&](float ir) {
-                            const float ir2 = ir * ir;
+                        // LJ from 1/r^2 (and r, linear-force-shifted only)
+                        auto lj = [&](float ir2, float r) {
                             const float ir6 = ir2 * ir2 * ir2;
                             const float ir12 = ir6 * ir6;
 #pragma unroll
                             for (int g = 0; g < NG; ++g) {
                                 float w = fmaf(cg[g].y, ir6, cg[g].z);
-                                if constexpr (LFS) w = fmaf(r2 * ir, cg[g].w, w);
+                                if constexpr (LFS) w = fmaf(r, cg[g].w, w);
                                 acc[p][g] += (double)fmaf(cg[g].x, ir12, -w);
                             }
                         };
@@ -242,13 +244,17 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
                         };
                         if constexpr (kOne) {
                             if (r2 <= (NG > 0 ? rc2 : rcr2)) {
-                                const float ir = __builtin_amdgcn_rsqf(r2);
-                                if constexpr (NG > 0) lj(ir);
-                                if constexpr (COUL) coul(ir);
+                                if constexpr (!COUL && !LFS) {
+                                    lj(__builtin_amdgcn_rcpf(r2), 0.0f);  // LJ alone needs no 1/r
+                                } else {
+                                    const float ir = __builtin_amdgcn_rsqf(r2);
+                                    if constexpr (NG > 0) lj(ir * ir, r2 * ir);
+                                    if constexpr (COUL) coul(ir);
+                                }
                             }
                         } else {
                             const float ir = __builtin_amdgcn_rsqf(r2);
-                            if (r2 <= rc2) lj(ir);
+                            if (r2 <= rc2) lj(ir * ir, r2 * ir);
                             if (r2 <= rcr2) coul(ir);
                         }
                     }
@@ -358,7 +364,7 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
                 if (!(mask & (1u << s))) continue;
                 const double g0 = h0 - (s & 1), g1 = h1 - ((s >> 1) & 1), g2 = h2 - (s >> 2);
                 ent.put(w, g0 * a.mi[0] + g1 * a.mi[3] + g2 * a.mi[6], g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7],
-                        g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8], qa, ta);
+                        g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8], qa, ta, COUL);
                 ++w;
             }
         }

@@ -369,7 +369,7 @@ template <typename R>
 struct WEntries {
     R x[kWCap], y[kWCap], z[kWCap], q[kWCap];
     int t[kWCap];
-    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_) {
+    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_, bool) {
         x[w] = (R)x_;
         y[w] = (R)y_;
         z[w] = (R)z_;
@@ -377,13 +377,14 @@ struct WEntries {
         t[w] = t_;
     }
 };
-// FP32: (x, y, z, q) packed in one float4, so an entry is two LDS reads
+// FP32: (x, y, z, q) packed in one float4, so an entry is two LDS reads; with_charge false: the
+// fourth word carries the type bits instead of the charge, and an entry is one LDS read
 template <>
 struct WEntries<float> {
     float4 p[kWCap];
     int t[kWCap];
-    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_) {
-        p[w] = make_float4((float)x_, (float)y_, (float)z_, (float)q_);
+    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_, bool with_charge) {
+        p[w] = make_float4((float)x_, (float)y_, (float)z_, with_charge ? (float)q_ : __int_as_float(t_));
         t[w] = t_;
     }
 };
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
                     constexpr bool kOne = decltype(same_cutoffs)::value;
                     for (int e = 0; e < count; ++e) {
                         const float4 P = ent.p[e];
-                        const int ty = ent.t[e];
+                        const int ty = CHARGED ? ent.t[e] : __float_as_int(P.w);
 #pragma unroll
                         for (int s = 0; s < NS; ++s) {
                             const float4 cg = tabf[s][ty];
@@ -482,11 +483,11 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
                             for (int k = 0; k < TPT; ++k) {
                                 const float dx = P.x - ox[k][s], dy = P.y - oy[k][s], dz = P.z - oz[k][s];
                                 const float r2 = dx * dx + dy * dy + dz * dz;
-                                auto lj_term = [&](float ir) {
-                                    const float ir2 = ir * ir;
+                                // LJ from 1/r^2 (and r, linear-force-shifted only)
+                                auto lj_term = [&](float ir2, float r) {
                                     const float ir6 = ir2 * ir2 * ir2;
                                     float w = fmaf(cg.y, ir6, cg.z);
-                                    if constexpr (LFS) w = fmaf(r2 * ir, cg.w, w);
+                                    if constexpr (LFS) w = fmaf(r, cg.w, w);
                                     lj[k] += (double)fmaf(cg.x, ir6 * ir6, -w);
                                 };
                                 auto real_term = [&](float ir) {
@@ -498,13 +499,17 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
                                 };
                                 if constexpr (kOne) {
                                     if (r2 <= rc2) {
-                                        const float ir = __builtin_amdgcn_rsqf(r2);
-                                        lj_term(ir);
-                                        if constexpr (CHARGED) real_term(ir);
+                                        if constexpr (!CHARGED && !LFS) {
+                                            lj_term(__builtin_amdgcn_rcpf(r2), 0.0f);  // LJ alone needs no 1/r
+                                        } else {
+                                            const float ir = __builtin_amdgcn_rsqf(r2);
+                                            lj_term(ir * ir, r2 * ir);
+                                            if constexpr (CHARGED) real_term(ir);
+                                        }
                                     }
                                 } else {
                                     const float ir = __builtin_amdgcn_rsqf(r2);
-                                    if (r2 <= rc2) lj_term(ir);
+                                    if (r2 <= rc2) lj_term(ir * ir, r2 * ir);
                                     if (r2 <= rcr2) real_term(ir);
                                 }
                             }
@@ -590,7 +595,7 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
                     if (!(mask & (1u << sh))) continue;
                     const double g0 = h0 - (sh & 1), g1 = h1 - ((sh >> 1) & 1), g2 = h2 - (sh >> 2);
                     ent.put(w, g0 * a.m[0] + g1 * a.m[3] + g2 * a.m[6], g0 * a.m[1] + g1 * a.m[4] + g2 * a.m[7],
-                            g0 * a.m[2] + g1 * a.m[5] + g2 * a.m[8], qa, ta);
+                            g0 * a.m[2] + g1 * a.m[5] + g2 * a.m[8], qa, ta, CHARGED);
                     ++w;
                 }
             }
