@@ -234,6 +234,52 @@ def cpu_baseline_c1(w, sample_nodes=1200, seed=3, threads=1):
                       f"{os.cpu_count()} logical cores on host"}
 
 
+def widom_cpu_reference(w, c0, threads):
+    """The reference engine's widomInsertionHenryCoefficient (compiled in oracle/_ref) on the
+    host: c0 CH4 (2e4 trials per thread) and c1 CO2 with Ewald (150 trials per thread), on one
+    thread and on `threads` threads, each thread on its own Philox stream (the same work per
+    trial; the calls release the interpreter lock).  None without oracle/_ref."""
+    import threading
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes as C
+    import oracle_lib as ol
+    from oracle_lib import ptr, u32p
+
+    if not ol.have_ref():
+        return None
+    cases = {
+        "c0_ch4": (c0["m"], c0["x"], c0["y"], c0["z"], np.zeros(len(c0["x"])), c0["sp"], c0["eps"], c0["sig"],
+                   c0["rc"], [(0.0, 0.0, 0.0, 4, 0.0)], 0, 20000),
+        "c1_co2": (w["sup"], w["x"], w["y"], w["z"], w["q"], w["sp"], w["eps"], w["sig"], w["rc"],
+                   [(0.0, 0.0, -1.16, 5, -0.35), (0.0, 0.0, 0.0, 4, 0.70), (0.0, 0.0, 1.16, 5, -0.35)], 1, 150),
+    }
+    out = {}
+    for name, (m, x, y, z, q, sp, eps, sig, rc, sites, charged, n) in cases.items():
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (m, x, y, z, q, eps, sig)]
+        spa = np.ascontiguousarray(sp, dtype=np.uint32)
+        sx = np.array([s_[:3] for s_ in sites], dtype=np.float64).ravel()
+        ss = np.array([s_[3] for s_ in sites], dtype=np.uint32)
+        sq = np.array([s_[4] for s_ in sites], dtype=np.float64)
+
+        def one(stream):
+            kh = C.c_double()
+            ol.ref().ref_widom(C.c_uint32(len(eps)), ptr(arrs[5]), ptr(arrs[6]), C.c_double(rc), 0, 0, ptr(arrs[0]),
+                               C.c_uint64(len(arrs[1])), ptr(arrs[1]), ptr(arrs[2]), ptr(arrs[3]), ptr(arrs[4]),
+                               ptr(spa, u32p), len(sites), ptr(sx), ptr(ss, u32p), ptr(sq), charged,
+                               C.c_double(w["alpha"]), C.c_double(rc), w["kmax"], C.c_uint64(1), C.c_uint64(stream),
+                               C.c_uint64(n), C.c_double(298.0), C.byref(kh))
+
+        for nt in (1, threads):
+            pool = [threading.Thread(target=one, args=(k + 1,)) for k in range(nt)]
+            t0 = time.perf_counter()
+            for t in pool:
+                t.start()
+            for t in pool:
+                t.join()
+            out[f"cpu_reference_widom_{name}_{nt}_thread_inserts_per_s"] = nt * n / (time.perf_counter() - t0)
+    return out
+
+
 def host_threads() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -652,6 +698,17 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline and not args.headline_only:
+        if "c0" in legs:
+            ref_w = widom_cpu_reference(w, c0_workload(aa), host_threads())
+            if ref_w:
+                extra.update(ref_w)
+                th = host_threads()
+                extra["widom_c0_speedup_vs_cpu_reference_all_threads"] = (
+                    extra["c0_widom_ch4_10000000_mixed_inserts_per_s"] /
+                    ref_w[f"cpu_reference_widom_c0_ch4_{th}_thread_inserts_per_s"])
+                extra["widom_c1_speedup_vs_cpu_reference_all_threads"] = (
+                    extra[f"c1_widom_co2_1000000_{args.precision}_inserts_per_s"] /
+                    ref_w[f"cpu_reference_widom_c1_co2_{th}_thread_inserts_per_s"])
         one = cpu_baseline_c1(w)
         extra["cpu_baseline_1_thread"] = one
         extra["speedup_vs_cpu_baseline_1_thread"] = value / one["value"]
