@@ -634,6 +634,10 @@ def main():
             t = timed(plan2.execute, 3, 1, sync)
             extra["c2_batched_256_frameworks_pts_per_s"] = sum(nodes2) / t
             extra["c2_nodes"] = sum(nodes2)
+            # algorithmic work: sum over frameworks of nodes x atoms within rc (density x sphere) x 20 FLOP
+            work2 = sum(n_f * len(f["x"]) / abs(np.linalg.det(f["sup"])) * (4.0 / 3.0) * np.pi * f["rc"] ** 3 * F_LJ
+                        for n_f, f in zip(nodes2, fws2))
+            extra["c2_tflops"] = work2 / t / 1e12
             extra["c2_ms_per_batch"] = t * 1e3
             extra["c2_tile_shape"] = plan2.tile()[0]
             plan2.close()
