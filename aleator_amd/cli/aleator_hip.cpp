@@ -573,7 +573,9 @@ int gcmc_run(const fs::path& path, bool dry_run, bool json, int max_molecules) {
         aleator_gcmc* g;
         ~Free() { aleator_gpu_gcmc_destroy(g); }
     } free_g{g};
-    if (!grid_tables.empty())
+    // a cached grid without the adsorbate's table fails here (the reference's interpolate throws
+    // std::out_of_range at the first step)
+    if (cfg.gcmc.grid_spacing)
         ag::check(aleator_gpu_gcmc_set_energy_grid_host(g, grid_n, grid_species.data(), (std::int32_t)grid_species.size(),
                                                         grid_tables.data()));
     aleator_gcmc_stats st{};
