@@ -568,6 +568,12 @@ class GridPlan:
         _check(_load().aleator_gpu_grid_plan_tile(self._h, C.byref(t), ms))
         return t.value, list(ms)
 
+    def atoms_per_pass(self) -> int:
+        """Framework atoms staged per pass of the LDS list build, tuned with the tile shape."""
+        n = C.c_int32()
+        _check(_load().aleator_gpu_grid_plan_atoms_per_pass(self._h, C.byref(n)))
+        return n.value
+
     def close(self):
         if self._h:
             _load().aleator_gpu_grid_plan_destroy(self._h)

@@ -42,7 +42,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kCap = 1024;   // LDS list capacity (entries)
-constexpr int kChunk = 128;  // atoms staged per pass (8 images each <= kCap)
+// atoms staged per pass of the list build: 128 (each may add 8 images, 1024 <= kCap), or, when
+// only the nearest image can be within the search radius, 256 or 512 (1 or 2 atoms per thread);
+// the plan's autotuner picks it with the tile shape
+constexpr int kChunkSmall = 128;
 constexpr int kErfcIntervals = 256;  // FP32 erfc(alpha r) table: cubic Hermite pieces over [0, alpha rc]
 
 // LDS image list: position, charge and compact LJ type of each atom image.  FP64 keeps
@@ -312,15 +315,17 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
 
     int pending = 0;
     const int natoms = a.natoms;
-    for (int base = 0; base < natoms; base += kChunk) {
-        const int at = base + tid;
+    const int chunk = a.chunk;
+    const int apt = chunk > kThreads ? 2 : 1;  // atoms per thread and pass (contiguous, so the
+                                               // list stays in atom order for every chunk)
+    for (int base = 0; base < natoms; base += chunk) {
         int cnt = 0;
-        unsigned mask = 0;
-        double h0 = 0, h1 = 0, h2 = 0;
-        if (tid < kChunk && at < natoms) {
-            h0 = a.fx[at] - ci0;
-            h1 = a.fy[at] - ci1;
-            h2 = a.fz[at] - ci2;
+        unsigned mask[2] = {0u, 0u};  // images of each atom within the search radius
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int at = base + tid * apt + j;
+            if (j >= apt || tid * apt >= chunk || at >= natoms) continue;
+            double h0 = a.fx[at] - ci0, h1 = a.fy[at] - ci1, h2 = a.fz[at] - ci2;
             h0 -= floor(h0);
             h1 -= floor(h1);
             h2 -= floor(h2);
@@ -333,8 +338,8 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
                 const double cy = g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7];
                 const double cz = g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8];
                 if (cx * cx + cy * cy + cz * cz <= a.search2) {
-                    mask = 1u << s;
-                    cnt = 1;
+                    mask[j] = 1u << s;
+                    ++cnt;
                 }
             } else {
 #pragma unroll
@@ -344,7 +349,7 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
                     const double cy = g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7];
                     const double cz = g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8];
                     if (cx * cx + cy * cy + cz * cz <= a.search2) {
-                        mask |= 1u << s;
+                        mask[j] |= 1u << s;
                         ++cnt;
                     }
                 }
@@ -356,12 +361,20 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
             compute(pending);
             pending = 0;
         }
-        if (mask) {
-            int w = pending + off;
+        int w = pending + off;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (!mask[j]) continue;
+            const int at = base + tid * apt + j;
             const double qa = a.q[at];
             const int ta = a.type[at];
+            // the wrapped offsets again (not kept across the scan, to spare registers)
+            double h0 = a.fx[at] - ci0, h1 = a.fy[at] - ci1, h2 = a.fz[at] - ci2;
+            h0 -= floor(h0);
+            h1 -= floor(h1);
+            h2 -= floor(h2);
             for (int s = 0; s < 8; ++s) {
-                if (!(mask & (1u << s))) continue;
+                if (!(mask[j] & (1u << s))) continue;
                 const double g0 = h0 - (s & 1), g1 = h1 - ((s >> 1) & 1), g2 = h2 - (s >> 2);
                 ent.put(w, g0 * a.mi[0] + g1 * a.mi[3] + g2 * a.mi[6], g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7],
                         g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8], qa, ta, COUL);
@@ -623,7 +636,7 @@ bool GridJob::shape_fits(int shape) const {
     return true;
 }
 
-void GridJob::set_shape(int s) {
+void GridJob::set_shape(int s, int chunk) {
     // the 8-image scheme needs every image within the search radius to have fractional offsets
     // inside (-1, 1): guaranteed when the search radius is below L_perp
     if (s < 0 || s >= kNumTileShapes) fail(ALEATOR_INVALID_ARGUMENT, "grid: unknown tile shape");
@@ -634,8 +647,19 @@ void GridJob::set_shape(int s) {
     // below half the smallest perpendicular width at most one image of an atom, the nearest,
     // is within the search radius (the fractional rounding image is then the nearest one)
     args.single_image = (2.0 * sr < std::min(lperp[0], std::min(lperp[1], lperp[2]))) ? 1 : 0;
+    // more than 128 atoms per pass only with one image per atom (a pass then adds <= 512 entries)
+    if (!(chunk == kChunkSmall || (args.single_image && (chunk == 256 || chunk == 512))))
+        fail(ALEATOR_INVALID_ARGUMENT, "grid: atoms per list pass must be 128, or 256 or 512 with one image per atom");
+    args.chunk = chunk;
     blocks = tile_blocks(args.n, s);
     shape = s;
+}
+
+bool GridJob::chunk_fits(int s, int chunk) const {
+    if (chunk == kChunkSmall) return true;
+    if (chunk != 256 && chunk != 512) return false;
+    const double sr = rmax + tile_radius(s) + 1e-6 * (1.0 + rmax);
+    return 2.0 * sr < std::min(lperp[0], std::min(lperp[1], lperp[2]));
 }
 
 void GridJob::bind(char* base) {
@@ -691,8 +715,8 @@ void launch_grid(Framework& fw, const GridRequest& req) {
     Context& ctx = *fw.ctx;
     GridJob job;
     job.prepare(fw, req);
-    const int shape = ctx.tile_shape;
-    job.set_shape(job.shape_fits(shape) ? shape : 0);
+    const int shape = job.shape_fits(ctx.tile_shape) ? ctx.tile_shape : 0;
+    job.set_shape(shape, job.chunk_fits(shape, ctx.tile_chunk) ? ctx.tile_chunk : 128);
     const size_t off_desc = job.bytes, off_prefix = align256(off_desc + sizeof(GridArgs));
     char* base = static_cast<char*>(ctx.ensure_scratch(off_prefix + 256));
     job.bind(base);

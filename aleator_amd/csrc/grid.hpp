@@ -23,6 +23,7 @@ struct GridArgs {
     int natoms;
     double search2;
     int single_image;  // search radius < L_perp/2: test only the nearest image of each atom
+    int chunk;         // atoms staged per list-build pass: 128, or 256 / 512 with single_image
     double rc_lj, rc2_lj;
     double rc2_real, alpha, kc;
     double cap;
@@ -54,7 +55,8 @@ struct GridJob {
     void prepare(Framework& fw, const GridRequest& req);
     double tile_radius(int shape) const;
     bool shape_fits(int shape) const;
-    void set_shape(int s);
+    void set_shape(int s, int chunk);           // chunk: atoms per list-build pass
+    bool chunk_fits(int s, int chunk) const;    // allowed with tile shape s
     int shape_used() const { return shape; }
     void bind(char* base);                 // device addresses of this job's `bytes`
     void upload(hipStream_t s) const;      // host constants -> device

@@ -21,6 +21,7 @@ struct aleator_grid_plan {
     aleator_gpu::Context* ctx = nullptr;
     std::vector<aleator_gpu::GridJob> jobs;
     int shape = 0;
+    int chunk = 128;  // atoms per list-build pass
     int total_blocks = 0;
     char* d_base = nullptr;
     aleator_gpu::GridArgs* d_descs = nullptr;
@@ -36,15 +37,16 @@ namespace {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-void plan_bind_shape(aleator_grid_plan* p, int shape) {
+void plan_bind_shape(aleator_grid_plan* p, int shape, int chunk) {
     std::vector<GridArgs> descs;
     std::vector<int> prefix(1, 0);
     for (auto& j : p->jobs) {
-        j.set_shape(shape);
+        j.set_shape(shape, chunk);
         descs.push_back(j.args);
         prefix.push_back(prefix.back() + j.blocks);
     }
     p->shape = shape;
+    p->chunk = chunk;
     p->total_blocks = prefix.back();
     Context& ctx = *p->ctx;
     ALEATOR_HIP_CHECK(hipMemcpyAsync(p->d_descs, descs.data(), descs.size() * sizeof(GridArgs), hipMemcpyHostToDevice,
@@ -130,41 +132,58 @@ aleator_status aleator_gpu_grid_plan_create(aleator_ctx* ctx_in, int32_t n_frame
             }
             require(!shapes.empty(), ALEATOR_INVALID_ARGUMENT,
                     "grid: cutoff plus tile radius exceeds the perpendicular width");
-            int chosen = shapes.front();
+            // atoms per list-build pass allowed with a shape: 128 always, 256 and 512 when every
+            // framework keeps one image per atom at that shape
+            auto chunks_for = [&](int s) {
+                std::vector<int> c{128};
+                for (int k : {256, 512}) {
+                    bool ok = true;
+                    for (const auto& j : p->jobs) ok = ok && j.chunk_fits(s, k);
+                    if (ok) c.push_back(k);
+                }
+                return c;
+            };
+            int chosen = shapes.front(), chosen_chunk = chunks_for(shapes.front()).back();
             if (tile >= 0) {
                 require(std::find(shapes.begin(), shapes.end(), tile) != shapes.end(), ALEATOR_INVALID_ARGUMENT,
                         "grid plan: tile shape too large for a framework");
                 chosen = tile;
-            } else if (shapes.size() > 1) {
-                // autotune: one warm launch, then the median of three timed launches per shape
+                chosen_chunk = chunks_for(tile).back();
+            } else {
+                // autotune over (tile shape, atoms per pass): one warm launch, then the median of
+                // three timed launches each; tuning_ms keeps each shape's best
                 hipEvent_t e0, e1;
                 ALEATOR_HIP_CHECK(hipEventCreate(&e0));
                 ALEATOR_HIP_CHECK(hipEventCreate(&e1));
                 float best = 1e30f;
                 p->tuning_ms.assign(kNumTileShapes, -1.0f);
                 for (int s : shapes) {
-                    plan_bind_shape(p.get(), s);
-                    plan_launch(p.get());
-                    float t[3];
-                    for (float& v : t) {
-                        ALEATOR_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+                    for (int c : chunks_for(s)) {
+                        plan_bind_shape(p.get(), s, c);
                         plan_launch(p.get());
-                        ALEATOR_HIP_CHECK(hipEventRecord(e1, ctx->stream));
-                        ALEATOR_HIP_CHECK(hipEventSynchronize(e1));
-                        ALEATOR_HIP_CHECK(hipEventElapsedTime(&v, e0, e1));
-                    }
-                    std::sort(t, t + 3);
-                    p->tuning_ms[s] = t[1];
-                    if (t[1] < best) {
-                        best = t[1];
-                        chosen = s;
+                        float t[3];
+                        for (float& v : t) {
+                            ALEATOR_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+                            plan_launch(p.get());
+                            ALEATOR_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+                            ALEATOR_HIP_CHECK(hipEventSynchronize(e1));
+                            ALEATOR_HIP_CHECK(hipEventElapsedTime(&v, e0, e1));
+                        }
+                        std::sort(t, t + 3);
+                        if (p->tuning_ms[s] < 0.0f || t[1] < p->tuning_ms[s]) p->tuning_ms[s] = t[1];
+                        if (t[1] < best) {
+                            best = t[1];
+                            chosen = s;
+                            chosen_chunk = c;
+                        }
                     }
                 }
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);
                 ctx->tile_shape = chosen;
+                ctx->tile_chunk = chosen_chunk;
             }
-            plan_bind_shape(p.get(), chosen);
+            plan_bind_shape(p.get(), chosen, chosen_chunk);
             if (use_graph) {
                 p->use_graph = true;
                 ALEATOR_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -209,6 +228,13 @@ aleator_status aleator_gpu_grid_plan_tile(const aleator_grid_plan* p, int32_t* t
         if (tuning_ms)
             for (int s = 0; s < kNumTileShapes; ++s)
                 tuning_ms[s] = p->tuning_ms.empty() ? -1.0f : p->tuning_ms[s];
+    });
+}
+
+aleator_status aleator_gpu_grid_plan_atoms_per_pass(const aleator_grid_plan* p, int32_t* atoms) {
+    return guarded([&] {
+        require(p && atoms, ALEATOR_INVALID_ARGUMENT, "null argument");
+        *atoms = p->chunk;
     });
 }
 

@@ -95,6 +95,7 @@ struct Context {
     void* ensure_scratch(size_t bytes);
     // grid tile shape of the per-call grid entry points (set by a grid plan's autotuning)
     int tile_shape = 0;
+    int tile_chunk = 512;  // atoms per list-build pass chosen with tile_shape (if allowed, else 128)
     // optional HIP-event timing of each call's dominant kernel, on `stream`
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

@@ -575,7 +575,7 @@ def main():
 
     # ---------------- extras
     extra = {"nodes_per_s": nodes / t_step, "tile_kernel_ms": k_avg * 1e3, "pairs_per_node": nbar,
-             "tile_shape": tile, "autotune_ms_per_shape": tuning}
+             "tile_shape": tile, "tile_atoms_per_pass": plan.atoms_per_pass(), "autotune_ms_per_shape": tuning}
     if args.copy_calibration:
         # a plain device copy of known size, for scaling HBM byte counters (FETCH_SIZE, WRITE_SIZE)
         src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
@@ -640,6 +640,7 @@ def main():
             extra["c2_tflops"] = work2 / t / 1e12
             extra["c2_ms_per_batch"] = t * 1e3
             extra["c2_tile_shape"] = plan2.tile()[0]
+            extra["c2_tile_atoms_per_pass"] = plan2.atoms_per_pass()
             plan2.close()
             del buf2, devs
         except Exception as e:  # report, do not hide the headline

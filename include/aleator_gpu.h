@@ -166,8 +166,9 @@ aleator_status aleator_gpu_grid_build_device(aleator_framework* fw,
  * call; each framework's tables equal aleator_gpu_grid_build_device's).  Creation uploads every
  * constant once; execute only launches kernels -- one tile launch over all frameworks -- and,
  * with use_graph, replays them from a captured hipGraph.  tile: a tile-shape index in
- * [0, 4), or -1 to time every shape that fits on the context stream and keep the fastest (the
- * choice is also used by later per-call builds on the context).  Per framework f:
+ * [0, 4), or -1 to time every shape that fits, with every allowed number of atoms per list pass,
+ * on the context stream and keep the fastest (the choice is also used by later per-call builds
+ * on the context).  Per framework f:
  * grid_lattices[f] (or NULL for all: the framework lattices), n_points[3f..3f+2], and
  * out_device[f] with (n_guest + coulomb) * nodes doubles. */
 typedef struct aleator_grid_plan aleator_grid_plan;
@@ -179,9 +180,13 @@ aleator_status aleator_gpu_grid_plan_create(aleator_ctx* ctx, int32_t n_framewor
                                             double cap, double* const* out_device, int32_t tile,
                                             int32_t use_graph, aleator_grid_plan** out);
 aleator_status aleator_gpu_grid_plan_execute(aleator_grid_plan* plan);
-/* The tile shape in use and, when the plan autotuned, the median time (ms) of each shape
- * (-1 where a shape was not tried); tuning_ms may be NULL, else holds 4 floats. */
+/* The tile shape in use and, when the plan autotuned, the median time (ms) of each shape at
+ * its best atoms-per-pass (-1 where a shape was not tried); tuning_ms may be NULL, else holds
+ * 4 floats. */
 aleator_status aleator_gpu_grid_plan_tile(const aleator_grid_plan* plan, int32_t* tile, float* tuning_ms);
+/* Framework atoms staged per pass of the tile kernel's LDS list build, tuned with the shape
+ * (128; 256 or 512 when at most one image of an atom can be listed). */
+aleator_status aleator_gpu_grid_plan_atoms_per_pass(const aleator_grid_plan* plan, int32_t* atoms);
 aleator_status aleator_gpu_grid_plan_destroy(aleator_grid_plan* plan);
 
 /* ---- energy-grid cache file, format v1 -------------------------------------------------
