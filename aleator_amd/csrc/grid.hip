@@ -43,8 +43,8 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kCap = 1024;   // LDS list capacity (entries)
 // atoms staged per pass of the list build: 128 (each may add 8 images, 1024 <= kCap), or, when
-// only the nearest image can be within the search radius, 256 or 512 (1 or 2 atoms per thread);
-// the plan's autotuner picks it with the tile shape
+// only the nearest image can be within the search radius, 256, 512 or 1024 (1, 2 or 4 atoms per
+// thread); the plan's autotuner picks it with the tile shape
 constexpr int kChunkSmall = 128;
 constexpr int kErfcIntervals = 256;  // FP32 erfc(alpha r) table: cubic Hermite pieces over [0, alpha rc]
 
@@ -316,13 +316,13 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
     int pending = 0;
     const int natoms = a.natoms;
     const int chunk = a.chunk;
-    const int apt = chunk > kThreads ? 2 : 1;  // atoms per thread and pass (contiguous, so the
-                                               // list stays in atom order for every chunk)
+    const int apt = chunk > kThreads ? chunk / kThreads : 1;  // atoms per thread and pass (contiguous,
+                                                              // so the list stays in atom order)
     for (int base = 0; base < natoms; base += chunk) {
         int cnt = 0;
-        unsigned mask[2] = {0u, 0u};  // images of each atom within the search radius
+        unsigned mask[4] = {0u, 0u, 0u, 0u};  // images of each atom within the search radius
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 4; ++j) {
             const int at = base + tid * apt + j;
             if (j >= apt || tid * apt >= chunk || at >= natoms) continue;
             double h0 = a.fx[at] - ci0, h1 = a.fy[at] - ci1, h2 = a.fz[at] - ci2;
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __r
         }
         int w = pending + off;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 4; ++j) {
             if (!mask[j]) continue;
             const int at = base + tid * apt + j;
             const double qa = a.q[at];
@@ -647,9 +647,10 @@ void GridJob::set_shape(int s, int chunk) {
     // below half the smallest perpendicular width at most one image of an atom, the nearest,
     // is within the search radius (the fractional rounding image is then the nearest one)
     args.single_image = (2.0 * sr < std::min(lperp[0], std::min(lperp[1], lperp[2]))) ? 1 : 0;
-    // more than 128 atoms per pass only with one image per atom (a pass then adds <= 512 entries)
-    if (!(chunk == kChunkSmall || (args.single_image && (chunk == 256 || chunk == 512))))
-        fail(ALEATOR_INVALID_ARGUMENT, "grid: atoms per list pass must be 128, or 256 or 512 with one image per atom");
+    // more than 128 atoms per pass only with one image per atom (a pass then adds <= 1024 entries)
+    if (!(chunk == kChunkSmall || (args.single_image && (chunk == 256 || chunk == 512 || chunk == 1024))))
+        fail(ALEATOR_INVALID_ARGUMENT,
+             "grid: atoms per list pass must be 128, or 256, 512 or 1024 with one image per atom");
     args.chunk = chunk;
     blocks = tile_blocks(args.n, s);
     shape = s;
@@ -657,7 +658,7 @@ void GridJob::set_shape(int s, int chunk) {
 
 bool GridJob::chunk_fits(int s, int chunk) const {
     if (chunk == kChunkSmall) return true;
-    if (chunk != 256 && chunk != 512) return false;
+    if (chunk != 256 && chunk != 512 && chunk != 1024) return false;
     const double sr = rmax + tile_radius(s) + 1e-6 * (1.0 + rmax);
     return 2.0 * sr < std::min(lperp[0], std::min(lperp[1], lperp[2]));
 }

@@ -23,7 +23,7 @@ struct GridArgs {
     int natoms;
     double search2;
     int single_image;  // search radius < L_perp/2: test only the nearest image of each atom
-    int chunk;         // atoms staged per list-build pass: 128, or 256 / 512 with single_image
+    int chunk;         // atoms staged per list-build pass: 128, or 256 / 512 / 1024 with single_image
     double rc_lj, rc2_lj;
     double rc2_real, alpha, kc;
     double cap;

@@ -132,11 +132,11 @@ aleator_status aleator_gpu_grid_plan_create(aleator_ctx* ctx_in, int32_t n_frame
             }
             require(!shapes.empty(), ALEATOR_INVALID_ARGUMENT,
                     "grid: cutoff plus tile radius exceeds the perpendicular width");
-            // atoms per list-build pass allowed with a shape: 128 always, 256 and 512 when every
-            // framework keeps one image per atom at that shape
+            // atoms per list-build pass allowed with a shape: 128 always, 256, 512 and 1024 when
+            // every framework keeps one image per atom at that shape
             auto chunks_for = [&](int s) {
                 std::vector<int> c{128};
-                for (int k : {256, 512}) {
+                for (int k : {256, 512, 1024}) {
                     bool ok = true;
                     for (const auto& j : p->jobs) ok = ok && j.chunk_fits(s, k);
                     if (ok) c.push_back(k);

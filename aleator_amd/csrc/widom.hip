@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void widom_recip_runs_kernel(const WidomArgs a
 // a site lies in the list, and at most one image of an atom does (cutoff <= L_perp/2), so the
 // sum is the reference's minimum-image sum, taken in atom order per trial.
 constexpr int kWCap = 1024;   // LDS list capacity (entries)
-constexpr int kWChunk = 128;  // atoms tested per pass (8 images each <= kWCap)
+constexpr int kWChunk = 128;  // atoms tested per pass with 8 images each (<= kWCap)
 
 struct CellArgs {
     int nc[3];         // cubes per Cartesian axis
@@ -555,28 +555,31 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
             __syncthreads();
         };
 
+        // atoms per pass: 128 (8 images each <= kWCap), or 1024 (4 per thread, contiguous) when only
+        // the nearest image of an atom can be within R
+        const int chunk = c.single_image ? 4 * kWThreads : kWChunk;
+        const int apt = c.single_image ? 4 : 1;
         int pending = 0;
-        for (int base = 0; base < a.natoms; base += kWChunk) {
-            const int at = base + tid;
+        for (int base = 0; base < a.natoms; base += chunk) {
             int cnt = 0;
-            unsigned mask = 0;
-            double h0 = 0, h1 = 0, h2 = 0;
-            if (tid < kWChunk && at < a.natoms) {
-                h0 = a.fx[at] - cf[0];
-                h1 = a.fy[at] - cf[1];
-                h2 = a.fz[at] - cf[2];
+            unsigned mask[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int at = base + tid * apt + j;
+                if (j >= apt || tid * apt >= chunk || at >= a.natoms) continue;
+                double h0 = a.fx[at] - cf[0], h1 = a.fy[at] - cf[1], h2 = a.fz[at] - cf[2];
                 h0 -= floor(h0);
                 h1 -= floor(h1);
                 h2 -= floor(h2);
+                const int near = (h0 >= 0.5 ? 1 : 0) | (h1 >= 0.5 ? 2 : 0) | (h2 >= 0.5 ? 4 : 0);
                 for (int sh = 0; sh < 8; ++sh) {
-                    if (c.single_image && sh != ((h0 >= 0.5 ? 1 : 0) | (h1 >= 0.5 ? 2 : 0) | (h2 >= 0.5 ? 4 : 0)))
-                        continue;
+                    if (c.single_image && sh != near) continue;
                     const double g0 = h0 - (sh & 1), g1 = h1 - ((sh >> 1) & 1), g2 = h2 - (sh >> 2);
                     const double x = g0 * a.m[0] + g1 * a.m[3] + g2 * a.m[6];
                     const double y = g0 * a.m[1] + g1 * a.m[4] + g2 * a.m[7];
                     const double z = g0 * a.m[2] + g1 * a.m[5] + g2 * a.m[8];
                     if (x * x + y * y + z * z <= c.rlist2) {
-                        mask |= 1u << sh;
+                        mask[j] |= 1u << sh;
                         ++cnt;
                     }
                 }
@@ -587,12 +590,19 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
                 compute(pending);
                 pending = 0;
             }
-            if (mask) {
-                int w = pending + off;
+            int w = pending + off;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (!mask[j]) continue;
+                const int at = base + tid * apt + j;
                 const double qa = a.q[at];
                 const int ta = a.type[at];
+                double h0 = a.fx[at] - cf[0], h1 = a.fy[at] - cf[1], h2 = a.fz[at] - cf[2];
+                h0 -= floor(h0);
+                h1 -= floor(h1);
+                h2 -= floor(h2);
                 for (int sh = 0; sh < 8; ++sh) {
-                    if (!(mask & (1u << sh))) continue;
+                    if (!(mask[j] & (1u << sh))) continue;
                     const double g0 = h0 - (sh & 1), g1 = h1 - ((sh >> 1) & 1), g2 = h2 - (sh >> 2);
                     ent.put(w, g0 * a.m[0] + g1 * a.m[3] + g2 * a.m[6], g0 * a.m[1] + g1 * a.m[4] + g2 * a.m[7],
                             g0 * a.m[2] + g1 * a.m[5] + g2 * a.m[8], qa, ta, CHARGED);
