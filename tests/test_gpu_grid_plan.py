@@ -47,6 +47,26 @@ def test_batched_plan_equals_per_call_builds(ctx):
     assert rel_err(bufs[1].to_host()[nodes], ref).max() <= 1e-10
 
 
+def test_atoms_per_pass_on_a_single_image_framework(ctx):
+    """The headline framework (c1 2x2x2 supercell, search radius below half its perpendicular
+    width) lets the list build stage up to 1024 atoms per pass; every shape's tables agree with
+    the per-call build and the oracle."""
+    fw, lj = ol.c1_system()
+    sc = ol.replicate(fw, ol.min_supercell(fw.m, 12.0))
+    dev = device_framework(ctx, sc, lj)
+    counts = ol.grid_counts(fw.m, 0.3)  # 65^3 nodes: every tile shape keeps 2 (rc + tile radius) < L_perp
+    buf = aa.DeviceBuffer(int(np.prod(counts)))
+    ref = dev.grid([4], counts, precision=aa.FP64, grid_lattice=fw.m)[0].ravel()
+    for shape in range(4):
+        plan = aa.GridPlan(ctx, [dev], [4], [counts], [int(buf)], precision=aa.FP64, grid_lattices=[fw.m], tile=shape)
+        assert plan.atoms_per_pass() == 1024
+        plan.execute()
+        assert rel_err(buf.to_host(), ref).max() <= 1e-10  # FP64 bar; tile-relative rounding differs by shape
+        plan.close()
+    nodes = np.arange(0, int(np.prod(counts)), 997, dtype=np.int64)
+    assert rel_err(ref[nodes], ol.grid_nodes_lj(lj, sc, fw.m, counts, 4, nodes)).max() <= 1e-10
+
+
 def test_graph_replay_is_bitwise_and_repeatable(ctx):
     systems = frameworks(ctx, 2)
     devs = [device_framework(ctx, sc, lj) for fw, sc, lj, counts in systems]
@@ -73,8 +93,10 @@ def test_autotune_picks_a_fitting_shape(ctx):
     buf = aa.DeviceBuffer(int(np.prod(counts)))
     plan = aa.GridPlan(ctx, [dev], [4], [counts], [int(buf)], precision=aa.MIXED, tile=-1)
     shape, ms = plan.tile()
-    print(f"autotune c0 MIXED: chose {shape}, medians (ms) {ms}")
+    per_pass = plan.atoms_per_pass()
+    print(f"autotune c0 MIXED: chose {shape} with {per_pass} atoms per list pass, medians (ms) {ms}")
     assert 0 <= shape < 4 and ms[shape] > 0
+    assert per_pass == 128  # c0's search radius admits two images of an atom: 8 images per atom, 128 per pass
     plan.execute()
     got = buf.to_host()
     ref = dev.grid([4], counts, precision=aa.MIXED)[0].ravel()
