@@ -298,6 +298,15 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     __shared__ uint64_t s_call;                   // Philox call index of the next step's first draw
     __shared__ int s_rbeg[2 * kMaxSites], s_rlen[2 * kMaxSites];  // framework list of each site row
     __shared__ double s_lj[4 * kMaxSites * kMaxTypes];             // site vs framework type LJ constants
+    // statistics and pending-move state, read and written by thread 0 only; kept in LDS so the
+    // other threads hold no registers for them
+    __shared__ struct {
+        double self_e, excl_e, sum_n;
+        double bias;                  // V p(anchor) of the insertion density (1: uniform insertion)
+        double pend_self, pend_excl;  // self / exclusion deltas of the pending move
+        uint64_t att[4], acc[4];
+    } t0;
+    __shared__ double s_ex[kMaxSites * (kMaxSites - 1)];  // exclusion pair terms: new sites, then old
 
     const int chain = blockIdx.x;
     const int tid = threadIdx.x;
@@ -319,15 +328,20 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     int n_mol = st->n_mol;
     int since_resync = st->moves_since_resync;
     int error = st->error;
-    if (tid == 0) s_call = st->call;
-    // used by thread 0 only
-    double self_e = st->self_energy, excl_e = st->exclusion_energy;
-    double sum_n = st->sum_n;
-    uint64_t att[4] = {st->attempted[0], st->attempted[1], st->attempted[2], st->attempted[3]};
-    uint64_t acc[4] = {st->accepted[0], st->accepted[1], st->accepted[2], st->accepted[3]};
+    if (tid == 0) {
+        s_call = st->call;
+        t0.self_e = st->self_energy;
+        t0.excl_e = st->exclusion_energy;
+        t0.sum_n = st->sum_n;
+        t0.bias = 1.0;
+        t0.pend_self = 0.0;
+        t0.pend_excl = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            t0.att[i] = st->attempted[i];
+            t0.acc[i] = st->accepted[i];
+        }
+    }
     const double volume = fabs(a.volume);
-    double bias = 1.0;  // V p(anchor) of the insertion density (1: uniform insertion), thread 0
-    double pend_self = 0.0, pend_excl = 0.0;  // self / exclusion deltas of the pending move, thread 0
 
     __syncthreads();
     for (uint64_t step = 0; step < a.n_steps && !error; ++step) {
@@ -352,7 +366,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                     for (int s = 0; s < ns; ++s)
                         for (int j = 0; j < 3; ++j) oldc[s][j] = pos[((size_t)mi * ns + s) * 3 + j];
                 }
-                trial_positions(a, uni, move, prop_bin, oldc, newc, bias);
+                trial_positions(a, uni, move, prop_bin, oldc, newc, t0.bias);
                 int row = 0;
                 if (move != 2)
                     for (int s = 0; s < ns; ++s, ++row) {
@@ -373,7 +387,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         const int move = ctl[0], mi = ctl[1];
         if (move < 0) {
             if (tid == 0) {
-                sum_n += (double)n_mol;
+                t0.sum_n += (double)n_mol;
                 if (a.n_trace) a.n_trace[(size_t)chain * a.n_steps + step] = n_mol;
             }
             __syncthreads();  // ctl, s_u and s_call are rewritten next step
@@ -491,28 +505,38 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             recip = a.recip_scale * r4[0];
         }
         // ---------------- acceptance (thread 0), gcmc_acceptance.hpp:28-101
+        // The intramolecular exclusion terms come first, one (site pair, new or old) per lane of
+        // wave 0 in the reference's pair order; thread 0 adds them up in that order.
+        double ex_new = 0.0, ex_old = 0.0;
+        if (CHARGED && tid < 64) {
+            const int npair = ns * (ns - 1) / 2;
+            double term = 0.0;
+            if (tid < 2 * npair) {
+                const bool of_new = tid < npair;
+                int p = of_new ? tid : tid - npair, i = 0;
+                while (p >= ns - 1 - i) {
+                    p -= ns - 1 - i;
+                    ++i;
+                }
+                const int j2 = i + 1 + p;
+                if (of_new ? has_new : has_old) {
+                    const double* fi = &site4[4 * ((of_new ? n_rem : 0) + i)];
+                    const double* fj = &site4[4 * ((of_new ? n_rem : 0) + j2)];
+                    const double r = sqrt(dist2(a, fj[0] - fi[0], fj[1] - fi[1], fj[2] - fi[2]));
+                    term = (a.charge[i] * a.charge[j2]) *
+                           (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
+                }
+            }
+            for (int p = 0; p < npair; ++p) {
+                ex_new += __shfl(term, p);
+                ex_old += __shfl(term, npair + p);
+            }
+        }
         if (tid == 0) {
             double self_d = 0.0, excl_d = 0.0;
             if (CHARGED) {
                 double q2 = 0.0;
                 for (int s = 0; s < ns; ++s) q2 += a.charge[s] * a.charge[s];
-                double ex_new = 0.0, ex_old = 0.0;
-                for (int i = 0; i < ns; ++i)
-                    for (int j2 = i + 1; j2 < ns; ++j2) {
-                        const double qq = a.charge[i] * a.charge[j2];
-                        if (has_new) {
-                            const double* fi = &site4[4 * (n_rem + i)];
-                            const double* fj = &site4[4 * (n_rem + j2)];
-                            const double r = sqrt(dist2(a, fj[0] - fi[0], fj[1] - fi[1], fj[2] - fi[2]));
-                            ex_new += qq * (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
-                        }
-                        if (has_old) {
-                            const double* fi = &site4[4 * i];
-                            const double* fj = &site4[4 * j2];
-                            const double r = sqrt(dist2(a, fj[0] - fi[0], fj[1] - fi[1], fj[2] - fi[2]));
-                            ex_old += qq * (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
-                        }
-                    }
                 self_d = a.self_pref * ((has_new ? q2 : 0.0) - (has_old ? q2 : 0.0));
                 excl_d = -a.kc * (ex_new - ex_old);
             }
@@ -525,18 +549,18 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             } else if (move == 2) {
                 du = e4[1] + a.kc * e4[3] + recip_total;
                 ratio = (P.fugacity_internal * volume / T) / (double)(n_mol + 1) * exp(-du / T);
-                if (a.prop_cdf) ratio = ratio / bias;
+                if (a.prop_cdf) ratio = ratio / t0.bias;
             } else {
                 du = -(e4[0] + a.kc * e4[2]) + recip_total;
                 ratio = (double)n_mol / (P.fugacity_internal * volume / T) * exp(-du / T);
-                if (a.prop_cdf) ratio = ratio * bias;
+                if (a.prop_cdf) ratio = ratio * t0.bias;
             }
             const bool accepted = uni() < (ratio < 1.0 ? ratio : 1.0);
             s_call = call + (uint64_t)uni.used;
-            att[move] += 1;
-            if (accepted) acc[move] += 1;
-            pend_self = self_d;
-            pend_excl = excl_d;
+            t0.att[move] += 1;
+            if (accepted) t0.acc[move] += 1;
+            t0.pend_self = self_d;
+            t0.pend_excl = excl_d;
             ctl[2] = accepted ? 1 : 0;
         }
         __syncthreads();
@@ -546,8 +570,8 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                 break;
             }
             if (tid == 0 && CHARGED) {
-                self_e += pend_self;
-                excl_e += pend_excl;
+                t0.self_e += t0.pend_self;
+                t0.excl_e += t0.pend_excl;
             }
             // commit S(k) (the same delta, recomputed, as commit* does, :161-172)
             if (CHARGED) {
@@ -630,7 +654,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                 if (tid == 0) {
                     double q2 = 0.0;
                     for (int s = 0; s < ns; ++s) q2 += a.charge[s] * a.charge[s];
-                    self_e = a.fw_self + a.self_pref * (q2 * (double)n_mol);
+                    t0.self_e = a.fw_self + a.self_pref * (q2 * (double)n_mol);
                     double ex = 0.0;
                     for (int g = 0; g < n_mol; ++g)
                         for (int i = 0; i < ns; ++i)
@@ -645,13 +669,13 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                                 ex += a.charge[i] * a.charge[j2] *
                                       (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
                             }
-                    excl_e = -a.kc * ex;
+                    t0.excl_e = -a.kc * ex;
                 }
                 __syncthreads();
             }
         }
         if (tid == 0) {
-            sum_n += (double)n_mol;
+            t0.sum_n += (double)n_mol;
             if (a.n_trace) a.n_trace[(size_t)chain * a.n_steps + step] = n_mol;
         }
     }
@@ -665,13 +689,13 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         st->call = s_call;
         st->n_mol = n_mol;
         st->moves_since_resync = since_resync;
-        st->self_energy = self_e;
-        st->exclusion_energy = excl_e;
-        st->sum_n = sum_n;
+        st->self_energy = t0.self_e;
+        st->exclusion_energy = t0.excl_e;
+        st->sum_n = t0.sum_n;
         st->steps += a.n_steps;
         for (int i = 0; i < 4; ++i) {
-            st->attempted[i] = att[i];
-            st->accepted[i] = acc[i];
+            st->attempted[i] = t0.att[i];
+            st->accepted[i] = t0.acc[i];
         }
         st->error = error;
     }
