@@ -287,6 +287,9 @@ template <int NT, bool CHARGED>
 __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     constexpr int NW = NT / 64;
     __shared__ double s_re[kMaxKHalf], s_im[kMaxKHalf];
+    // k vectors of the half set, (nx + kmax) | (ny + kmax) << 8 | (nz + kmax) << 16, and weights
+    __shared__ int s_kn[CHARGED ? kMaxKHalf : 1];
+    __shared__ double s_kw[CHARGED ? kMaxKHalf : 1];
     __shared__ cplx phase[3 * (2 * kMaxK + 1) * 2 * kMaxSites];
     __shared__ cplx rphase[3 * (2 * kMaxK + 1) * kResyncChunk];
     __shared__ double site4[2 * kMaxSites * 4];  // removed rows, then added rows: f0 f1 f2 q
@@ -320,6 +323,9 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         for (int t = tid; t < a.nk_half; t += NT) {
             s_re[t] = a.sre[(size_t)chain * a.nk_half + t];
             s_im[t] = a.sim[(size_t)chain * a.nk_half + t];
+            const int4 n = a.kn[t];
+            s_kn[t] = (n.x + a.kmax) | (n.y + a.kmax) << 8 | (n.z + a.kmax) << 16;
+            s_kw[t] = a.kw[t];
         }
     for (int t = tid; t < 4 * kMaxSites * kMaxTypes; t += NT) s_lj[t] = a.ljtab[t];
     __syncthreads();
@@ -489,16 +495,16 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             __syncthreads();
             double accr = 0.0;
             for (int t = tid; t < a.nk_half; t += NT) {
-                const int4 n = a.kn[t];
+                const int kp = s_kn[t];
                 double dre = 0.0, dim = 0.0;
                 for (int s = 0; s < nsite; ++s) {
-                    const cplx e = cmul(cmul(phase[(n.x + a.kmax) * nsite + s], phase[(nkk + n.y + a.kmax) * nsite + s]),
-                                        phase[(2 * nkk + n.z + a.kmax) * nsite + s]);
+                    const cplx e = cmul(cmul(phase[(kp & 0xff) * nsite + s], phase[(nkk + ((kp >> 8) & 0xff)) * nsite + s]),
+                                        phase[(2 * nkk + (kp >> 16)) * nsite + s]);
                     const double qs = s < n_rem ? -site4[4 * s + 3] : site4[4 * s + 3];
                     dre += qs * e.re;
                     dim += qs * e.im;
                 }
-                accr += a.kw[t] * (2.0 * (s_re[t] * dre + s_im[t] * dim) + (dre * dre + dim * dim));
+                accr += s_kw[t] * (2.0 * (s_re[t] * dre + s_im[t] * dim) + (dre * dre + dim * dim));
             }
             double r4[4] = {accr, 0.0, 0.0, 0.0};
             block_sum4<NW>(r4, red);
@@ -576,11 +582,11 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             // commit S(k) (the same delta, recomputed, as commit* does, :161-172)
             if (CHARGED) {
                 for (int t = tid; t < a.nk_half; t += NT) {
-                    const int4 n = a.kn[t];
+                    const int kp = s_kn[t];
                     double dre = 0.0, dim = 0.0;
                     for (int s = 0; s < nsite; ++s) {
-                        const cplx e = cmul(cmul(phase[(n.x + a.kmax) * nsite + s], phase[(nkk + n.y + a.kmax) * nsite + s]),
-                                            phase[(2 * nkk + n.z + a.kmax) * nsite + s]);
+                        const cplx e = cmul(cmul(phase[(kp & 0xff) * nsite + s], phase[(nkk + ((kp >> 8) & 0xff)) * nsite + s]),
+                                            phase[(2 * nkk + (kp >> 16)) * nsite + s]);
                         const double qs = s < n_rem ? -site4[4 * s + 3] : site4[4 * s + 3];
                         dre += qs * e.re;
                         dim += qs * e.im;
@@ -636,12 +642,12 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                     }
                     __syncthreads();
                     for (int t = tid; t < a.nk_half; t += NT) {
-                        const int4 n = a.kn[t];
+                        const int kp = s_kn[t];
                         double dre = 0.0, dim = 0.0;
                         for (int s = 0; s < cnt; ++s) {
-                            const cplx e = cmul(cmul(rphase[(n.x + a.kmax) * kResyncChunk + s],
-                                                     rphase[(nkk + n.y + a.kmax) * kResyncChunk + s]),
-                                                rphase[(2 * nkk + n.z + a.kmax) * kResyncChunk + s]);
+                            const cplx e = cmul(cmul(rphase[(kp & 0xff) * kResyncChunk + s],
+                                                     rphase[(nkk + ((kp >> 8) & 0xff)) * kResyncChunk + s]),
+                                                rphase[(2 * nkk + (kp >> 16)) * kResyncChunk + s]);
                             const double qs = a.charge[(base + s) % ns];
                             dre += qs * e.re;
                             dim += qs * e.im;
