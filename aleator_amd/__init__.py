@@ -14,7 +14,9 @@ std::invalid_argument, IndexError for std::out_of_range, RuntimeError otherwise)
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import os
+import weakref
 
 import numpy as np
 
@@ -402,12 +404,22 @@ def synth_framework(lattice, n_atoms: int, seed=1, stream=0, min_dist=1.0,
     return x, y, z, q, sp
 
 
+_creation = itertools.count()
+
+
 class Context:
-    """One GPU and its HIP stream (no reference counterpart)."""
+    """One GPU and its HIP stream (no reference counterpart).  Closing it first closes every
+    object still open on it (frameworks, plans, chains, ...), newest first, so that no native
+    handle outlives the context it points into."""
 
     def __init__(self, device: int = 0):
         self._h = C.c_void_p()
+        self._owned = []  # (creation number, weak reference) of objects created on this context
         _check(_load().aleator_gpu_init(C.c_int32(device), C.byref(self._h)))
+
+    def _adopt(self, obj):
+        self._owned = [(n, r) for n, r in self._owned if r() is not None]
+        self._owned.append((next(_creation), weakref.ref(obj)))
 
     @property
     def handle(self):
@@ -427,6 +439,11 @@ class Context:
 
     def close(self):
         if self._h:
+            for _, r in sorted(self._owned, key=lambda e: e[0], reverse=True):
+                obj = r()
+                if obj is not None:
+                    obj.close()
+            self._owned = []
             _load().aleator_gpu_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -457,6 +474,7 @@ class Framework:
         self._h = C.c_void_p()
         _check(_load().aleator_gpu_framework_create(ctx.handle, C.byref(lat), C.byref(parts), C.byref(lj),
                                                     C.byref(ew) if ew is not None else None, C.byref(self._h)))
+        ctx._adopt(self)
         self.n_species = len(eps)
 
     @property
@@ -558,6 +576,7 @@ class GridPlan:
                                                     C.c_int32(1 if coulomb else 0), _p(pts, _i32p),
                                                     C.c_int32(precision), C.c_double(cap), outs, C.c_int32(tile),
                                                     C.c_int32(1 if use_graph else 0), C.byref(self._h)))
+        ctx._adopt(self)
 
     def execute(self):
         _check(_load().aleator_gpu_grid_plan_execute(self._h))
@@ -663,6 +682,7 @@ class Gcmc:
         _check(_load().aleator_gpu_gcmc_create(fw.handle, arr, C.c_int32(len(sites)), desc, C.c_int32(len(chains)),
                                                C.c_double(max_translation), C.c_double(max_rotation),
                                                C.c_int32(max_molecules), C.byref(self._h)))
+        fw.ctx._adopt(self)
 
     def close(self):
         if self._h:
@@ -734,6 +754,18 @@ class Gcmc:
             _check(_load().aleator_gpu_gcmc_set_energy_grid(self._h, n, _p(sp, _u32p), C.c_int32(sp.size),
                                                             C.cast(C.c_void_p(int(tables)), _dp)))
 
+    def set_guest_cells(self, mode: int):
+        """Guest-guest candidates through guest cells (aleator_gpu_gcmc_set_guest_cells): 0 full
+        scan, 1 cells when every axis has at least 4 (the default), 2 cells whenever the guest
+        sites fit (IndexError above 4096)."""
+        _check(_load().aleator_gpu_gcmc_set_guest_cells(self._h, C.c_int32(int(mode))))
+
+    def guest_cells(self):
+        """Guest cells per lattice axis in use; (0, 0, 0) for the full scan."""
+        n = (C.c_int32 * 3)()
+        _check(_load().aleator_gpu_gcmc_guest_cells(self._h, n))
+        return tuple(int(v) for v in n)
+
     def positions(self, chain: int) -> np.ndarray:
         n = C.c_int32()
         _check(_load().aleator_gpu_gcmc_positions(self._h, C.c_int32(chain), None, C.c_uint64(0), C.byref(n)))
@@ -777,6 +809,7 @@ class Mlp:
         self.max_rows = int(max_rows)
         self._h = C.c_void_p()
         _check(_load().aleator_gpu_mlp_create(ctx.handle, C.c_int32(self.max_rows), C.byref(self._h)))
+        ctx._adopt(self)
         if seed is not None:
             self.init(seed)
 
@@ -876,6 +909,7 @@ class EwaldState:
         self.fw = fw
         self._h = C.c_void_p()
         _check(_load().aleator_gpu_ewald_state_create(fw.handle, C.byref(self._h)))
+        fw.ctx._adopt(self)
 
     def close(self):
         if self._h:

@@ -77,6 +77,16 @@ __device__ inline double dist2(const GcmcArgs& a, double d0, double d1, double d
     return x * x + y * y + z * z;
 }
 
+// Guest cell of a fractional position: gc[j] cells along lattice axis j, position wrapped
+// into the cell first.
+__device__ inline void guest_cell(const GcmcArgs& a, const double f[3], int c[3]) {
+    for (int j = 0; j < 3; ++j) {
+        const double t = (f[j] - floor(f[j])) * (double)a.gc[j];
+        const int k = t >= 0.0 ? (int)t : 0;
+        c[j] = k < a.gc[j] - 1 ? k : a.gc[j] - 1;
+    }
+}
+
 __device__ inline double lj_term(const GcmcArgs& a, const double* tab4, int stride, int idx, double r2) {
     // tab4: [4][stride] = sigma^2, 4 eps, V(rc), V'(rc) (lennard_jones.cc:70-86)
     const double s2 = tab4[idx] / r2;
@@ -301,6 +311,9 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     __shared__ uint64_t s_call;                   // Philox call index of the next step's first draw
     __shared__ int s_rbeg[2 * kMaxSites], s_rlen[2 * kMaxSites];  // framework list of each site row
     __shared__ double s_lj[4 * kMaxSites * kMaxTypes];             // site vs framework type LJ constants
+    __shared__ int s_cand[kMaxGuestCand];       // guest cell filter: candidate guest sites, ascending
+    __shared__ int s_rowcell[2 * kMaxSites][3];  // guest cell of each site row
+    __shared__ int s_wtot[NW];
     // statistics and pending-move state, read and written by thread 0 only; kept in LDS so the
     // other threads hold no registers for them
     __shared__ struct {
@@ -309,7 +322,6 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         double pend_self, pend_excl;  // self / exclusion deltas of the pending move
         uint64_t att[4], acc[4];
     } t0;
-    __shared__ double s_ex[kMaxSites * (kMaxSites - 1)];  // exclusion pair terms: new sites, then old
 
     const int chain = blockIdx.x;
     const int tid = threadIdx.x;
@@ -418,6 +430,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             }
             s_rbeg[tid] = beg;
             s_rlen[tid] = len;
+            if (a.gc[0] > 0) guest_cell(a, &site4[4 * tid], s_rowcell[tid]);
         }
         __syncthreads();
         {
@@ -449,13 +462,54 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                 }
             }
         }
-        // other guests: every (site row, guest site) pair in one flattened loop over the
-        // workgroup, so that a few dozen molecules still occupy every thread
+        // other guests: every (site row, candidate guest site) pair in one flattened loop over
+        // the workgroup, so that a few dozen molecules still occupy every thread.  Without guest
+        // cells every site of the other molecules is a candidate.  With them (cells at least the
+        // guest cutoff wide, perpendicular to each lattice plane) a site is a candidate when its
+        // cell is a periodic neighbour of some row's cell, which holds for every site within the
+        // cutoff of a row; the candidates are compacted in ascending site order, the order of
+        // the reference's sorted cell-list candidates (monte_carlo_engine.cc:358-366).
         {
             const int gsites = n_mol * ns;
-            const int total = nsite * gsites;
+            int ncand = gsites;
+            if (a.gc[0] > 0) {
+                const int wave = tid >> 6, lane = tid & 63;
+                int out = 0;
+                for (int base = 0; base < gsites; base += NT) {
+                    const int gs = base + tid;
+                    bool take = false;
+                    if (gs < gsites && gs / ns != mi) {
+                        const double* gp = pos + (size_t)gs * 3;
+                        const double gc[3] = {gp[0], gp[1], gp[2]};
+                        double gf[3];
+                        frac_of(a, gc, gf);
+                        int c[3];
+                        guest_cell(a, gf, c);
+                        for (int r = 0; r < nsite && !take; ++r) {
+                            bool near = true;
+                            for (int j = 0; j < 3; ++j) {
+                                int d = c[j] - s_rowcell[r][j];
+                                d = d < 0 ? d + a.gc[j] : d;
+                                near = near && (d <= 1 || d == a.gc[j] - 1);
+                            }
+                            take = near;
+                        }
+                    }
+                    const unsigned long long mask = __ballot(take);
+                    if (lane == 0) s_wtot[wave] = __popcll(mask);
+                    __syncthreads();
+                    int off = out;
+                    for (int w = 0; w < wave; ++w) off += s_wtot[w];
+                    if (take) s_cand[off + __popcll(mask & ((1ull << lane) - 1ull))] = gs;
+                    for (int w = 0; w < NW; ++w) out += s_wtot[w];
+                    __syncthreads();
+                }
+                ncand = out;
+            }
+            const int total = nsite * ncand;
             for (int i = tid; i < total; i += NT) {
-                const int row = i / gsites, gs = i - row * gsites;
+                const int row = i / ncand, k = i - row * ncand;
+                const int gs = a.gc[0] > 0 ? s_cand[k] : k;
                 const int g = gs / ns, u = gs - g * ns;
                 if (g == mi) continue;
                 const double* gp = pos + (size_t)gs * 3;

@@ -8,6 +8,8 @@
 
 namespace aleator_gpu {
 
+constexpr int kMaxGuestCand = 4096;  // guest sites (max molecules x sites) the guest cell filter holds
+
 struct ChainParams {
     double fugacity_internal;  // K/A^3 (fugacity in Pa times 1/(k_B 1e30), monte_carlo_engine.hpp:184)
     double temperature;
@@ -72,6 +74,9 @@ struct GcmcArgs {
     const double4* bin_pos;    // list entries: fractional x, y, z and charge of the atom
     const int32_t* bin_type;   // list entries: compact LJ type of the atom
     int fb[3];
+    // guest cells per lattice axis for the guest-guest candidate filter (gc[0] == 0: every
+    // guest site is a candidate); each cell is at least the guest cutoff wide
+    int gc[3];
     // energy grid (null: framework LJ summed over the atoms).  With a grid, a site's framework
     // LJ energy is table grid_table[s] interpolated at its Cartesian position, the framework scan
     // keeps only real-space Coulomb, and guest-guest LJ is unchanged (moleculeDispersionEnergy

@@ -130,6 +130,32 @@ void set_energy_grid(aleator_gcmc* g, const int32_t n_points[3], const uint32_t*
 
 using namespace aleator_gpu;
 
+namespace {
+
+// Guest cells for the guest-guest candidate filter.  mode 0: none; 1: only when every axis has
+// at least 4 cells (with 3 or fewer each cell's neighbours are all cells); 2: whenever the guest
+// sites fit the filter's capacity.  A cell is at least the guest cutoff (LJ, and real-space
+// Coulomb when charged) wide perpendicular to its lattice plane: L_perp_j / cells_j >= rc.
+void set_guest_cells(aleator_gpu::GcmcArgs& a, int mode) {
+    using namespace aleator_gpu;
+    a.gc[0] = a.gc[1] = a.gc[2] = 0;
+    if (mode == 0) return;
+    const bool fits = (int64_t)a.max_mol * a.ns <= kMaxGuestCand;
+    require(fits || mode == 1, ALEATOR_OUT_OF_RANGE, "gcmc: guest cells hold at most 4096 guest sites");
+    if (!fits) return;
+    const double rc = std::sqrt(a.charged ? std::max(a.rc2_lj, a.rc2_real) : a.rc2_lj);
+    int n[3];
+    for (int j = 0; j < 3; ++j) {
+        // |b_j|, b_j = column j of the inverse lattice, is 1 / L_perp_j
+        const double b = std::sqrt(a.inv[j] * a.inv[j] + a.inv[3 + j] * a.inv[3 + j] + a.inv[6 + j] * a.inv[6 + j]);
+        n[j] = std::max(1, (int)std::floor(1.0 / (b * rc)));
+    }
+    if (mode == 1 && std::min({n[0], n[1], n[2]}) < 4) return;
+    for (int j = 0; j < 3; ++j) a.gc[j] = n[j];
+}
+
+}  // namespace
+
 extern "C" {
 
 aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_molecule_site* sites, int32_t n_sites,
@@ -202,6 +228,7 @@ aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_m
         a.max_translation = max_translation;
         a.max_rotation = max_rotation;
         a.max_mol = max_molecules;
+        set_guest_cells(a, 1);
         // LJ constants: site vs framework type, and site vs site
         std::vector<double> tab(4 * kMaxSites * kMaxTypes, 0.0), gg(4 * kMaxSites * kMaxSites, 0.0);
         for (int k = 0; k < n_sites; ++k) {
@@ -440,6 +467,21 @@ aleator_status aleator_gpu_gcmc_electrostatic_energy(aleator_gcmc* g, int32_t ch
         }
         const double recip = sum * kCoulomb * (2.0 * kPi / std::abs(fw.lat.volume));
         *energy = recip + st[chain].self_energy + st[chain].exclusion_energy;
+    });
+}
+
+aleator_status aleator_gpu_gcmc_set_guest_cells(aleator_gcmc* g, int32_t mode) {
+    return guarded([&] {
+        require(g != nullptr, ALEATOR_INVALID_ARGUMENT, "null chains");
+        require(mode >= 0 && mode <= 2, ALEATOR_INVALID_ARGUMENT, "gcmc: guest cell mode is 0, 1 or 2");
+        set_guest_cells(g->args, mode);
+    });
+}
+
+aleator_status aleator_gpu_gcmc_guest_cells(const aleator_gcmc* g, int32_t cells[3]) {
+    return guarded([&] {
+        require(g != nullptr && cells != nullptr, ALEATOR_INVALID_ARGUMENT, "null argument");
+        for (int j = 0; j < 3; ++j) cells[j] = g->args.gc[j];
     });
 }
 

@@ -367,6 +367,17 @@ aleator_status aleator_gpu_gcmc_set_energy_grid(aleator_gcmc* gcmc, const int32_
 aleator_status aleator_gpu_gcmc_set_energy_grid_host(aleator_gcmc* gcmc, const int32_t n_points[3],
                                                      const uint32_t* species, int32_t n_tables,
                                                      const double* tables);
+/* Guest-guest candidates through guest cells, the device form of the engine's guest CellList
+ * (MonteCarloEngine::moleculeDispersionEnergy with guestCellList_.forEachIndexNear and the
+ * ascending sort, monte_carlo_engine.cc:341-374; CellList, cell_list.hpp:26-123).  Cells at
+ * least the guest cutoff wide along each lattice axis; a guest site is a candidate of a trial
+ * when its cell neighbours a trial site's cell, compacted in ascending site order, so the
+ * pairs within the cutoff are those of the full scan.  mode 0: full scan; 1 (the default):
+ * cells when every axis has at least 4; 2: cells whenever max_molecules x sites <= 4096
+ * (ALEATOR_OUT_OF_RANGE above).  Takes effect at the next run. */
+aleator_status aleator_gpu_gcmc_set_guest_cells(aleator_gcmc* gcmc, int32_t mode);
+/* cells per lattice axis in use (0, 0, 0: full scan) */
+aleator_status aleator_gpu_gcmc_guest_cells(const aleator_gcmc* gcmc, int32_t cells[3]);
 
 /* ---- learned insertion density: Fourier-feature MLP (no reference counterpart) ----
  * 64 features (sin, cos of 2 pi k.f for 32 integer vectors k, aleator_gpu_mlp_frequencies)

@@ -251,3 +251,75 @@ def test_gcmc_energy_grid_errors(ctx):
         gc.set_energy_grid((0, 4, 4), [4], np.zeros(0))
     gc.run(10)
     assert gc.stats()[0]["steps"] == 10
+
+
+def test_guest_cells_follow_the_full_scan_and_the_oracle(ctx):
+    """Guest-guest candidates through guest cells (the engine's guest CellList): in a 28 A box with
+    a 6.5 A cutoff the default picks 4 x 4 x 4 cells, and with a few hundred methanes the chains
+    still follow the oracle's full-scan trajectory step for step, as the reference's cell list
+    follows its full scan (test_gcmc_cell_list_bit_identical.cc:111-175); forcing the full scan
+    gives the same trajectory."""
+    print("gcmc guest cells: CH4, 28 A box, 3000 steps", flush=True)
+    fw, lj = ol.c0_system(40, a=28.0)
+    lj.cutoff, lj.truncation = 6.5, ol.SHIFTED
+    sites = [(0.0, 0.0, 0.0, 4, 0.0)]
+    dev = device_framework(ctx, fw, lj)
+    chains = [(1e8, 300.0, 5, 0), (3e8, 300.0, 6, 2)]
+    gc = aa.Gcmc(dev, sites, chains, max_molecules=1024)
+    assert gc.guest_cells() == (4, 4, 4)
+    trace = gc.run(3000, trace=True)
+    full = aa.Gcmc(dev, sites, chains, max_molecules=1024)
+    full.set_guest_cells(0)
+    assert full.guest_cells() == (0, 0, 0)
+    trace_full = full.run(3000, trace=True)
+    assert np.array_equal(trace, trace_full)
+    for c, (f, t, seed, stream) in enumerate(chains):
+        g = ol.Gcmc(lj, fw, sites, f, t, seed, stream)
+        ref_trace = g.run(3000, trace=True)
+        assert np.array_equal(trace[c], ref_trace.astype(np.int32)), f"chain {c} trajectory diverged"
+        assert g.n_molecules > 100
+        assert np.allclose(gc.positions(c), oracle_positions(g), rtol=0, atol=1e-9)
+
+
+def test_guest_cells_forced_on_a_small_charged_cell(ctx):
+    """Mode 2 uses cells even with 3 or fewer per axis, where each cell's periodic neighbours
+    cover the box: the charged CO2 chains still follow the oracle."""
+    print("gcmc guest cells forced: charged CO2, 1500 steps", flush=True)
+    m = ol.cell_from_params(16.0, 17.0, 18.0, 80.0, 95.0, 100.0)
+    fw = ol.synth_framework(m, 120, seed=4, charge_sigma=0.5)
+    eps = np.array([t[0] for t in ol.C0_TYPES] + [ol.CO2_C[0], ol.CO2_O[0]])
+    sig = np.array([t[1] for t in ol.C0_TYPES] + [ol.CO2_C[1], ol.CO2_O[1]])
+    lj = ol.LJ(eps, sig, 5.0)
+    dev = device_framework(ctx, fw, lj, ewald=(0.35, 5.0, 5))
+    chains = [(5e6, 298.0, 4, 2)]
+    gc = aa.Gcmc(dev, ol.co2_sites(), chains, max_molecules=256)
+    assert gc.guest_cells() == (0, 0, 0)
+    gc.set_guest_cells(2)
+    cells = gc.guest_cells()
+    assert min(cells) >= 2 and max(cells) <= 3
+    trace = gc.run(1500, trace=True)
+    g = ol.Gcmc(lj, fw, ol.co2_sites(), *chains[0], ewald=(0.35, 5.0, 5))
+    ref_trace = g.run(1500, trace=True)
+    assert np.array_equal(trace[0], ref_trace.astype(np.int32))
+    assert g.n_molecules > 0
+    assert np.allclose(gc.positions(0), oracle_positions(g), rtol=0, atol=1e-9)
+    e_dev, e_ref = gc.electrostatic_energy(0), g.electrostatic_energy()
+    assert abs(e_dev - e_ref) <= 1e-9 * abs(e_ref)
+    with pytest.raises(IndexError):
+        aa.Gcmc(dev, ol.co2_sites(), chains, max_molecules=2000).set_guest_cells(2)
+    with pytest.raises(ValueError):
+        gc.set_guest_cells(3)
+
+
+def test_closing_a_context_closes_what_was_created_on_it():
+    """Objects still open when their context closes are closed first, newest first, so none of
+    their native handles is left pointing into the freed context."""
+    c = aa.Context(0)
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff = 6.5
+    dev = device_framework(c, fw, lj)
+    chains = aa.Gcmc(dev, [(0.0, 0.0, 0.0, 4, 0.0)], [(1e6, 300.0, 1, 0)], max_molecules=16)
+    chains.run(10)
+    c.close()
+    assert not chains._h and not dev._h and not c._h
+    chains.close()
