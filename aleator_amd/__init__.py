@@ -756,8 +756,9 @@ class Gcmc:
 
     def set_guest_cells(self, mode: int):
         """Guest-guest candidates through guest cells (aleator_gpu_gcmc_set_guest_cells): 0 full
-        scan, 1 cells when every axis has at least 4 (the default), 2 cells whenever the guest
-        sites fit (IndexError above 4096)."""
+        scan (the default), 1 cells when every axis has at least 4 and a step starts with 1024 or
+        more guest sites, 2 cells in every step whenever the guest sites fit (IndexError above
+        4096)."""
         _check(_load().aleator_gpu_gcmc_set_guest_cells(self._h, C.c_int32(int(mode))))
 
     def guest_cells(self):

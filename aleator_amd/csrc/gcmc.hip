@@ -471,8 +471,9 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         // the reference's sorted cell-list candidates (monte_carlo_engine.cc:358-366).
         {
             const int gsites = n_mol * ns;
+            const bool cells = a.gc[0] > 0 && gsites >= a.gc_min_sites;
             int ncand = gsites;
-            if (a.gc[0] > 0) {
+            if (cells) {
                 const int wave = tid >> 6, lane = tid & 63;
                 int out = 0;
                 for (int base = 0; base < gsites; base += NT) {
@@ -509,7 +510,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             const int total = nsite * ncand;
             for (int i = tid; i < total; i += NT) {
                 const int row = i / ncand, k = i - row * ncand;
-                const int gs = a.gc[0] > 0 ? s_cand[k] : k;
+                const int gs = cells ? s_cand[k] : k;
                 const int g = gs / ns, u = gs - g * ns;
                 if (g == mi) continue;
                 const double* gp = pos + (size_t)gs * 3;

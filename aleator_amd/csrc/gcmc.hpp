@@ -9,6 +9,7 @@
 namespace aleator_gpu {
 
 constexpr int kMaxGuestCand = 4096;  // guest sites (max molecules x sites) the guest cell filter holds
+constexpr int kGuestCellMinSites = 1024;  // default: guest sites from which a step uses the cells
 
 struct ChainParams {
     double fugacity_internal;  // K/A^3 (fugacity in Pa times 1/(k_B 1e30), monte_carlo_engine.hpp:184)
@@ -77,6 +78,7 @@ struct GcmcArgs {
     // guest cells per lattice axis for the guest-guest candidate filter (gc[0] == 0: every
     // guest site is a candidate); each cell is at least the guest cutoff wide
     int gc[3];
+    int gc_min_sites;  // guest sites from which a step uses the cells
     // energy grid (null: framework LJ summed over the atoms).  With a grid, a site's framework
     // LJ energy is table grid_table[s] interpolated at its Cartesian position, the framework scan
     // keeps only real-space Coulomb, and guest-guest LJ is unchanged (moleculeDispersionEnergy

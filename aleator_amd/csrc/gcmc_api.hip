@@ -133,12 +133,14 @@ using namespace aleator_gpu;
 namespace {
 
 // Guest cells for the guest-guest candidate filter.  mode 0: none; 1: only when every axis has
-// at least 4 cells (with 3 or fewer each cell's neighbours are all cells); 2: whenever the guest
-// sites fit the filter's capacity.  A cell is at least the guest cutoff (LJ, and real-space
+// at least 4 cells (with 3 or fewer each cell's neighbours are all cells), and in a step only
+// from kGuestCellMinSites guest sites on (below, the filter pass and its barriers cost more than
+// the pairs it saves); 2: whenever the guest sites fit the filter's capacity, in every step.  A cell is at least the guest cutoff (LJ, and real-space
 // Coulomb when charged) wide perpendicular to its lattice plane: L_perp_j / cells_j >= rc.
 void set_guest_cells(aleator_gpu::GcmcArgs& a, int mode) {
     using namespace aleator_gpu;
     a.gc[0] = a.gc[1] = a.gc[2] = 0;
+    a.gc_min_sites = mode == 1 ? kGuestCellMinSites : 0;
     if (mode == 0) return;
     const bool fits = (int64_t)a.max_mol * a.ns <= kMaxGuestCand;
     require(fits || mode == 1, ALEATOR_OUT_OF_RANGE, "gcmc: guest cells hold at most 4096 guest sites");
@@ -228,7 +230,7 @@ aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_m
         a.max_translation = max_translation;
         a.max_rotation = max_rotation;
         a.max_mol = max_molecules;
-        set_guest_cells(a, 1);
+        set_guest_cells(a, 0);
         // LJ constants: site vs framework type, and site vs site
         std::vector<double> tab(4 * kMaxSites * kMaxTypes, 0.0), gg(4 * kMaxSites * kMaxSites, 0.0);
         for (int k = 0; k < n_sites; ++k) {

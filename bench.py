@@ -287,7 +287,41 @@ def host_threads() -> int:
         return os.cpu_count() or 1
 
 
-LEGS = ("c1", "c0", "c2", "c3", "ewald", "mlp", "irmof1", "gcmc_grid", "c4")
+LEGS = ("c1", "c0", "c2", "c3", "ewald", "mlp", "irmof1", "gcmc_grid", "guest_cells", "c4")
+
+
+def guest_cell_rates(aa, ctx, sync, n_chains=64, fugacity=1e7):
+    """Guest cells against the full guest scan at high occupancy: 3-site CO2-shaped LJ molecules
+    (no charges) in a sparse synthetic framework (424 atoms in a 51.66 A cube, rc 12.8 A: 4 cells
+    per axis), 64 chains at 1e7 Pa and 298 K filled by 3e4 steps to several hundred molecules,
+    then 5000 steps timed with cells (mode 1: from 1024 guest sites on) and 5000 more with the
+    full scan (the default)."""
+    from aleator_amd import workloads as wl
+    m = np.diag([51.66] * 3)
+    x, y, z, _, sp = wl.synth_framework(m, 424, seed=1, stream=0)
+    eps = np.array([t[0] for t in wl.C0_TYPES] + [wl.CO2_C[0], wl.CO2_O[0]])
+    sig = np.array([t[1] for t in wl.C0_TYPES] + [wl.CO2_C[1], wl.CO2_O[1]])
+    fw = aa.Framework(ctx, m, x, y, z, sp, eps, sig, 12.8)
+    sites = [(x, y, z, sp, 0.0) for x, y, z, sp, _ in wl.CO2_SITES]
+    gc = aa.Gcmc(fw, sites, [(fugacity, 298.0, 11, c) for c in range(n_chains)], max_molecules=1365)
+    gc.set_guest_cells(1)
+    out = {"guest_cells_per_axis": list(gc.guest_cells())}
+    gc.run(30000)
+    sync()
+    for mode, name in ((1, "cells"), (0, "full_scan")):
+        gc.set_guest_cells(mode)
+        gc.reset_stats()
+        t0 = time.perf_counter()
+        gc.run(5000)
+        sync()
+        dt = time.perf_counter() - t0
+        st = gc.stats()
+        out[f"guest_cells_{name}_chain_steps_per_s"] = n_chains * 5000 / dt
+        out[f"guest_cells_{name}_mean_N"] = float(np.mean([x["sum_molecules"] / x["steps"] for x in st]))
+        out[f"guest_cells_{name}_errors"] = sum(x["error"] for x in st)
+    gc.close()
+    fw.close()
+    return out
 
 
 def ewald_proposal_rate(aa, ctx, fw, sync, n_prop=100_000):
@@ -693,6 +727,12 @@ def main():
             extra.update(gcmc_grid_rates(aa, ctx, sync))
         except Exception as e:
             extra["gcmc_grid_error"] = repr(e)
+
+    if "guest_cells" in legs:
+        try:
+            extra.update(guest_cell_rates(aa, ctx, sync))
+        except Exception as e:
+            extra["guest_cells_error"] = repr(e)
 
     if "c4" in legs:
         # c4: learned insertion density vs plain uniform insertion on the c3 chains

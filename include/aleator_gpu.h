@@ -372,8 +372,10 @@ aleator_status aleator_gpu_gcmc_set_energy_grid_host(aleator_gcmc* gcmc, const i
  * ascending sort, monte_carlo_engine.cc:341-374; CellList, cell_list.hpp:26-123).  Cells at
  * least the guest cutoff wide along each lattice axis; a guest site is a candidate of a trial
  * when its cell neighbours a trial site's cell, compacted in ascending site order, so the
- * pairs within the cutoff are those of the full scan.  mode 0: full scan; 1 (the default):
- * cells when every axis has at least 4; 2: cells whenever max_molecules x sites <= 4096
+ * pairs within the cutoff are those of the full scan.  mode 0 (the default): full scan, the
+ * faster one at the occupancies measured (a filter pass costs about what the out-of-range pairs
+ * it drops cost); 1: cells when every axis has at least 4, in the steps that start with at
+ * least 1024 guest sites; 2: cells in every step whenever max_molecules x sites <= 4096
  * (ALEATOR_OUT_OF_RANGE above).  Takes effect at the next run. */
 aleator_status aleator_gpu_gcmc_set_guest_cells(aleator_gcmc* gcmc, int32_t mode);
 /* cells per lattice axis in use (0, 0, 0: full scan) */

@@ -381,8 +381,9 @@ public:
         const int32_t counts[3] = {n[0], n[1], n[2]};
         check(aleator_gpu_gcmc_set_energy_grid_host(g_, counts, ids.data(), (int32_t)ids.size(), all.data()));
     }
-    // Guest-guest candidates through guest cells (the engine's guestCellList_): 0 full scan,
-    // 1 cells when every axis has at least 4 (the default), 2 cells whenever the sites fit.
+    // Guest-guest candidates through guest cells (the engine's guestCellList_): 0 full scan
+    // (the default), 1 cells when every axis has at least 4 and a step starts with 1024+ guest
+    // sites, 2 cells in every step whenever the sites fit.
     void setGuestCells(int mode) { check(aleator_gpu_gcmc_set_guest_cells(g_, mode)); }
     std::array<int, 3> guestCells() const {
         int32_t n[3];

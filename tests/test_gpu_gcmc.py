@@ -255,7 +255,7 @@ def test_gcmc_energy_grid_errors(ctx):
 
 def test_guest_cells_follow_the_full_scan_and_the_oracle(ctx):
     """Guest-guest candidates through guest cells (the engine's guest CellList): in a 28 A box with
-    a 6.5 A cutoff the default picks 4 x 4 x 4 cells, and with a few hundred methanes the chains
+    a 6.5 A cutoff there are 4 x 4 x 4 cells, and with a few hundred methanes the chains
     still follow the oracle's full-scan trajectory step for step, as the reference's cell list
     follows its full scan (test_gcmc_cell_list_bit_identical.cc:111-175); forcing the full scan
     gives the same trajectory."""
@@ -266,10 +266,13 @@ def test_guest_cells_follow_the_full_scan_and_the_oracle(ctx):
     dev = device_framework(ctx, fw, lj)
     chains = [(1e8, 300.0, 5, 0), (3e8, 300.0, 6, 2)]
     gc = aa.Gcmc(dev, sites, chains, max_molecules=1024)
+    assert gc.guest_cells() == (0, 0, 0)  # the full scan is the default
+    gc.set_guest_cells(1)
+    assert gc.guest_cells() == (4, 4, 4)
+    gc.set_guest_cells(2)  # every step, not only from 1024 guest sites on
     assert gc.guest_cells() == (4, 4, 4)
     trace = gc.run(3000, trace=True)
     full = aa.Gcmc(dev, sites, chains, max_molecules=1024)
-    full.set_guest_cells(0)
     assert full.guest_cells() == (0, 0, 0)
     trace_full = full.run(3000, trace=True)
     assert np.array_equal(trace, trace_full)
@@ -293,7 +296,8 @@ def test_guest_cells_forced_on_a_small_charged_cell(ctx):
     dev = device_framework(ctx, fw, lj, ewald=(0.35, 5.0, 5))
     chains = [(5e6, 298.0, 4, 2)]
     gc = aa.Gcmc(dev, ol.co2_sites(), chains, max_molecules=256)
-    assert gc.guest_cells() == (0, 0, 0)
+    gc.set_guest_cells(1)
+    assert gc.guest_cells() == (0, 0, 0)  # fewer than 4 cells per axis
     gc.set_guest_cells(2)
     cells = gc.guest_cells()
     assert min(cells) >= 2 and max(cells) <= 3
