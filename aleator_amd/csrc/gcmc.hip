@@ -293,7 +293,7 @@ __device__ void trial_positions(const GcmcArgs& a, Draws& uni, int move, int pro
 // thread 0 applies the acceptance rule and publishes the decision.
 // CHARGED: the framework carries Ewald charges (CHARGED); the uncharged instances drop the S(k)
 // arrays and code.
-template <int NT, bool CHARGED>
+template <int NT, bool CHARGED, bool CELLS>
 __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     constexpr int NW = NT / 64;
     __shared__ double s_re[kMaxKHalf], s_im[kMaxKHalf];
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     __shared__ uint64_t s_call;                   // Philox call index of the next step's first draw
     __shared__ int s_rbeg[2 * kMaxSites], s_rlen[2 * kMaxSites];  // framework list of each site row
     __shared__ double s_lj[4 * kMaxSites * kMaxTypes];             // site vs framework type LJ constants
-    __shared__ int s_cand[kMaxGuestCand];       // guest cell filter: candidate guest sites, ascending
+    __shared__ int s_cand[CELLS ? kMaxGuestCand : 1];  // guest cell filter: candidate sites, ascending
     __shared__ int s_rowcell[2 * kMaxSites][3];  // guest cell of each site row
     __shared__ int s_wtot[NW];
     // statistics and pending-move state, read and written by thread 0 only; kept in LDS so the
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             }
             s_rbeg[tid] = beg;
             s_rlen[tid] = len;
-            if (a.gc[0] > 0) guest_cell(a, &site4[4 * tid], s_rowcell[tid]);
+            if (CELLS) guest_cell(a, &site4[4 * tid], s_rowcell[tid]);
         }
         __syncthreads();
         {
@@ -471,9 +471,9 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         // the reference's sorted cell-list candidates (monte_carlo_engine.cc:358-366).
         {
             const int gsites = n_mol * ns;
-            const bool cells = a.gc[0] > 0 && gsites >= a.gc_min_sites;
+            const bool cells = CELLS && gsites >= a.gc_min_sites;
             int ncand = gsites;
-            if (cells) {
+            if (CELLS && cells) {
                 const int wave = tid >> 6, lane = tid & 63;
                 int out = 0;
                 for (int base = 0; base < gsites; base += NT) {
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             const int total = nsite * ncand;
             for (int i = tid; i < total; i += NT) {
                 const int row = i / ncand, k = i - row * ncand;
-                const int gs = cells ? s_cand[k] : k;
+                const int gs = CELLS && cells ? s_cand[k] : k;
                 const int g = gs / ns, u = gs - g * ns;
                 if (g == mi) continue;
                 const double* gp = pos + (size_t)gs * 3;
@@ -792,12 +792,22 @@ void build_framework_bins(const GcmcArgs& a, double reach, int32_t** start, doub
 
 void launch_gcmc(const GcmcArgs& a, int n_chains, hipStream_t s) {
     if (a.nk_half > kMaxKHalf) fail(ALEATOR_NOT_IMPLEMENTED, "gcmc: more than 4096 reciprocal terms");
+    // guest cells compile in only where used, so the default full scan keeps its registers
+    const bool cells = a.gc[0] > 0;
+#define LAUNCH(NTH, CH)                                                                                     \
+    do {                                                                                                    \
+        if (cells)                                                                                          \
+            hipLaunchKernelGGL((gcmc_chain_kernel<NTH, CH, true>), dim3(n_chains), dim3(NTH), 0, s, a);  \
+        else                                                                                                \
+            hipLaunchKernelGGL((gcmc_chain_kernel<NTH, CH, false>), dim3(n_chains), dim3(NTH), 0, s, a); \
+    } while (0)
     if (a.charged)
-        hipLaunchKernelGGL((gcmc_chain_kernel<kGThreads, true>), dim3(n_chains), dim3(kGThreads), 0, s, a);
+        LAUNCH(kGThreads, true);
     else if (a.grid)  // per-step work of a few guests: one wave per chain, barriers near free
-        hipLaunchKernelGGL((gcmc_chain_kernel<64, false>), dim3(n_chains), dim3(64), 0, s, a);
+        LAUNCH(64, false);
     else
-        hipLaunchKernelGGL((gcmc_chain_kernel<kGThreads, false>), dim3(n_chains), dim3(kGThreads), 0, s, a);
+        LAUNCH(kGThreads, false);
+#undef LAUNCH
     ALEATOR_HIP_CHECK(hipGetLastError());
 }
 
