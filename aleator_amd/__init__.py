@@ -390,6 +390,17 @@ def lj_tail_insertion_delta(epsilon, sigma, cutoff, lattice, framework_species, 
     return out.value
 
 
+def erf_kernel_poly(alpha: float, rmax: float, tol: float, min_degree: int = 0):
+    """The device's erf(alpha r)/r polynomial: (coefficients lowest first, xs) with
+    erf(alpha r)/r = sum_k c_k x^k, x = r^2 xs - 1, for r in [0, rmax]."""
+    coef = np.zeros(29)
+    xs = C.c_double()
+    deg = C.c_int32()
+    _check(_load().aleator_erf_kernel_poly(C.c_double(alpha), C.c_double(rmax), C.c_double(tol),
+                                           C.c_int32(min_degree), _p(coef), C.byref(xs), C.byref(deg)))
+    return coef[:deg.value + 1].copy(), xs.value
+
+
 def synth_framework(lattice, n_atoms: int, seed=1, stream=0, min_dist=1.0,
                     proportions=(0.45, 0.30, 0.17, 0.08), charge_sigma=0.0, charge_stream=2):
     """Synthetic framework (no reference counterpart): returns (x, y, z, charge, species)."""

@@ -1,0 +1,85 @@
+// erf_poly.cpp -- host fit of the erf kernel polynomial used by the FP64 pair loops (pair.hpp).
+//
+// erf(alpha r)/r = (2 alpha/sqrt(pi)) G(alpha^2 r^2),  G(u) = int_0^1 exp(-u t^2) dt,
+// and G is entire with |G^(m)| <= 1/(2m + 1), so its Chebyshev coefficients on [0, U] fall off
+// like (U/4)^m / (m! (2m + 1)).  The interpolant at 64 Chebyshev points is computed in long
+// double, truncated at the degree asked for, converted to monomials in x = s xs - 1 (s = r^2 in
+// [0, rmax^2]) and rounded to double once.
+#include <cmath>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace aleator_gpu {
+
+namespace {
+
+long double g_kernel(long double u) {
+    if (u < 0.5L) {
+        // sum_n (-u)^n / (n! (2n + 1))
+        long double term = 1.0L, sum = 1.0L;
+        for (int n = 1; n < 40; ++n) {
+            term *= -u / n;
+            const long double add = term / (2 * n + 1);
+            sum += add;
+            if (fabsl(add) < 1e-24L) break;
+        }
+        return sum;
+    }
+    const long double su = sqrtl(u);
+    return erfl(su) * sqrtl(3.141592653589793238462643383279502884L) / (2.0L * su);
+}
+
+}  // namespace
+
+int erf_kernel_poly(double alpha, double rmax, double tol, int min_deg, double* coef, double* xs) {
+    constexpr int kN = 64;
+    constexpr long double kPiL = 3.141592653589793238462643383279502884L;
+    const long double smax = (long double)rmax * rmax;
+    const long double a2 = (long double)alpha * alpha;
+    long double f[kN], cheb[kN];
+    for (int k = 0; k < kN; ++k) {
+        const long double x = cosl(kPiL * (k + 0.5L) / kN);
+        f[k] = g_kernel(a2 * (x + 1.0L) * 0.5L * smax);
+    }
+    for (int j = 0; j < kN; ++j) {
+        long double s = 0.0L;
+        for (int k = 0; k < kN; ++k) s += f[k] * cosl(kPiL * j * (k + 0.5L) / kN);
+        cheb[j] = 2.0L * s / kN;
+    }
+    cheb[0] *= 0.5L;
+    // smallest allowed degree (at least min_deg) whose dropped coefficients sum below tol
+    static const int kDegrees[] = {18, 24, kErfPolyMax};
+    int deg = kErfPolyMax;
+    for (int d : kDegrees) {
+        if (d < min_deg) continue;
+        long double tail = 0.0L;
+        for (int j = d + 1; j < kN; ++j) tail += fabsl(cheb[j]);
+        if (tail <= tol) {
+            deg = d;
+            break;
+        }
+    }
+    // Chebyshev -> monomials: T_{j+1} = 2x T_j - T_{j-1}
+    std::vector<long double> tprev(deg + 1, 0.0L), tcur(deg + 1, 0.0L), mono(deg + 1, 0.0L);
+    tprev[0] = 1.0L;  // T_0
+    mono[0] += cheb[0];
+    if (deg >= 1) {
+        tcur[1] = 1.0L;  // T_1
+        mono[1] += cheb[1];
+    }
+    for (int j = 2; j <= deg; ++j) {
+        std::vector<long double> next(deg + 1, 0.0L);
+        for (int i = 0; i < deg; ++i) next[i + 1] += 2.0L * tcur[i];
+        for (int i = 0; i <= deg; ++i) next[i] -= tprev[i];
+        for (int i = 0; i <= deg; ++i) mono[i] += cheb[j] * next[i];
+        tprev.swap(tcur);
+        tcur.swap(next);
+    }
+    const long double pref = 2.0L * alpha / sqrtl(kPiL);
+    for (int i = 0; i <= deg; ++i) coef[i] = (double)(pref * mono[i]);
+    *xs = (double)(2.0L / smax);
+    return deg;
+}
+
+}  // namespace aleator_gpu

@@ -1,412 +1,30 @@
-// grid.hip -- framework energy grids on gfx950.
+// grid.hip -- framework energy grids on gfx950: host side of the build and the reciprocal
+// pre-contractions.  The tile kernel is in grid_tile.hpp.
 //
 // Reference: FrameworkEnergyGrid constructor body
-// (src/engines/monte_carlo/framework_energy_grid.cc:221-248): for every node,
-// E = min(sum over framework atoms of pairEnergy(|minimum image|, mix), cap).
-//
-// MI355X design.  One workgroup (256 threads) owns a tile of TX x TY x (TZ*NPT) nodes;
-// each thread owns NPT nodes stacked along z, so every LDS entry it reads is reused NPT
-// times from registers.  The workgroup walks the framework atoms in chunks of 128, and for
-// each atom tests the 8 periodic images whose fractional offset from the tile centre lies in
-// [-1,1)^3; every image within (cutoff + tile radius) of the tile centre is appended to an
-// LDS list (structure of arrays: dx, dy, dz relative to the tile centre, charge, type)
-// through a deterministic block scan, in atom order.  Because the cutoff is at most half the
-// smallest perpendicular width, at most one image of an atom lies within the cutoff of any
-// node, so summing every listed image inside the cutoff is the reference's minimum-image
-// sum.  The list is in atom order, so each node adds its terms in atom order and a build is
-// bitwise reproducible run to run.  Every lane reads the same list entry (an LDS broadcast).  The work is VALU-bound: the list
-// is reused by all nodes of the tile and the output is 8 B per node per table.
-//
-// Many frameworks (or one) are built by one launch: workgroup b finds its framework in a
-// prefix table of tile counts and reads that framework's descriptor from device memory.
+// (src/engines/monte_carlo/framework_energy_grid.cc:221-248).
 //
 // Coulomb table: real space as Ewald::realSpaceParticleEnergy of a unit charge
 // (ewald.cc:223-249), reciprocal space as the linear term of computeDelta
 // (ewald_incremental_state.cc:117-121).  Every node sits at fractional coordinates
 // i/(n*rep) of the k-set's lattice, so exp(i k.r) factorises per axis and the sum over the
-// (2K+1)^3 index cube is three small separable contractions in FP64: two pre-kernels, the
+// (2K+1)^3 index cube is three small separable contractions in FP64: two pre-kernels here, the
 // last one fused into the tile kernel.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <type_traits>
 #include <vector>
 
 #include "device.hpp"
 #include "grid.hpp"
+#include "grid_tile.hpp"
 #include "internal.hpp"
 
 namespace aleator_gpu {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kCap = 1024;   // LDS list capacity (entries)
-// atoms staged per pass of the list build: 128 (each may add 8 images, 1024 <= kCap), or, when
-// only the nearest image can be within the search radius, 256, 512 or 1024 (1, 2 or 4 atoms per
-// thread); the plan's autotuner picks it with the tile shape
-constexpr int kChunkSmall = 128;
-constexpr int kErfcIntervals = 256;  // FP32 erfc(alpha r) table: cubic Hermite pieces over [0, alpha rc]
-
-// LDS image list: position, charge and compact LJ type of each atom image.  FP64 keeps
-// structure-of-arrays; FP32 packs (x, y, z, q) in one float4 so an entry is two LDS reads.
-template <typename R>
-struct Entries {
-    R x[kCap], y[kCap], z[kCap], q[kCap];
-    int t[kCap];
-    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_, bool) {
-        x[w] = (R)x_;
-        y[w] = (R)y_;
-        z[w] = (R)z_;
-        q[w] = (R)q_;
-        t[w] = t_;
-    }
-};
-template <>
-struct Entries<float> {
-    float4 p[kCap];
-    int t[kCap];
-    // with_charge false: the fourth word carries the type bits instead of the charge
-    __device__ void put(int w, double x_, double y_, double z_, double q_, int t_, bool with_charge) {
-        p[w] = make_float4((float)x_, (float)y_, (float)z_, with_charge ? (float)q_ : __int_as_float(t_));
-        t[w] = t_;
-    }
-};
-
-template <typename R>
-__device__ inline R rcp(R v);
-template <>
-__device__ inline double rcp<double>(double v) {
-    return 1.0 / v;
-}
-template <>
-__device__ inline float rcp<float>(float v) {
-    return __builtin_amdgcn_rcpf(v);
-}
-
-// LFS: some framework of the launch uses linear-force-shifted LJ (FP32 path: the slope term is
-// compiled in; the FP64 path reads the truncation per framework at run time).
-template <typename R, int NG, bool COUL, bool LFS, int TX, int TY, int TZ, int NPT>
-__global__ __launch_bounds__(kThreads) void grid_tile_kernel(const GridArgs* __restrict__ descs,
-                                                             const int* __restrict__ prefix, int n_desc) {
-    static_assert(TX * TY * TZ == kThreads, "one thread per (x, y, z-group)");
-    constexpr bool kF32 = std::is_same_v<R, float>;
-    constexpr int NZ = TZ * NPT;  // nodes per tile along z
-    __shared__ Entries<R> ent;
-    __shared__ double tab[4][kMaxGuests][kMaxTypes];
-    // FP32 pair constants per (guest, type): A = 4 eps sigma^12, B = 4 eps sigma^6, the shift
-    // V(rc) - rc V'(rc) and the slope V'(rc) (shift 0 when truncated, slope 0 unless
-    // linear-force-shifted): V = A/r^12 - (B/r^6 + shift + r slope), the slope term only with LFS
-    __shared__ float4 tabf[kMaxGuests][kMaxTypes];
-    __shared__ float4 erfc_tab[(std::is_same_v<R, float> && COUL) ? kErfcIntervals : 1];
-    __shared__ int scan_tmp[kThreads / 64 + 1];
-
-    // framework of this workgroup
-    const int b = blockIdx.x;
-    int lo = 0, hi = n_desc - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const GridArgs& a = descs[lo];
-    const int tile = b - prefix[lo];
-
-    const int tid = threadIdx.x;
-    if constexpr (kF32) {
-        constexpr int kPairs = kMaxGuests * kMaxTypes;
-        for (int i = tid; i < kPairs; i += kThreads) {
-            const double s2 = a.ljtab[i], e4 = a.ljtab[kPairs + i];
-            const double s6 = s2 * s2 * s2;
-            const double vrc = a.trunc != ALEATOR_LJ_TRUNCATED ? a.ljtab[2 * kPairs + i] : 0.0;
-            const double slope = a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED ? a.ljtab[3 * kPairs + i] : 0.0;
-            (&tabf[0][0])[i] =
-                make_float4((float)(e4 * s6 * s6), (float)(e4 * s6), (float)(vrc - a.rc_lj * slope), (float)slope);
-        }
-    } else {
-        for (int i = tid; i < 4 * kMaxGuests * kMaxTypes; i += kThreads) (&tab[0][0][0])[i] = a.ljtab[i];
-    }
-
-    // erfc(alpha r) for the FP32 pair loop: piece k covers z = alpha r in [k h, (k+1) h],
-    // h = alpha rc / (kErfcIntervals - 1), as the cubic Hermite interpolant of the FP64 erfc and
-    // its derivative -2/sqrt(pi) exp(-z^2) at both ends, in powers of the fraction of the piece.
-    // Interpolation error below 1.3e-7 relative on [0, 3.2] (emulated against the FP64 erfc).
-    float erfc_scale = 0.0f;  // r -> piece coordinate u = z / h
-    if constexpr (kF32 && COUL) {
-        static_assert(kErfcIntervals == kThreads, "one piece per thread");
-        const double h = a.alpha * sqrt(a.rc2_real) / (kErfcIntervals - 1);
-        const double z0 = tid * h, z1 = z0 + h;
-        const double f0 = erfc(z0), f1 = erfc(z1);
-        const double d0 = -1.1283791670955126 * exp(-z0 * z0) * h, d1 = -1.1283791670955126 * exp(-z1 * z1) * h;
-        erfc_tab[tid] = make_float4((float)f0, (float)d0, (float)(3.0 * (f1 - f0) - (2.0 * d0 + d1)),
-                                    (float)(2.0 * (f0 - f1) + (d0 + d1)));
-        erfc_scale = (float)(a.alpha / h);
-    }
-
-    const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
-    const int t0 = (n0 + TX - 1) / TX, t1 = (n1 + TY - 1) / TY;
-    const int bx = tile % t0;
-    const int by = (tile / t0) % t1;
-    const int bz = tile / (t0 * t1);
-    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
-    const int ix = bx * TX + tx, iy = by * TY + ty, iz0 = bz * NZ + tz * NPT;
-
-    // tile centre in grid and image fractional coordinates
-    const double cg0 = (bx * TX + 0.5 * (TX - 1)) / n0;
-    const double cg1 = (by * TY + 0.5 * (TY - 1)) / n1;
-    const double cg2 = (bz * NZ + 0.5 * (NZ - 1)) / n2;
-    const double ci0 = cg0 / a.rep[0], ci1 = cg1 / a.rep[1], ci2 = cg2 / a.rep[2];
-    // node offsets from the tile centre (Cartesian)
-    R ox[NPT], oy[NPT], oz[NPT];
-    bool valid[NPT];
-    const double d0 = (tx - 0.5 * (TX - 1)) / n0;
-    const double d1 = (ty - 0.5 * (TY - 1)) / n1;
-#pragma unroll
-    for (int p = 0; p < NPT; ++p) {
-        const double d2 = (tz * NPT + p - 0.5 * (NZ - 1)) / n2;
-        ox[p] = (R)(d0 * a.mg[0] + d1 * a.mg[3] + d2 * a.mg[6]);
-        oy[p] = (R)(d0 * a.mg[1] + d1 * a.mg[4] + d2 * a.mg[7]);
-        oz[p] = (R)(d0 * a.mg[2] + d1 * a.mg[5] + d2 * a.mg[8]);
-        valid[p] = ix < n0 && iy < n1 && iz0 + p < n2;
-    }
-    const bool any_valid = valid[0];
-    // node p of a thread sits p steps s (one grid step along the third lattice row) past node 0,
-    // so |e - o_p|^2 = |e - o_0|^2 - 2p (e - o_0).s + p^2 |s|^2: two instructions per extra node
-    // instead of six, and no cancellation because |s| is a grid spacing
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f, s2p[NPT], m2p[NPT];
-    if constexpr (kF32) {
-        const double gx = a.mg[6] / n2, gy = a.mg[7] / n2, gz = a.mg[8] / n2;
-        sx = (float)gx;
-        sy = (float)gy;
-        sz = (float)gz;
-#pragma unroll
-        for (int p = 0; p < NPT; ++p) {
-            s2p[p] = (float)((double)(p * p) * (gx * gx + gy * gy + gz * gz));
-            m2p[p] = (float)(-2 * p);
-        }
-    }
-
-    double acc[NPT][NG > 0 ? NG : 1];
-    double accq[NPT];
-#pragma unroll
-    for (int p = 0; p < NPT; ++p) {
-        accq[p] = 0.0;
-#pragma unroll
-        for (int g = 0; g < (NG > 0 ? NG : 1); ++g) acc[p][g] = 0.0;
-    }
-
-    const R rc2 = (R)a.rc2_lj;
-    const R rcr2 = (R)a.rc2_real;
-    const R alpha = (R)a.alpha;
-    const R rc = (R)a.rc_lj;
-    const int trunc = a.trunc;
-
-    auto compute = [&](int count) {
-        __syncthreads();
-        if constexpr (kF32) {
-            // 1/r from one reciprocal square root: r = r2 / sqrt(r2), 1/r^2 = (1/r)^2.  When the LJ
-            // and real-space cutoffs are equal (or only one term is built) one branch covers both
-            // terms; otherwise each term has its own branch.
-            auto pairs = [&](auto same_cutoffs) {
-                constexpr bool kOne = decltype(same_cutoffs)::value;
-#pragma unroll 2
-                for (int e = 0; e < count; ++e) {
-                    const float4 P = ent.p[e];
-                    // without a Coulomb table the entry's fourth word holds its type (one LDS read)
-                    const int t = COUL ? ent.t[e] : __float_as_int(P.w);
-                    float4 cg[NG > 0 ? NG : 1];
-#pragma unroll
-                    for (int g = 0; g < NG; ++g) cg[g] = tabf[g][t];
-                    const float dx = P.x - ox[0];
-                    const float dy = P.y - oy[0];
-                    const float dz = P.z - oz[0];
-                    const float r20 = dx * dx + dy * dy + dz * dz;
-                    const float tds = NPT > 1 ? dx * sx + dy * sy + dz * sz : 0.0f;
-#pragma unroll
-                    for (int p = 0; p < NPT; ++p) {
-                        const float r2 = p == 0 ? r20 : fmaf(tds, m2p[p], r20 + s2p[p]);
-                        // LJ from 1/r^2 (and r, linear-force-shifted only)
-                        auto lj = [&](float ir2, float r) {
-                            const float ir6 = ir2 * ir2 * ir2;
-                            const float ir12 = ir6 * ir6;
-#pragma unroll
-                            for (int g = 0; g < NG; ++g) {
-                                float w = fmaf(cg[g].y, ir6, cg[g].z);
-                                if constexpr (LFS) w = fmaf(r, cg[g].w, w);
-                                acc[p][g] += (double)fmaf(cg[g].x, ir12, -w);
-                            }
-                        };
-                        auto coul = [&](float ir) {
-                            // r <= rc(1 + 2^-22) here, so 0 <= u < kErfcIntervals and the piece
-                            // index (int)u = floor(u) stays inside the table
-                            const float u = (r2 * ir) * erfc_scale;
-                            const float f = __builtin_amdgcn_fractf(u);
-                            const float4 c = erfc_tab[(int)u];
-                            const float erfcz = fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
-                            accq[p] += (double)(P.w * (erfcz * ir));
-                        };
-                        if constexpr (kOne) {
-                            if (r2 <= (NG > 0 ? rc2 : rcr2)) {
-                                if constexpr (!COUL && !LFS) {
-                                    lj(__builtin_amdgcn_rcpf(r2), 0.0f);  // LJ alone needs no 1/r
-                                } else {
-                                    const float ir = __builtin_amdgcn_rsqf(r2);
-                                    if constexpr (NG > 0) lj(ir * ir, r2 * ir);
-                                    if constexpr (COUL) coul(ir);
-                                }
-                            }
-                        } else {
-                            const float ir = __builtin_amdgcn_rsqf(r2);
-                            if (r2 <= rc2) lj(ir * ir, r2 * ir);
-                            if (r2 <= rcr2) coul(ir);
-                        }
-                    }
-                }
-            };
-            if (any_valid) {
-                if (NG == 0 || !COUL || rc2 == rcr2)
-                    pairs(std::true_type{});
-                else
-                    pairs(std::false_type{});
-            }
-        } else if (any_valid) {
-#pragma unroll 2
-            for (int e = 0; e < count; ++e) {
-                const R ex = ent.x[e], ey = ent.y[e], ez = ent.z[e];
-                const int t = ent.t[e];
-                R s2c[NG > 0 ? NG : 1], e4c[NG > 0 ? NG : 1];
-#pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    s2c[g] = (R)tab[0][g][t];
-                    e4c[g] = (R)tab[1][g][t];
-                }
-#pragma unroll
-                for (int p = 0; p < NPT; ++p) {
-                    const R dx = ex - ox[p];
-                    const R dy = ey - oy[p];
-                    const R dz = ez - oz[p];
-                    const R r2 = dx * dx + dy * dy + dz * dz;
-                    if (NG > 0 && r2 <= rc2) {
-                        const R ir2 = rcp<R>(r2);
-                        R r = 0;
-                        if (trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED) r = sqrt(r2);
-#pragma unroll
-                        for (int g = 0; g < NG; ++g) {
-                            const R s2 = s2c[g] * ir2;
-                            const R s6 = s2 * s2 * s2;
-                            R v = e4c[g] * (s6 * s6 - s6);
-                            if (trunc == ALEATOR_LJ_SHIFTED) {
-                                v -= (R)tab[2][g][t];
-                            } else if (trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED) {
-                                v = v - (R)tab[2][g][t] - (r - rc) * (R)tab[3][g][t];
-                            }
-                            acc[p][g] += (double)v;
-                        }
-                    }
-                    if (COUL && r2 <= rcr2) {
-                        const R r = sqrt(r2);
-                        accq[p] += (double)(ent.q[e] * erfc_over_r<R>(alpha, r));
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    };
-
-    int pending = 0;
-    const int natoms = a.natoms;
-    const int chunk = a.chunk;
-    const int apt = chunk > kThreads ? chunk / kThreads : 1;  // atoms per thread and pass (contiguous,
-                                                              // so the list stays in atom order)
-    for (int base = 0; base < natoms; base += chunk) {
-        int cnt = 0;
-        unsigned mask[4] = {0u, 0u, 0u, 0u};  // images of each atom within the search radius
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int at = base + tid * apt + j;
-            if (j >= apt || tid * apt >= chunk || at >= natoms) continue;
-            double h0 = a.fx[at] - ci0, h1 = a.fy[at] - ci1, h2 = a.fz[at] - ci2;
-            h0 -= floor(h0);
-            h1 -= floor(h1);
-            h2 -= floor(h2);
-            if (a.single_image) {
-                // the search radius is below L_perp/2: only the nearest image (fractional
-                // offsets in [-1/2, 1/2)) can be within it
-                const int s = (h0 >= 0.5 ? 1 : 0) | (h1 >= 0.5 ? 2 : 0) | (h2 >= 0.5 ? 4 : 0);
-                const double g0 = h0 - (s & 1), g1 = h1 - ((s >> 1) & 1), g2 = h2 - (s >> 2);
-                const double cx = g0 * a.mi[0] + g1 * a.mi[3] + g2 * a.mi[6];
-                const double cy = g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7];
-                const double cz = g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8];
-                if (cx * cx + cy * cy + cz * cz <= a.search2) {
-                    mask[j] = 1u << s;
-                    ++cnt;
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    const double g0 = h0 - (s & 1), g1 = h1 - ((s >> 1) & 1), g2 = h2 - (s >> 2);
-                    const double cx = g0 * a.mi[0] + g1 * a.mi[3] + g2 * a.mi[6];
-                    const double cy = g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7];
-                    const double cz = g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8];
-                    if (cx * cx + cy * cy + cz * cz <= a.search2) {
-                        mask[j] |= 1u << s;
-                        ++cnt;
-                    }
-                }
-            }
-        }
-        int total;
-        const int off = block_exclusive_scan(cnt, scan_tmp, &total);
-        if (pending + total > kCap) {
-            compute(pending);
-            pending = 0;
-        }
-        int w = pending + off;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (!mask[j]) continue;
-            const int at = base + tid * apt + j;
-            const double qa = a.q[at];
-            const int ta = a.type[at];
-            // the wrapped offsets again (not kept across the scan, to spare registers)
-            double h0 = a.fx[at] - ci0, h1 = a.fy[at] - ci1, h2 = a.fz[at] - ci2;
-            h0 -= floor(h0);
-            h1 -= floor(h1);
-            h2 -= floor(h2);
-            for (int s = 0; s < 8; ++s) {
-                if (!(mask[j] & (1u << s))) continue;
-                const double g0 = h0 - (s & 1), g1 = h1 - ((s >> 1) & 1), g2 = h2 - (s >> 2);
-                ent.put(w, g0 * a.mi[0] + g1 * a.mi[3] + g2 * a.mi[6], g0 * a.mi[1] + g1 * a.mi[4] + g2 * a.mi[7],
-                        g0 * a.mi[2] + g1 * a.mi[5] + g2 * a.mi[8], qa, ta, COUL);
-                ++w;
-            }
-        }
-        pending += total;
-    }
-    compute(pending);
-
-    const int nk = 2 * a.kmax + 1;
-#pragma unroll
-    for (int p = 0; p < NPT; ++p) {
-        if (!valid[p]) continue;
-        const int iz = iz0 + p;
-        const long long flat = ((long long)iz * n1 + iy) * n0 + ix;
-#pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            const double e = acc[p][g];
-            a.out[g * a.stride + flat] = (a.cap < e) ? a.cap : e;  // std::min(e, cap), NaN kept
-        }
-        if (COUL) {
-            double rec = 0.0;
-            for (int m = 0; m < nk; ++m) {
-                const double2 tt = a.t2[((long long)m * n2 + iz) * n1 + iy];
-                const double2 ee = a.e0[(long long)m * n0 + ix];
-                rec += tt.x * ee.x - tt.y * ee.y;
-            }
-            a.out[NG * a.stride + flat] = a.kc * accq[p] + a.recip_pref * rec;
-        }
-    }
-}
+constexpr int kChunkSmall = 128;  // atoms per list pass when an atom may add 8 images
 
 // T1[n0][n1][iz] = sum_n2 C[n0][n1][n2] * E2[n2][iz]
 __global__ void recip_stage_a(const double2* c, const double2* e2, int nk, int nz, double2* t1) {
@@ -439,51 +57,6 @@ __global__ void recip_stage_b(const double2* t1, const double2* e1, int nk, int 
         im += a.x * b.y + a.y * b.x;
     }
     t2[idx] = make_double2(re, im);
-}
-
-// ---------------------------------------------------------------- tile shapes
-struct TileShape {
-    int tx, ty, tz, npt;
-};
-constexpr TileShape kShapes[kNumTileShapes] = {{8, 8, 4, 1}, {8, 8, 4, 2}, {8, 8, 4, 4}, {4, 8, 8, 2}};
-
-template <typename R, int NG, bool COUL, bool LFS>
-void launch_shape(int shape, const GridArgs* d, const int* prefix, int n_desc, int blocks, hipStream_t s) {
-    switch (shape) {
-#define SHAPE(I, X, Y, Z, P)                                                                                   \
-    case I:                                                                                                    \
-        hipLaunchKernelGGL((grid_tile_kernel<R, NG, COUL, LFS, X, Y, Z, P>), dim3(blocks), dim3(kThreads), 0, s, \
-                           d, prefix, n_desc);                                                                 \
-        return;
-        SHAPE(0, 8, 8, 4, 1)
-        SHAPE(1, 8, 8, 4, 2)
-        SHAPE(2, 8, 8, 4, 4)
-        SHAPE(3, 4, 8, 8, 2)
-#undef SHAPE
-    }
-    fail(ALEATOR_INVALID_ARGUMENT, "grid: unknown tile shape");
-}
-
-template <typename R, bool LFS>
-void dispatch(int shape, int ng, bool coul, const GridArgs* d, const int* prefix, int n_desc, int blocks,
-              hipStream_t s) {
-    if (coul) {
-        switch (ng) {
-            case 0: return launch_shape<R, 0, true, LFS>(shape, d, prefix, n_desc, blocks, s);
-            case 1: return launch_shape<R, 1, true, LFS>(shape, d, prefix, n_desc, blocks, s);
-            case 2: return launch_shape<R, 2, true, LFS>(shape, d, prefix, n_desc, blocks, s);
-            case 3: return launch_shape<R, 3, true, LFS>(shape, d, prefix, n_desc, blocks, s);
-            case 4: return launch_shape<R, 4, true, LFS>(shape, d, prefix, n_desc, blocks, s);
-        }
-    } else {
-        switch (ng) {
-            case 1: return launch_shape<R, 1, false, LFS>(shape, d, prefix, n_desc, blocks, s);
-            case 2: return launch_shape<R, 2, false, LFS>(shape, d, prefix, n_desc, blocks, s);
-            case 3: return launch_shape<R, 3, false, LFS>(shape, d, prefix, n_desc, blocks, s);
-            case 4: return launch_shape<R, 4, false, LFS>(shape, d, prefix, n_desc, blocks, s);
-        }
-    }
-    fail(ALEATOR_INVALID_ARGUMENT, "grid: unsupported guest count");
 }
 
 double norm3(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
@@ -612,7 +185,16 @@ void GridJob::prepare(Framework& fw, const GridRequest& req) {
         off_t2 = bytes;
         bytes += align256((size_t)nk * req.n[2] * req.n[1] * sizeof(double2));
         a.recip_pref = kCoulomb * (4.0 * kPi / std::abs(fw.lat.volume));
+        fit_poly(0);
     }
+}
+
+void GridJob::fit_poly(int min_deg) {
+    if (!coulomb) return;
+    // FP64: the dropped part of the erf kernel below the last bits of a term; MIXED: below 1e-12
+    // of it (a term of 3e4 K then carries under 3e-8 K)
+    const double tol = precision == ALEATOR_PRECISION_FP64 ? 1e-17 : 1e-12;
+    deg = erf_kernel_poly(args.alpha, std::sqrt(args.rc2_real), tol, min_deg, args.poly, &args.poly_xs);
 }
 
 double GridJob::tile_radius(int shape) const {
@@ -644,6 +226,12 @@ void GridJob::set_shape(int s, int chunk) {
         fail(ALEATOR_INVALID_ARGUMENT, "grid: cutoff plus tile radius exceeds the perpendicular width");
     const double sr = rmax + tile_radius(s) + 1e-6 * (1.0 + rmax);
     args.search2 = sr * sr;
+    // images this close to the tile centre are inside both cutoffs for every node of the tile
+    double rmin = 1e300;
+    if (ng > 0) rmin = std::min(rmin, args.rc_lj);
+    if (coulomb) rmin = std::min(rmin, std::sqrt(args.rc2_real));
+    const double inner = rmin - tile_radius(s) - 1e-9 * (1.0 + rmin);
+    args.inner2 = inner > 0.0 ? inner * inner : -1.0;
     // below half the smallest perpendicular width at most one image of an atom, the nearest,
     // is within the search radius (the fractional rounding image is then the nearest one)
     args.single_image = (2.0 * sr < std::min(lperp[0], std::min(lperp[1], lperp[2]))) ? 1 : 0;
@@ -701,14 +289,15 @@ void GridJob::launch_pre(hipStream_t s) const {
                        args.n[2], d_t2);
 }
 
-void launch_tiles(int precision, int shape, int ng, bool coulomb, bool lfs, const GridArgs* d_descs,
-                  const int* d_prefix, int n_desc, int blocks, hipStream_t s) {
-    if (precision == ALEATOR_PRECISION_FP64)
-        dispatch<double, false>(shape, ng, coulomb, d_descs, d_prefix, n_desc, blocks, s);
-    else if (lfs)
-        dispatch<float, true>(shape, ng, coulomb, d_descs, d_prefix, n_desc, blocks, s);
-    else
-        dispatch<float, false>(shape, ng, coulomb, d_descs, d_prefix, n_desc, blocks, s);
+void launch_tiles(int precision, int shape, int ng, int deg, bool lfs, const GridArgs* d_descs, const int* d_prefix,
+                  int n_desc, int blocks, hipStream_t s) {
+    if (precision == ALEATOR_PRECISION_FP64) {
+        if (lfs) launch_tiles_fp64_lfs(shape, ng, deg, d_descs, d_prefix, n_desc, blocks, s);
+        else launch_tiles_fp64(shape, ng, deg, d_descs, d_prefix, n_desc, blocks, s);
+    } else {
+        if (lfs) launch_tiles_mixed_lfs(shape, ng, deg, d_descs, d_prefix, n_desc, blocks, s);
+        else launch_tiles_mixed(shape, ng, deg, d_descs, d_prefix, n_desc, blocks, s);
+    }
 }
 
 // single build through the context scratch buffer (the per-call entry points)
@@ -727,7 +316,7 @@ void launch_grid(Framework& fw, const GridRequest& req) {
     ALEATOR_HIP_CHECK(hipMemcpyAsync(base + off_prefix, prefix, sizeof(prefix), hipMemcpyHostToDevice, ctx.stream));
     job.launch_pre(ctx.stream);
     ctx.time_begin();
-    launch_tiles(job.precision, job.shape_used(), job.ng, job.coulomb, job.args.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED,
+    launch_tiles(job.precision, job.shape_used(), job.ng, job.deg, job.args.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED,
                  reinterpret_cast<const GridArgs*>(base + off_desc),
                  reinterpret_cast<const int*>(base + off_prefix), 1, job.blocks, ctx.stream);
     ctx.time_end();

@@ -35,6 +35,10 @@ struct GridArgs {
     double recip_pref;
     double* out;
     long long stride;
+    double inner2;                    // images within sqrt(inner2) of the tile centre are inside
+                                      // every node's cutoffs (-1: none)
+    double poly[kErfPolyMax + 1];     // erf(alpha r)/r in x = r^2 poly_xs - 1 (pair.hpp)
+    double poly_xs;
 };
 
 // One framework's build: host-side constants, their device placement, pre-kernels.
@@ -43,6 +47,7 @@ struct GridJob {
     int ng = 0;
     bool coulomb = false;
     int precision = 0;
+    int deg = 0;  // erf polynomial degree (0 without a Coulomb table)
     int blocks = 0;
     int shape = 0;
     int nk = 0;
@@ -53,6 +58,7 @@ struct GridJob {
     char* dev_base = nullptr;
 
     void prepare(Framework& fw, const GridRequest& req);
+    void fit_poly(int min_deg);                 // erf polynomial of degree >= min_deg (Coulomb only)
     double tile_radius(int shape) const;
     bool shape_fits(int shape) const;
     void set_shape(int s, int chunk);           // chunk: atoms per list-build pass
@@ -64,8 +70,18 @@ struct GridJob {
 };
 
 int tile_blocks(const int32_t n[3], int shape);
-// lfs: some framework of the launch uses linear-force-shifted LJ
-void launch_tiles(int precision, int shape, int ng, bool coulomb, bool lfs, const GridArgs* d_descs,
-                  const int* d_prefix, int n_desc, int blocks, hipStream_t s);
+// deg: erf polynomial degree of every framework of the launch (0: no Coulomb table); lfs: some
+// framework of the launch uses linear-force-shifted LJ
+void launch_tiles(int precision, int shape, int ng, int deg, bool lfs, const GridArgs* d_descs, const int* d_prefix,
+                  int n_desc, int blocks, hipStream_t s);
+// the tile kernels of each precision and LJ truncation kind (grid_tile_*.hip)
+void launch_tiles_fp64(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
+                       hipStream_t s);
+void launch_tiles_fp64_lfs(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
+                           hipStream_t s);
+void launch_tiles_mixed(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
+                        hipStream_t s);
+void launch_tiles_mixed_lfs(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
+                            hipStream_t s);
 
 }  // namespace aleator_gpu

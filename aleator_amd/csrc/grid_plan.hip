@@ -62,7 +62,7 @@ void plan_launch(aleator_grid_plan* p) {
     const GridJob& j0 = p->jobs.front();
     bool lfs = false;
     for (const auto& j : p->jobs) lfs = lfs || j.args.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED;
-    launch_tiles(j0.precision, p->shape, j0.ng, j0.coulomb, lfs, p->d_descs, p->d_prefix, (int)p->jobs.size(),
+    launch_tiles(j0.precision, p->shape, j0.ng, j0.deg, lfs, p->d_descs, p->d_prefix, (int)p->jobs.size(),
                  p->total_blocks, s);
 }
 
@@ -111,6 +111,11 @@ aleator_status aleator_gpu_grid_plan_create(aleator_ctx* ctx_in, int32_t n_frame
             offs[f] = total;
             total += align256(p->jobs[f].bytes);
         }
+        // one erf polynomial degree for the whole launch: the largest any framework needs
+        int deg = 0;
+        for (const auto& j : p->jobs) deg = std::max(deg, j.deg);
+        for (auto& j : p->jobs)
+            if (j.deg != deg) j.fit_poly(deg);
         const size_t off_desc = total;
         total += align256((size_t)n_frameworks * sizeof(GridArgs));
         const size_t off_prefix = total;

@@ -7,6 +7,7 @@
 
 #include "aleator_gpu.h"
 #include "device.hpp"
+#include "internal.hpp"
 
 #pragma clang fp contract(off)
 
@@ -163,6 +164,15 @@ aleator_status aleator_lj_tail_insertion_delta(const aleator_lj_params* lj, cons
     }
     const double vol = volume(lattice->m);
     *delta = tail_energy(lj, after, vol) - tail_energy(lj, before, vol);
+    return ALEATOR_OK;
+}
+
+aleator_status aleator_erf_kernel_poly(double alpha, double rmax, double tol, int32_t min_degree, double* coef,
+                                       double* xs, int32_t* degree) {
+    if (!coef || !xs || !degree || !(alpha > 0.0) || !(rmax > 0.0) || !(tol > 0.0))
+        return ALEATOR_INVALID_ARGUMENT;
+    for (int k = 0; k <= aleator_gpu::kErfPolyMax; ++k) coef[k] = 0.0;
+    *degree = aleator_gpu::erf_kernel_poly(alpha, rmax, tol, min_degree, coef, xs);
     return ALEATOR_OK;
 }
 

@@ -63,6 +63,13 @@ constexpr int kMaxK = 16;           // kmax bound for the per-axis phase tables
 constexpr int kSegLen = 8;          // k-set run length (reciprocal Widom kernel)
 constexpr double kCoulomb = 167100.9468982874;  // ewald.hpp:62
 constexpr double kPi = 3.14159265358979323846;
+constexpr int kErfPolyMax = 28;     // highest degree of the erf kernel polynomial (pair.hpp)
+
+// Coefficients (lowest first) of erf(alpha r)/r as a polynomial in x = r^2 xs - 1 over
+// r in [0, rmax] (erf_poly.cpp).  The degree is the smallest of 18, 24 and 28, at least min_deg,
+// whose dropped Chebyshev coefficients of int_0^1 exp(-alpha^2 r^2 t^2) dt sum below tol (28
+// when none does); it is returned, and coef receives degree + 1 values.
+int erf_kernel_poly(double alpha, double rmax, double tol, int min_deg, double* coef, double* xs);
 
 // ---------------------------------------------------------------- host lattice
 // Row-vector lattice (core/geometry/lattice.hpp:21-30) with the reciprocal-vector

@@ -1,0 +1,14 @@
+// widom_fp64.hip -- Widom direct-sum kernels for FP64 precision (widom.hpp).
+#include <type_traits>
+
+#include "widom.hpp"
+
+namespace aleator_gpu {
+
+void widom_direct_fp64(const widom_detail::WidomArgs& a, const widom_detail::CellArgs* c, unsigned n_cells,
+                       bool wide, unsigned blocks, int deg, bool lfs, hipStream_t s) {
+    if (lfs) widom_detail::launch_direct<true, true>(a, c, n_cells, wide, blocks, deg, s);
+    else widom_detail::launch_direct<true, false>(a, c, n_cells, wide, blocks, deg, s);
+}
+
+}  // namespace aleator_gpu

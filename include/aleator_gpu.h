@@ -441,6 +441,14 @@ aleator_status aleator_lj_tail_insertion_delta(const aleator_lj_params* lj, cons
                                                const aleator_molecule_site* sites, int32_t n_sites,
                                                double* delta);
 
+/* The erf kernel of the FP64 pair loops (no reference counterpart: it is how the device evaluates
+ * Ewald::realSpaceParticleEnergy's erfc(alpha r)/r, ewald.cc:223-249, as 1/r - erf(alpha r)/r):
+ * erf(alpha r)/r = sum_k coef[k] x^k, x = r^2 xs - 1, over r in [0, rmax], with the smallest
+ * degree of 18, 24 and 28 (at least min_degree) whose dropped Chebyshev terms sum below tol.
+ * coef holds 29 doubles; *degree receives the degree. */
+aleator_status aleator_erf_kernel_poly(double alpha, double rmax, double tol, int32_t min_degree, double* coef,
+                                       double* xs, int32_t* degree);
+
 /* ---- structure files: CIF reader (host only) ---------------------------- */
 /* io::readCif (io/structure_io.hpp:39-50, structure_io.cc:173-250): cell from _cell_length_*
  * and _cell_angle_* (a along x, b in the xy plane), symmetry operations from the first loop

@@ -94,3 +94,49 @@ def test_device_calls_without_a_context_are_rejected():
     lib.aleator_gpu_last_error.restype = C.c_char_p
     assert b"null" in lib.aleator_gpu_last_error()
     assert lib.aleator_gpu_ewald_state_propose(None, None, 0, None, 0, None) == 1
+
+
+def _fma_horner(c, x):
+    """Horner's rule with each step one correctly rounded fused multiply-add (as the device does)."""
+    import mpmath
+    mpmath.mp.prec = 200
+    p = mpmath.mpf(c[-1])
+    out = float(p)
+    for k in range(len(c) - 2, -1, -1):
+        out = float(mpmath.mpf(out) * mpmath.mpf(x) + mpmath.mpf(c[k]))
+    return out
+
+
+@pytest.mark.parametrize("alpha,rc,tol,deg", [(0.265, 12.0, 1e-17, 24), (0.265, 12.0, 1e-12, 18),
+                                              (0.2, 10.0, 1e-17, 18), (0.3, 13.3, 1e-17, 28)])
+def test_erf_kernel_polynomial_against_mpmath(alpha, rc, tol, deg):
+    """erf(alpha r)/r from the polynomial the device evaluates, against 50-digit mpmath.  Pins the
+    Chebyshev fit, its degree choice and the monomial conversion: the FP64 bound is a few ulps of
+    2 alpha/sqrt(pi) (the value at r = 0), the MIXED bound its 1e-12."""
+    import mpmath
+    mpmath.mp.dps = 50
+    c, xs = aa.erf_kernel_poly(alpha, rc, tol)
+    assert len(c) == deg + 1
+    scale = 2 * alpha / np.sqrt(np.pi)
+    worst = 0.0
+    for r in np.concatenate([np.linspace(0.0, rc, 241), [1e-9, 0.5 * rc, rc]]):
+        s = r * r
+        x = float(mpmath.mpf(s) * xs - 1)  # fma(s, xs, -1), one rounding
+        got = _fma_horner(c, x)
+        ref = scale if r == 0 else float(mpmath.erf(alpha * mpmath.mpf(r)) / mpmath.mpf(r))
+        worst = max(worst, abs(got - ref) / scale)
+    bound = 8e-16 if tol <= 1e-17 else 2e-12
+    assert worst <= bound, worst
+
+
+def test_erf_kernel_polynomial_planted_mistake_fails():
+    """A dropped highest term (the degree cut one step early) must fail the FP64 bound."""
+    import mpmath
+    mpmath.mp.dps = 50
+    c, xs = aa.erf_kernel_poly(0.265, 12.0, 1e-17)
+    c = c[:-2]
+    scale = 2 * 0.265 / np.sqrt(np.pi)
+    r = 11.5
+    got = _fma_horner(c, float(np.float64(r * r) * np.float64(xs) - 1.0))
+    ref = float(mpmath.erf(0.265 * mpmath.mpf(r)) / mpmath.mpf(r))
+    assert abs(got - ref) / scale > 8e-16
