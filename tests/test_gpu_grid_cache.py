@@ -1,7 +1,9 @@
 """Device FrameworkEnergyGrid::interpolate (framework_energy_grid.cc:251-284) against the oracle's
 restatement (itself equal to the reference's interpolate through the cache-file tests), and a
-device-built grid written in cache format v1, loaded by the reference's own loader, whose
-interpolate then equals the device's bit for bit."""
+device-built grid written in cache format v1 and loaded back bit for bit, whose interpolate
+equals the oracle's on the loaded table.  That files written by this writer load in the
+reference's own loader bit for bit is tested on the CPU (test_grid_cache.py), where the compiled
+reference exists."""
 import ctypes as C
 import tempfile
 
@@ -12,7 +14,7 @@ import aleator_amd as aa
 import oracle_lib as ol
 from gpu_helpers import device_framework
 from oracle_lib import i32p, ptr
-from test_grid_cache import key, ref_load_interpolate, system
+from test_grid_cache import key, system
 
 pytestmark = pytest.mark.gpu
 
@@ -38,8 +40,8 @@ def test_device_interpolate_equals_oracle(ctx, triclinic):
     assert np.array_equal(got, oracle_interpolate(fw.m, counts, table, pos))
 
 
-def test_device_grid_round_trips_through_the_reference_loader(ctx):
-    print("device grid -> cache v1 -> reference loader", flush=True)
+def test_device_grid_round_trips_through_the_cache_file(ctx):
+    print("device grid -> cache v1 -> loader", flush=True)
     fw, lj = system(triclinic=False)  # the device requires rc <= L_perp / 2
     dev = device_framework(ctx, fw, lj)
     counts = aa.grid_point_counts(fw.m, 0.7)
@@ -49,10 +51,12 @@ def test_device_grid_round_trips_through_the_reference_loader(ctx):
     pos = rng.uniform(-10.0, 25.0, size=(1000, 3))
     with tempfile.TemporaryDirectory() as d:
         aa.grid_cache_serialize(k.file_path(d), k, counts, [4, 5], tables)
-        for slot, guest in enumerate([4, 5]):
-            rc, hit, rcounts, ref = ref_load_interpolate(fw, lj, [4, 5], 0.7, d, guest, pos)
-            assert rc == 0 and hit == 1 and rcounts == tuple(counts)
-            assert np.array_equal(aa.grid_interpolate(ctx, fw.m, counts, tables[slot], pos), ref)
+        n, sp, loaded = aa.grid_cache_load(d, k)
+        assert n == tuple(counts) and list(sp) == [4, 5]
+        assert np.array_equal(loaded, tables)
+        for slot in range(2):
+            got = aa.grid_interpolate(ctx, fw.m, counts, loaded[slot], pos)
+            assert np.array_equal(got, oracle_interpolate(fw.m, counts, loaded[slot], pos))
 
 
 def test_interpolate_errors(ctx):
