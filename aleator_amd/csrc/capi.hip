@@ -511,7 +511,7 @@ aleator_status aleator_gpu_grid_point_counts(const aleator_lattice* grid_lattice
 
 static void grid_common(aleator_framework* fw, const aleator_lattice* grid_lattice, const uint32_t* guests,
                         int32_t n_guest, int32_t coulomb, const int32_t n_points[3], int32_t precision,
-                        double cap, double* out_device) {
+                        double cap, double* out_device, bool no_alloc = false) {
     require(fw && n_points && out_device, ALEATOR_INVALID_ARGUMENT, "null argument");
     require(n_guest >= 0 && (n_guest == 0 || guests), ALEATOR_INVALID_ARGUMENT, "guest species list");
     GridRequest req;
@@ -526,7 +526,7 @@ static void grid_common(aleator_framework* fw, const aleator_lattice* grid_latti
     req.precision = precision;
     req.cap = cap;
     req.out_device = out_device;
-    launch_grid(*fw, req);
+    launch_grid(*fw, req, no_alloc);
 }
 
 aleator_status aleator_gpu_grid_build_device(aleator_framework* fw, const aleator_lattice* grid_lattice,
@@ -535,6 +535,15 @@ aleator_status aleator_gpu_grid_build_device(aleator_framework* fw, const aleato
                                              double* out_device) {
     return guarded([&] {
         grid_common(fw, grid_lattice, guest_species, n_guest, coulomb, n_points, precision, cap, out_device);
+    });
+}
+
+aleator_status aleator_gpu_grid_build_device_noalloc(aleator_framework* fw, const aleator_lattice* grid_lattice,
+                                                     const uint32_t* guest_species, int32_t n_guest, int32_t coulomb,
+                                                     const int32_t n_points[3], int32_t precision, double cap,
+                                                     double* out_device) {
+    return guarded([&] {
+        grid_common(fw, grid_lattice, guest_species, n_guest, coulomb, n_points, precision, cap, out_device, true);
     });
 }
 

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "device.hpp"
@@ -289,26 +290,62 @@ void GridJob::launch_pre(hipStream_t s) const {
                        args.n[2], d_t2);
 }
 
-void launch_tiles(int precision, int shape, int ng, int deg, bool lfs, const GridArgs* d_descs, const int* d_prefix,
+void launch_tiles(int precision, int shape, int ng, int deg, int ljk, const GridArgs* d_descs, const int* d_prefix,
                   int n_desc, int blocks, hipStream_t s) {
-    if (precision == ALEATOR_PRECISION_FP64) {
-        if (lfs) launch_tiles_fp64_lfs(shape, ng, deg, d_descs, d_prefix, n_desc, blocks, s);
-        else launch_tiles_fp64(shape, ng, deg, d_descs, d_prefix, n_desc, blocks, s);
-    } else {
-        if (lfs) launch_tiles_mixed_lfs(shape, ng, deg, d_descs, d_prefix, n_desc, blocks, s);
-        else launch_tiles_mixed(shape, ng, deg, d_descs, d_prefix, n_desc, blocks, s);
-    }
+    using Fn = void (*)(int, int, int, const GridArgs*, const int*, int, int, hipStream_t);
+    static const Fn kFp64[3] = {launch_tiles_fp64_trunc, launch_tiles_fp64_shift, launch_tiles_fp64_lfs};
+    static const Fn kMixed[3] = {launch_tiles_mixed_trunc, launch_tiles_mixed_shift, launch_tiles_mixed_lfs};
+    if (ljk < 0 || ljk > 2) fail(ALEATOR_INVALID_ARGUMENT, "grid: unknown LJ truncation");
+    (precision == ALEATOR_PRECISION_FP64 ? kFp64 : kMixed)[ljk](shape, ng, deg, d_descs, d_prefix, n_desc, blocks, s);
 }
 
-// single build through the context scratch buffer (the per-call entry points)
-void launch_grid(Framework& fw, const GridRequest& req) {
+// ---------------------------------------------------------------- allocation-free builds
+// The build's device scratch (LJ constants, reciprocal contraction tables, descriptor) in a fixed
+// array of this library, for callers that cannot allocate.  Each GPU holds its own copy for the
+// life of the process; its address is looked up once per device, with that device current.  All
+// calls on one GPU share it, so calls that overlap on one GPU race.
+namespace {
+__device__ __align__(256) char scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mutex;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void* no_alloc_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0 || device >= kMaxDevices) fail(ALEATOR_INVALID_ARGUMENT, "grid: device index out of range");
+    if (!g_scratch_ptr[device]) {
+        int current = 0;
+        ALEATOR_HIP_CHECK(hipGetDevice(&current));
+        if (current != device) ALEATOR_HIP_CHECK(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (current != device) (void)hipSetDevice(current);
+        if (e != hipSuccess || p == nullptr)
+            fail(ALEATOR_DEVICE_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+// single build through the context scratch buffer, or the fixed array (the per-call entry points)
+void launch_grid(Framework& fw, const GridRequest& req, bool no_alloc) {
     Context& ctx = *fw.ctx;
     GridJob job;
     job.prepare(fw, req);
     const int shape = job.shape_fits(ctx.tile_shape) ? ctx.tile_shape : 0;
     job.set_shape(shape, job.chunk_fits(shape, ctx.tile_chunk) ? ctx.tile_chunk : 128);
     const size_t off_desc = job.bytes, off_prefix = align256(off_desc + sizeof(GridArgs));
-    char* base = static_cast<char*>(ctx.ensure_scratch(off_prefix + 256));
+    char* base = nullptr;
+    if (no_alloc) {
+        if (off_prefix + 256 > kNoAllocScratchBytes)
+            fail(ALEATOR_INVALID_ARGUMENT, "grid: this build needs " + std::to_string(off_prefix + 256) +
+                                               " bytes of scratch, above the allocation-free limit of " +
+                                               std::to_string(kNoAllocScratchBytes) + " bytes");
+        base = static_cast<char*>(no_alloc_scratch(ctx.device));
+    } else {
+        base = static_cast<char*>(ctx.ensure_scratch(off_prefix + 256));
+    }
     job.bind(base);
     job.upload(ctx.stream);
     const int prefix[2] = {0, job.blocks};
@@ -316,7 +353,7 @@ void launch_grid(Framework& fw, const GridRequest& req) {
     ALEATOR_HIP_CHECK(hipMemcpyAsync(base + off_prefix, prefix, sizeof(prefix), hipMemcpyHostToDevice, ctx.stream));
     job.launch_pre(ctx.stream);
     ctx.time_begin();
-    launch_tiles(job.precision, job.shape_used(), job.ng, job.deg, job.args.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED,
+    launch_tiles(job.precision, job.shape_used(), job.ng, job.deg, lj_kind(job.args.trunc),
                  reinterpret_cast<const GridArgs*>(base + off_desc),
                  reinterpret_cast<const int*>(base + off_prefix), 1, job.blocks, ctx.stream);
     ctx.time_end();

@@ -70,17 +70,25 @@ struct GridJob {
 };
 
 int tile_blocks(const int32_t n[3], int shape);
-// deg: erf polynomial degree of every framework of the launch (0: no Coulomb table); lfs: some
-// framework of the launch uses linear-force-shifted LJ
-void launch_tiles(int precision, int shape, int ng, int deg, bool lfs, const GridArgs* d_descs, const int* d_prefix,
+// deg: erf polynomial degree of every framework of the launch (0: no Coulomb table); ljk: the most
+// general LJ truncation of the launch's frameworks (0 truncated, 1 shifted, 2 linear-force-shifted)
+void launch_tiles(int precision, int shape, int ng, int deg, int ljk, const GridArgs* d_descs, const int* d_prefix,
                   int n_desc, int blocks, hipStream_t s);
+// truncation kind of one framework (ALEATOR_LJ_*) in the order used by launch_tiles
+inline int lj_kind(int trunc) {
+    return trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED ? 2 : trunc == ALEATOR_LJ_SHIFTED ? 1 : 0;
+}
 // the tile kernels of each precision and LJ truncation kind (grid_tile_*.hip)
-void launch_tiles_fp64(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
-                       hipStream_t s);
+void launch_tiles_fp64_trunc(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
+                             hipStream_t s);
+void launch_tiles_fp64_shift(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
+                             hipStream_t s);
 void launch_tiles_fp64_lfs(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
                            hipStream_t s);
-void launch_tiles_mixed(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
-                        hipStream_t s);
+void launch_tiles_mixed_trunc(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
+                              hipStream_t s);
+void launch_tiles_mixed_shift(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
+                              hipStream_t s);
 void launch_tiles_mixed_lfs(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
                             hipStream_t s);
 

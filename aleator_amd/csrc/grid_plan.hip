@@ -60,9 +60,9 @@ void plan_launch(aleator_grid_plan* p) {
     hipStream_t s = p->ctx->stream;
     for (const auto& j : p->jobs) j.launch_pre(s);
     const GridJob& j0 = p->jobs.front();
-    bool lfs = false;
-    for (const auto& j : p->jobs) lfs = lfs || j.args.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED;
-    launch_tiles(j0.precision, p->shape, j0.ng, j0.deg, lfs, p->d_descs, p->d_prefix, (int)p->jobs.size(),
+    int ljk = 0;
+    for (const auto& j : p->jobs) ljk = std::max(ljk, lj_kind(j.args.trunc));
+    launch_tiles(j0.precision, p->shape, j0.ng, j0.deg, ljk, p->d_descs, p->d_prefix, (int)p->jobs.size(),
                  p->total_blocks, s);
 }
 

@@ -21,9 +21,10 @@
 // output is 8 B per node per table.
 //
 // Pair arithmetic is FP64 (pair.hpp): 1/r or 1/r^2 from the hardware approximation refined by NR
-// Newton steps (1: MIXED, 2: FP64), LJ as A/r^12 - (B/r^6 + shift [+ slope r]) with A = 4 eps
-// sigma^12, B = 4 eps sigma^6 per (guest, type), real-space Coulomb as 1/r - erf(alpha r)/r with
-// the erf kernel a degree-DEG polynomial in r^2 (DEG 0: no Coulomb table).
+// Newton steps (1: MIXED, 2: FP64), LJ as r^-6 (A r^-6 - B) [- shift [- slope r]] with
+// A = 4 eps sigma^12, B = 4 eps sigma^6 per (guest, type) (two FMAs per table when truncated),
+// real-space Coulomb as 1/r - erf(alpha r)/r with the erf kernel a degree-DEG polynomial in r^2
+// (DEG 0: no Coulomb table).
 //
 // Many frameworks (or one) are built by one launch: workgroup b finds its framework in a prefix
 // table of tile counts and reads that framework's descriptor from device memory.  Workgroups are
@@ -61,15 +62,17 @@ __device__ inline int xcd_contiguous(int b, int n) {
 
 }  // namespace tile_detail
 
-// EXACT: FP64 precision (two Newton steps), else MIXED (one).  LFS: some framework of the launch
-// uses linear-force-shifted LJ (its slope term is compiled in).
-template <bool EXACT, int NG, int DEG, bool LFS, int TX, int TY, int TZ, int NPT>
+// EXACT: FP64 precision (two Newton steps), else MIXED (one).  LJK: the most general LJ truncation
+// of the launch's frameworks (0 truncated, 1 shifted, 2 linear-force-shifted): the shift term is
+// compiled in from 1 on, the slope term at 2.
+template <bool EXACT, int NG, int DEG, int LJK, int TX, int TY, int TZ, int NPT>
 __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const GridArgs* __restrict__ descs,
                                                                           const int* __restrict__ prefix,
                                                                           int n_desc) {
     using namespace tile_detail;
     static_assert(TX * TY * TZ == kThreads, "one thread per (x, y, z-group)");
     constexpr bool COUL = DEG > 0;
+    constexpr bool LFS = LJK == 2;
     constexpr int NR = EXACT ? 2 : 1;
     constexpr int NZ = TZ * NPT;  // nodes per tile along z
     constexpr int NGA = NG > 0 ? NG : 1;
@@ -180,12 +183,12 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
                 const double r2 = p == 0 ? r20 : fma(tds, m2p[p], r20 + s2p[p]);
                 auto lj = [&](double ir2, double r) {
                     const double ir6 = ir2 * ir2 * ir2;
-                    const double ir12 = ir6 * ir6;
 #pragma unroll
                     for (int g = 0; g < NG; ++g) {
-                        double w = fma(cg[g].y, ir6, cg[g].z);
-                        if constexpr (LFS) w = fma(r, cg[g].w, w);
-                        acc[p][g] += fma(cg[g].x, ir12, -w);
+                        double v = fma(ir6, fma(cg[g].x, ir6, -cg[g].y), acc[p][g]);
+                        if constexpr (LJK == 1) v -= cg[g].z;
+                        if constexpr (LFS) v -= fma(r, cg[g].w, cg[g].z);
+                        acc[p][g] = v;
                     }
                 };
                 auto coul = [&](double ir) {
@@ -324,12 +327,12 @@ struct TileShape {
 };
 constexpr TileShape kShapes[kNumTileShapes] = {{8, 8, 4, 1}, {8, 8, 4, 2}, {8, 8, 4, 4}, {4, 8, 8, 2}};
 
-template <bool EXACT, int NG, int DEG, bool LFS>
+template <bool EXACT, int NG, int DEG, int LJK>
 void launch_shape(int shape, const GridArgs* d, const int* prefix, int n_desc, int blocks, hipStream_t s) {
     switch (shape) {
 #define ALEATOR_TILE_SHAPE(I, X, Y, Z, P)                                                                       \
     case I:                                                                                                     \
-        hipLaunchKernelGGL((grid_tile_kernel<EXACT, NG, DEG, LFS, X, Y, Z, P>), dim3(blocks),                   \
+        hipLaunchKernelGGL((grid_tile_kernel<EXACT, NG, DEG, LJK, X, Y, Z, P>), dim3(blocks),                   \
                            dim3(tile_detail::kThreads), 0, s, d, prefix, n_desc);                               \
         return;
         ALEATOR_TILE_SHAPE(0, 8, 8, 4, 1)
@@ -341,28 +344,28 @@ void launch_shape(int shape, const GridArgs* d, const int* prefix, int n_desc, i
     fail(ALEATOR_INVALID_ARGUMENT, "grid: unknown tile shape");
 }
 
-template <bool EXACT, int DEG, bool LFS>
+template <bool EXACT, int DEG, int LJK>
 void dispatch_ng(int shape, int ng, const GridArgs* d, const int* prefix, int n_desc, int blocks, hipStream_t s) {
     switch (ng) {
         case 0:
-            if constexpr (DEG > 0) return launch_shape<EXACT, 0, DEG, LFS>(shape, d, prefix, n_desc, blocks, s);
+            if constexpr (DEG > 0) return launch_shape<EXACT, 0, DEG, LJK>(shape, d, prefix, n_desc, blocks, s);
             break;
-        case 1: return launch_shape<EXACT, 1, DEG, LFS>(shape, d, prefix, n_desc, blocks, s);
-        case 2: return launch_shape<EXACT, 2, DEG, LFS>(shape, d, prefix, n_desc, blocks, s);
-        case 3: return launch_shape<EXACT, 3, DEG, LFS>(shape, d, prefix, n_desc, blocks, s);
-        case 4: return launch_shape<EXACT, 4, DEG, LFS>(shape, d, prefix, n_desc, blocks, s);
+        case 1: return launch_shape<EXACT, 1, DEG, LJK>(shape, d, prefix, n_desc, blocks, s);
+        case 2: return launch_shape<EXACT, 2, DEG, LJK>(shape, d, prefix, n_desc, blocks, s);
+        case 3: return launch_shape<EXACT, 3, DEG, LJK>(shape, d, prefix, n_desc, blocks, s);
+        case 4: return launch_shape<EXACT, 4, DEG, LJK>(shape, d, prefix, n_desc, blocks, s);
     }
     fail(ALEATOR_INVALID_ARGUMENT, "grid: unsupported guest count");
 }
 
-template <bool EXACT, bool LFS>
+template <bool EXACT, int LJK>
 void dispatch_deg(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
                   hipStream_t s) {
     switch (deg) {
-        case 0: return dispatch_ng<EXACT, 0, LFS>(shape, ng, d, prefix, n_desc, blocks, s);
-        case 18: return dispatch_ng<EXACT, 18, LFS>(shape, ng, d, prefix, n_desc, blocks, s);
-        case 24: return dispatch_ng<EXACT, 24, LFS>(shape, ng, d, prefix, n_desc, blocks, s);
-        case kErfPolyMax: return dispatch_ng<EXACT, kErfPolyMax, LFS>(shape, ng, d, prefix, n_desc, blocks, s);
+        case 0: return dispatch_ng<EXACT, 0, LJK>(shape, ng, d, prefix, n_desc, blocks, s);
+        case 18: return dispatch_ng<EXACT, 18, LJK>(shape, ng, d, prefix, n_desc, blocks, s);
+        case 24: return dispatch_ng<EXACT, 24, LJK>(shape, ng, d, prefix, n_desc, blocks, s);
+        case kErfPolyMax: return dispatch_ng<EXACT, kErfPolyMax, LJK>(shape, ng, d, prefix, n_desc, blocks, s);
     }
     fail(ALEATOR_INVALID_ARGUMENT, "grid: unsupported erf polynomial degree");
 }

@@ -5,7 +5,7 @@ namespace aleator_gpu {
 
 void launch_tiles_fp64_lfs(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
                            hipStream_t s) {
-    dispatch_deg<true, true>(shape, ng, deg, d, prefix, n_desc, blocks, s);
+    dispatch_deg<true, 2>(shape, ng, deg, d, prefix, n_desc, blocks, s);
 }
 
 }  // namespace aleator_gpu

@@ -1,0 +1,11 @@
+// grid_tile_mixed_trunc.hip -- grid tile kernels for MIXED precision with truncated LJ (grid_tile.hpp).
+#include "grid_tile.hpp"
+
+namespace aleator_gpu {
+
+void launch_tiles_mixed_trunc(int shape, int ng, int deg, const GridArgs* d, const int* prefix, int n_desc, int blocks,
+                              hipStream_t s) {
+    dispatch_deg<false, 0>(shape, ng, deg, d, prefix, n_desc, blocks, s);
+}
+
+}  // namespace aleator_gpu

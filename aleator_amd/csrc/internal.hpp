@@ -174,7 +174,10 @@ struct GridRequest {
     double cap = 1e5;
     double* out_device = nullptr;
 };
-void launch_grid(Framework& fw, const GridRequest& req);
+// no_alloc: use the library's fixed device scratch array instead of the context's growable buffer
+// (no device allocation; requests larger than kNoAllocScratchBytes fail with invalid argument)
+void launch_grid(Framework& fw, const GridRequest& req, bool no_alloc = false);
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
 
 struct WidomRequest {
     std::vector<aleator_molecule_site> sites;

@@ -161,6 +161,17 @@ aleator_status aleator_gpu_grid_build_device(aleator_framework* fw,
                                              const uint32_t* guest_species, int32_t n_guest,
                                              int32_t coulomb, const int32_t n_points[3],
                                              int32_t precision, double cap, double* out_device);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Same as aleator_gpu_grid_build_device, with the
+ * build's scratch (constants, reciprocal contraction tables) in a fixed 32 MiB device array of
+ * the library instead of the context's growable buffer: no hipMalloc or hipFree.  A build whose
+ * scratch would exceed 32 MiB fails with ALEATOR_INVALID_ARGUMENT (the c1 three-table grid needs
+ * about 5.5 MB).  All calls on one GPU share the array: calls that overlap on one GPU race. */
+aleator_status aleator_gpu_grid_build_device_noalloc(aleator_framework* fw,
+                                                     const aleator_lattice* grid_lattice,
+                                                     const uint32_t* guest_species, int32_t n_guest,
+                                                     int32_t coulomb, const int32_t n_points[3],
+                                                     int32_t precision, double cap, double* out_device);
 
 /* A reusable build of the grids of one or many frameworks (no reference counterpart as one
  * call; each framework's tables equal aleator_gpu_grid_build_device's).  Creation uploads every
