@@ -156,6 +156,87 @@ def c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=1_000_000, learned_max
     }
 
 
+def c4_equilibrium(aa, ctx, sync, seeds=32, fill_steps=10_000, seg=250, prod_steps=1_600_000):
+    """Learned density against plain uniform insertion at equilibrium, on a framework where plain
+    GCMC equilibrates within the budget: the c0 framework (cubic 25.83 A, 424 atoms), CH4 at 298 K
+    at the 8 c3 pressures (0.1-30 bar, Peng-Robinson), `seeds` chains each, both from empty.
+    Per method: the filling phase in segments, then a production block.  Reported per pressure:
+    plain's loading N_p +- sigma_p (standard error over seeds), the learned N_l +- sigma_l, steps to
+    90 % of N_p, and the GPU-seconds the learned method needs for plain's precision:
+    t_density + t_learned (sigma_l/sigma_p)^2 (the standard error falls as 1/sqrt(steps)).  The
+    density's cost (grid, 2e6-sample training set drawn on the host, MLP training, bin table) is
+    counted in full."""
+    from aleator_amd import learned, workloads as wl
+    w = wl.c0()
+    fw = aa.Framework(ctx, w["m"], w["x"], w["y"], w["z"], w["sp"], w["eps"], w["sig"], w["rc"])
+    pres = [0.1 * 300.0 ** (i / 7.0) for i in range(8)]
+    chains = []
+    for i, p in enumerate(pres):
+        f = aa.peng_robinson_fugacity(190.6, 4.599e6, 0.011, 298.0, p * 1e5)
+        chains += [(f, 298.0, 500 + s, 64 * i + s) for s in range(seeds)]
+    nb = (64, 64, 64)
+    sync()
+    t0 = time.perf_counter()
+    tab = fw.grid([4], w["counts"], precision=aa.MIXED)
+    frac, e = learned.sample_grid(tab, [1.0], w["counts"], 2_000_000, low_fraction=0.5, low_cut=1000.0)
+    ld = learned.LearnedDensity(ctx, max_rows=8192)
+    ld.fit(frac, e, e_clip=1000.0, n_iters=3000, batch=4096, lr=1e-3)
+    weights = ld.bin_weights(nb, 298.0)
+    sync()
+    t_density = time.perf_counter() - t0
+
+    def run(wts):
+        g = aa.Gcmc(fw, wl.CH4_SITES, chains, max_molecules=512)
+        if wts is not None:
+            g.set_proposal(wts, nb, (1, 1, 1), 0.05)
+        fill = []
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(fill_steps // seg):
+            g.reset_stats()
+            g.run(seg)
+            st = g.stats()
+            fill.append(np.array([s["sum_molecules"] / s["steps"] for s in st]).reshape(8, seeds).mean(axis=1))
+        g.reset_stats()
+        g.run(prod_steps)
+        sync()
+        dt = time.perf_counter() - t0
+        st = g.stats()
+        prod = np.array([s["sum_molecules"] / s["steps"] for s in st]).reshape(8, seeds)
+        acc = sum(s["accepted"][2] for s in st) / max(1, sum(s["attempted"][2] for s in st))
+        err = sum(s["error"] for s in st)
+        g.close()
+        return np.array(fill), prod, acc, dt, err
+
+    fill_p, prod_p, acc_p, t_p, e1 = run(None)
+    fill_l, prod_l, acc_l, t_l, e2 = run(weights)
+    n_p, n_l = prod_p.mean(axis=1), prod_l.mean(axis=1)
+    s_p = prod_p.std(axis=1, ddof=1) / np.sqrt(seeds)
+    s_l = prod_l.std(axis=1, ddof=1) / np.sqrt(seeds)
+
+    def steps_to(fill, target):
+        hit = np.flatnonzero(fill >= target)
+        return int((hit[0] + 1) * seg) if hit.size else None
+
+    big = [i for i in range(8) if n_p[i] >= 0.5]
+    t_equal = [t_density + t_l * (s_l[i] / s_p[i]) ** 2 for i in big]
+    return {
+        "c4eq_system": "c0 framework, CH4, 298 K, 8 pressures 0.1-30 bar x 32 seeds, from empty",
+        "c4eq_pressures_bar": [round(p, 4) for p in pres],
+        "c4eq_plain_loading": n_p.round(4).tolist(), "c4eq_plain_sigma": s_p.round(4).tolist(),
+        "c4eq_learned_loading": n_l.round(4).tolist(), "c4eq_learned_sigma": s_l.round(4).tolist(),
+        "c4eq_agree_within_2sigma": bool(np.all(np.abs(n_l - n_p) <= 2 * np.hypot(s_p, s_l))),
+        "c4eq_steps_to_90pct_plain": [steps_to(fill_p[:, i], 0.9 * n_p[i]) for i in big],
+        "c4eq_steps_to_90pct_learned": [steps_to(fill_l[:, i], 0.9 * n_p[i]) for i in big],
+        "c4eq_insertion_acceptance": {"plain": acc_p, "learned": acc_l},
+        "c4eq_gpu_s_plain": t_p, "c4eq_gpu_s_learned_run": t_l, "c4eq_gpu_s_density": t_density,
+        "c4eq_gpu_s_learned_for_plain_precision": [round(v, 4) for v in t_equal],
+        "c4eq_gain_gpu_s_min_over_pressures": min(t_p / v for v in t_equal),
+        "c4eq_variance_ratio_equal_steps": [round(float((s_p[i] / s_l[i]) ** 2), 2) for i in big],
+        "c4eq_steps_per_method": fill_steps + prod_steps, "c4eq_errors": e1 + e2,
+    }
+
+
 def measured_traffic(tile_shape):
     """HBM bytes per launch of the headline tile kernel with the tile shape this run uses, from the
     committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE runs scaled by a device copy of
@@ -294,7 +375,7 @@ def host_threads() -> int:
         return os.cpu_count() or 1
 
 
-LEGS = ("c1", "c0", "c2", "c3", "ewald", "mlp", "irmof1", "gcmc_grid", "guest_cells", "c4")
+LEGS = ("c1", "c0", "c2", "c3", "ewald", "mlp", "irmof1", "gcmc_grid", "guest_cells", "c4", "c4eq")
 
 
 def guest_cell_rates(aa, ctx, sync, n_chains=64, fugacity=1e7):
@@ -528,6 +609,7 @@ def main():
         ap.error(f"unknown legs {sorted(legs - set(LEGS))}")
     if args.no_c4:
         legs.discard("c4")
+        legs.discard("c4eq")
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the hot path would shard by grid slab, trial range or chain, with no exchange "
@@ -713,6 +795,12 @@ def main():
             extra.update(guest_cell_rates(aa, ctx, sync))
         except Exception as e:
             extra["guest_cells_error"] = repr(e)
+
+    if "c4eq" in legs:
+        try:
+            extra.update(c4_equilibrium(aa, ctx, sync))
+        except Exception as e:
+            extra["c4eq_error"] = repr(e)
 
     if "c4" in legs:
         # c4: learned insertion density vs plain uniform insertion on the c3 chains
