@@ -685,6 +685,7 @@ class Gcmc:
     def __init__(self, fw: Framework, sites, chains, max_translation=1.5, max_rotation=0.5, max_molecules=512):
         self.fw = fw
         self.ns = len(sites)
+        self.max_molecules = int(max_molecules)
         chains = list(chains)
         self.n_chains = len(chains)
         arr = Framework._sites(sites)
@@ -708,9 +709,12 @@ class Gcmc:
 
     def run(self, n_steps: int, trace: bool = False):
         """Advances every chain by n_steps; with trace, returns int32 [n_chains][n_steps] molecule
-        counts after each step."""
+        counts after each step.  Raises RuntimeError when a chain stopped because an accepted
+        insertion would exceed max_molecules (the chain keeps its state before that step; its
+        trace rows from there on read -1)."""
         if not trace:
             _check(_load().aleator_gpu_gcmc_run(self._h, C.c_uint64(n_steps), None))
+            self._raise_on_error()
             return None
         hip = _hip_rt()
         nbytes = self.n_chains * n_steps * 4
@@ -722,9 +726,18 @@ class Gcmc:
             out = np.zeros((self.n_chains, n_steps), dtype=np.int32)
             if hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), d, C.c_size_t(nbytes), 2) != 0:  # device to host
                 raise DeviceError("hipMemcpy failed")
-            return out
         finally:
             hip.hipFree(d)
+        self._raise_on_error(out)
+        return out
+
+    def _raise_on_error(self, trace=None):
+        bad = [c for c, s in enumerate(self.stats()) if s["error"]]
+        if bad:
+            err = RuntimeError(f"gcmc: chain(s) {bad} stopped: an accepted insertion would exceed max_molecules "
+                               f"({self.max_molecules})")
+            err.trace = trace  # with trace: rows from the stopping step on read -1
+            raise err
 
     def stats(self):
         out = (_ChainStats * self.n_chains)()
@@ -734,6 +747,10 @@ class Gcmc:
 
     def reset_stats(self):
         _check(_load().aleator_gpu_gcmc_reset_stats(self._h))
+
+    def use_graph(self, enable: bool = True):
+        """Replay run() (without trace) from a captured hipGraph (aleator_gpu_gcmc_use_graph)."""
+        _check(_load().aleator_gpu_gcmc_use_graph(self._h, C.c_int32(1 if enable else 0)))
 
     def set_proposal(self, weights, nb, reps, uniform_mix=0.05):
         """Biased insertion from bin weights [nb0][nb1][nb2] over the cell divided by reps

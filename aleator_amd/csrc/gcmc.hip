@@ -321,6 +321,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         double bias;                  // V p(anchor) of the insertion density (1: uniform insertion)
         double pend_self, pend_excl;  // self / exclusion deltas of the pending move
         uint64_t att[4], acc[4];
+        uint64_t step_call;           // Philox call index of the current step's first draw
     } t0;
 
     const int chain = blockIdx.x;
@@ -362,7 +363,11 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     const double volume = fabs(a.volume);
 
     __syncthreads();
-    for (uint64_t step = 0; step < a.n_steps && !error; ++step) {
+    // a chain whose accepted insertion would exceed max_mol stops before that step takes effect:
+    // `overflow` marks it, the step's counts are taken back and `step` is the number of steps run
+    bool overflow = false;
+    uint64_t step = 0;
+    for (; step < a.n_steps && !error; ++step) {
         // ---------------- this step's uniforms, drawn ahead in parallel
         const uint64_t call = s_call;
         if (tid < kDraws) s_u[tid] = philox_uniform(P.seed, P.stream, call + tid);
@@ -374,6 +379,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         Draws uni{s_u, 0};
         // ---------------- trial sites (thread 0)
         if (tid == 0) {
+            t0.step_call = call;
             const double pick = uni();
             const int move = pick < 0.25 ? 0 : (pick < 0.5 ? 1 : (pick < 0.75 ? 2 : 3));
             const bool skip = (move != 2 && n_mol == 0) || (move == 1 && ns < 2);
@@ -628,6 +634,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
         if (ctl[2]) {
             if (move == 2 && n_mol >= a.max_mol) {
                 error = 1;
+                overflow = true;
                 break;
             }
             if (tid == 0 && CHARGED) {
@@ -747,13 +754,19 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             a.sim[(size_t)chain * a.nk_half + t] = s_im[t];
         }
     if (tid == 0) {
+        if (overflow) {
+            // the insertion that did not fit: not counted, and its draws are taken back
+            t0.att[2] -= 1;
+            t0.acc[2] -= 1;
+            s_call = t0.step_call;
+        }
         st->call = s_call;
         st->n_mol = n_mol;
         st->moves_since_resync = since_resync;
         st->self_energy = t0.self_e;
         st->exclusion_energy = t0.excl_e;
         st->sum_n = t0.sum_n;
-        st->steps += a.n_steps;
+        st->steps += step;
         for (int i = 0; i < 4; ++i) {
             st->attempted[i] = t0.att[i];
             st->accepted[i] = t0.acc[i];

@@ -33,6 +33,11 @@ struct aleator_gcmc {
     double4* d_bin_pos = nullptr;
     int32_t* d_bin_type = nullptr;
     double* d_grid = nullptr;  // energy-grid tables owned by the chains (a copy of the caller's)
+    // run without a trace replays one captured launch (recaptured when the launch arguments change)
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    aleator_gpu::GcmcArgs graph_args{};
 };
 
 namespace aleator_gpu {
@@ -322,9 +327,44 @@ aleator_status aleator_gpu_gcmc_run(aleator_gcmc* g, uint64_t n_steps, int32_t* 
         a.n_steps = n_steps;
         a.n_trace = trace_device;
         Context& ctx = *g->fw->ctx;
+        // rows a chain does not reach (it stopped on max_molecules) read -1
+        if (trace_device)
+            ALEATOR_HIP_CHECK(hipMemsetAsync(trace_device, 0xff, (size_t)g->n_chains * n_steps * sizeof(int32_t),
+                                             ctx.stream));
+        if (g->use_graph && !trace_device) {
+            if (!g->graph_exec || std::memcmp(&g->graph_args, &a, sizeof(a)) != 0) {
+                if (g->graph_exec) ALEATOR_HIP_CHECK(hipGraphExecDestroy(g->graph_exec));
+                if (g->graph) ALEATOR_HIP_CHECK(hipGraphDestroy(g->graph));
+                g->graph_exec = nullptr;
+                g->graph = nullptr;
+                ALEATOR_HIP_CHECK(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    launch_gcmc(a, g->n_chains, ctx.stream);
+                } catch (...) {
+                    hipGraph_t dead = nullptr;
+                    (void)hipStreamEndCapture(ctx.stream, &dead);
+                    if (dead) (void)hipGraphDestroy(dead);
+                    throw;
+                }
+                ALEATOR_HIP_CHECK(hipStreamEndCapture(ctx.stream, &g->graph));
+                ALEATOR_HIP_CHECK(hipGraphInstantiate(&g->graph_exec, g->graph, nullptr, nullptr, 0));
+                g->graph_args = a;
+            }
+            ctx.time_begin();
+            ALEATOR_HIP_CHECK(hipGraphLaunch(g->graph_exec, ctx.stream));
+            ctx.time_end();
+            return;
+        }
         ctx.time_begin();
         launch_gcmc(a, g->n_chains, ctx.stream);
         ctx.time_end();
+    });
+}
+
+aleator_status aleator_gpu_gcmc_use_graph(aleator_gcmc* g, int32_t enable) {
+    return guarded([&] {
+        require(g != nullptr, ALEATOR_INVALID_ARGUMENT, "null argument");
+        g->use_graph = enable != 0;
     });
 }
 
@@ -491,9 +531,11 @@ aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* g) {
     return guarded([&] {
         if (!g) return;
         (void)hipStreamSynchronize(g->fw->ctx->stream);
+        if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
         for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_state, (void*)g->d_pos,
                         (void*)g->d_sre, (void*)g->d_sim, (void*)g->d_prop, (void*)g->d_bin_start,
-                        (void*)g->d_bin_pos, (void*)g->d_bin_type})
+                        (void*)g->d_bin_pos, (void*)g->d_bin_type, (void*)g->d_grid})
             if (p) (void)hipFree(p);
         delete g;
     });

@@ -338,6 +338,10 @@ aleator_status aleator_gpu_gcmc_create(aleator_framework* fw, const aleator_mole
 /* Advances every chain by n_steps in one launch on the context stream (asynchronous).
  * trace_device (nullable): int32 [n_chains][n_steps] molecule count after each step. */
 aleator_status aleator_gpu_gcmc_run(aleator_gcmc* gcmc, uint64_t n_steps, int32_t* trace_device);
+/* Whether run without a trace replays a captured hipGraph of its launch (recaptured whenever the
+ * launch's arguments change: steps, proposal, grid, cells) or launches directly (default 0).  The
+ * chains' state lives in device memory, so replays continue them exactly as launches do. */
+aleator_status aleator_gpu_gcmc_use_graph(aleator_gcmc* gcmc, int32_t enable);
 aleator_status aleator_gpu_gcmc_stats(aleator_gcmc* gcmc, aleator_gcmc_stats* out /* [n_chains] */);
 aleator_status aleator_gpu_gcmc_reset_stats(aleator_gcmc* gcmc);
 /* Site positions of one chain in molecule order, [n_molecules][n_sites][3]; xyz may be NULL

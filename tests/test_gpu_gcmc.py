@@ -74,7 +74,8 @@ def test_gcmc_errors(ctx):
     with pytest.raises(ValueError):
         aa.Gcmc(dev, [(0, 0, 0, 4, 0.0)], [(0.0, 300.0, 1, 0)])
     gc = aa.Gcmc(dev, [(0, 0, 0, 4, 0.0)], [(1e9, 300.0, 1, 0)], max_molecules=2)
-    gc.run(2000)
+    with pytest.raises(RuntimeError):
+        gc.run(2000)
     assert gc.stats()[0]["error"] == 1  # capacity exceeded is reported, not silently dropped
 
 
@@ -327,3 +328,54 @@ def test_closing_a_context_closes_what_was_created_on_it():
     c.close()
     assert not chains._h and not dev._h and not c._h
     chains.close()
+
+
+def test_gcmc_overflow_stops_the_chain_at_the_step_that_does_not_fit(ctx):
+    """max_molecules 3 at a high fugacity: the fourth accepted insertion does not fit.  run raises,
+    the trace reads -1 from that step on, and the chain's state and counters are those of the
+    oracle after exactly the steps that ran; a second run raises without advancing."""
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff, lj.truncation = 6.5, ol.SHIFTED
+    sites = [(0.0, 0.0, 0.0, 4, 0.0)]
+    dev = device_framework(ctx, fw, lj)
+    chains = [(1e8, 300.0, 5, 0)]
+    gc = aa.Gcmc(dev, sites, chains, max_molecules=3)
+    with pytest.raises(RuntimeError) as exc:
+        gc.run(3000, trace=True)
+    trace = exc.value.trace[0]
+    st = gc.stats()[0]
+    ran = int(st["steps"])
+    assert 0 < ran < 3000 and st["n_molecules"] == 3
+    assert np.all(trace[ran:] == -1) and np.all(trace[:ran] >= 0)
+    g = ol.Gcmc(lj, fw, sites, *chains[0])
+    ref = g.run(ran, trace=True)
+    assert np.array_equal(trace[:ran], ref.astype(np.int32))
+    att, acc = g.counts
+    assert st["attempted"] == att and st["accepted"] == acc
+    assert np.allclose(gc.positions(0), oracle_positions(g), rtol=0, atol=1e-9)
+    with pytest.raises(RuntimeError):
+        gc.run(100)
+    assert gc.stats()[0]["steps"] == ran
+    gc.close()
+
+
+def test_gcmc_graph_replay_equals_launches(ctx):
+    """run() replayed from a captured hipGraph continues the chains exactly as direct launches."""
+    fw, lj = ol.c0_system(40, a=14.0)
+    lj.cutoff, lj.truncation = 6.5, ol.SHIFTED
+    sites = [(0.0, 0.0, 0.0, 4, 0.0)]
+    dev = device_framework(ctx, fw, lj)
+    chains = [(1e8, 300.0, 5, 0), (3e7, 300.0, 6, 2)]
+    a = aa.Gcmc(dev, sites, chains, max_molecules=256)
+    b = aa.Gcmc(dev, sites, chains, max_molecules=256)
+    b.use_graph(True)
+    for _ in range(5):
+        a.run(300)
+        b.run(300)
+    b.run(100)  # a different step count recaptures
+    a.run(100)
+    for c in range(2):
+        assert a.stats()[c] == b.stats()[c]
+        assert np.array_equal(a.positions(c), b.positions(c))
+    a.close()
+    b.close()
