@@ -509,6 +509,11 @@ class Framework:
         except Exception:
             pass
 
+    def set_lj_tail(self, enable: bool = True):
+        """Add the analytic LJ tail of one insertion to grid tables and Widom energies
+        (aleator_gpu_framework_set_lj_tail; off by default, as in the reference)."""
+        _check(_load().aleator_gpu_framework_set_lj_tail(self._h, C.c_int32(1 if enable else 0)))
+
     def grid(self, guests, n_points, coulomb=False, precision=FP64, cap=1.0e5, grid_lattice=None):
         """Energy tables [len(guests) (+1 Coulomb)][nz][ny][nx] in K (Coulomb: K/e)."""
         g = np.ascontiguousarray(guests, dtype=np.uint32)

@@ -529,6 +529,13 @@ static void grid_common(aleator_framework* fw, const aleator_lattice* grid_latti
     launch_grid(*fw, req, no_alloc);
 }
 
+aleator_status aleator_gpu_framework_set_lj_tail(aleator_framework* fw, int32_t enable) {
+    return guarded([&] {
+        require(fw != nullptr, ALEATOR_INVALID_ARGUMENT, "null argument");
+        fw->lj_tail = enable != 0;
+    });
+}
+
 aleator_status aleator_gpu_grid_build_device(aleator_framework* fw, const aleator_lattice* grid_lattice,
                                              const uint32_t* guest_species, int32_t n_guest, int32_t coulomb,
                                              const int32_t n_points[3], int32_t precision, double cap,

@@ -132,6 +132,8 @@ void GridJob::prepare(Framework& fw, const GridRequest& req) {
     a.out = req.out_device;
     a.stride = (long long)req.n[0] * req.n[1] * req.n[2];
 
+    // the analytic tail of one guest site per table, when the framework asks for it
+    for (int g = 0; g < ng; ++g) a.tail[g] = fw.lj_tail ? lj_tail_delta(fw, &req.guests[g], 1) : 0.0;
     // LJ constants per (guest slot, framework type)
     ljtab.assign(4 * kMaxGuests * kMaxTypes, 0.0);
     for (int g = 0; g < ng; ++g) {

@@ -39,6 +39,7 @@ struct GridArgs {
                                       // every node's cutoffs (-1: none)
     double poly[kErfPolyMax + 1];     // erf(alpha r)/r in x = r^2 poly_xs - 1 (pair.hpp)
     double poly_xs;
+    double tail[kMaxGuests];          // analytic LJ tail of one guest site, added before the cap
 };
 
 // One framework's build: host-side constants, their device placement, pre-kernels.

@@ -306,7 +306,7 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
         const long long flat = ((long long)iz * n1 + iy) * n0 + ix;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            const double e = acc[p][g];
+            const double e = acc[p][g] + a.tail[g];
             a.out[g * a.stride + flat] = (a.cap < e) ? a.cap : e;  // std::min(e, cap), NaN kept
         }
         if constexpr (COUL) {

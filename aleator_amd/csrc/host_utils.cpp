@@ -62,6 +62,29 @@ double tail_energy(const aleator_lj_params* lj, const std::vector<double>& count
 
 }  // namespace
 
+namespace aleator_gpu {
+
+double lj_tail_delta(const Framework& fw, const uint32_t* site_species, int n_sites) {
+    aleator_lj_params lj{};
+    lj.epsilon = fw.eps.data();
+    lj.sigma = fw.sigma.data();
+    lj.n_species = (uint32_t)fw.eps.size();
+    lj.cutoff = fw.rc;
+    lj.truncation = fw.trunc;
+    lj.mixing = fw.mixing;
+    std::vector<double> before(lj.n_species, 0.0);
+    for (uint32_t sp : fw.species) before[sp] += 1.0;
+    std::vector<double> after = before;
+    for (int s = 0; s < n_sites; ++s) {
+        if (site_species[s] >= lj.n_species) fail(ALEATOR_OUT_OF_RANGE, "tail correction: species out of range");
+        after[site_species[s]] += 1.0;
+    }
+    const double vol = fw.lat.volume;
+    return tail_energy(&lj, after, vol) - tail_energy(&lj, before, vol);
+}
+
+}  // namespace aleator_gpu
+
 extern "C" {
 
 aleator_status aleator_philox_uniforms(uint64_t seed, uint64_t stream, uint64_t first_call_index, uint64_t n,

@@ -141,6 +141,9 @@ struct Framework {
     double *d_sre = nullptr, *d_sim = nullptr;  // framework S(k), half set
     std::vector<double> h_sre, h_sim;           // host copy of the same
     double self_energy = 0.0;
+    // add the analytic LJ tail of one insertion to grid tables and Widom energies (off: the
+    // reference's outputs)
+    bool lj_tail = false;
 };
 
 struct EwaldState {
@@ -158,6 +161,10 @@ struct LjPairTable {
     double vrc[kMaxGuests > kMaxSites ? kMaxGuests : kMaxSites][kMaxTypes];
     double dvrc[kMaxGuests > kMaxSites ? kMaxGuests : kMaxSites][kMaxTypes];
 };
+
+// LennardJones::tailEnergyCorrection (lennard_jones.cc:218-257) of the framework plus one
+// molecule of these site species minus that of the framework alone
+double lj_tail_delta(const Framework& fw, const uint32_t* site_species, int n_sites);
 
 // mixed parameters and cutoff constants of one (guest species, framework type) pair
 // (lennard_jones.cc:15-30, 49-68)

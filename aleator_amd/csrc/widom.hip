@@ -260,6 +260,11 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
     a.nk_half = fw.nk_half;
     a.kmax = fw.kmax;
     a.recip_scale = 2.0 * kCoulomb * (2.0 * kPi / std::abs(fw.lat.volume));
+    if (fw.lj_tail) {
+        std::vector<uint32_t> site_species;
+        for (const auto& s : req.sites) site_species.push_back(s.lj_species);
+        a.tail_delta = lj_tail_delta(fw, site_species.data(), ns);
+    }
 
     std::vector<double> tab(4 * kMaxSites * kMaxTypes, 0.0);
     for (int s = 0; s < ns; ++s)

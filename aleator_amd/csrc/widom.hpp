@@ -62,6 +62,7 @@ struct WidomArgs {
     int split;            // direct-sum kernel: threads sharing one trial's atoms (1, 2, 4, 8 or 16)
     double poly[kErfPolyMax + 1];  // erf(alpha r)/r in x = r^2 poly_xs - 1 (pair.hpp)
     double poly_xs;
+    double tail_delta;  // analytic LJ tail of one insertion (0 unless the framework asks for it)
 };
 
 // Reference point and rotated sites of trial t, Cartesian, in the reference's
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
             sl += part_lj[j + p * tpb];
             sr += part_re[j + p * tpb];
         }
-        const double du = CH ? sl + a.kc * sr : sl;
+        const double du = (CH ? sl + a.kc * sr : sl) + a.tail_delta;
         a.du[t] = du;
         if (!CH) boltz = exp(-du / a.temperature);
     }
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
         compute(pending);
 #pragma unroll
         for (int k = 0; k < TPT; ++k)
-            if (ok[k]) a.du[tr[k]] = CH ? lj[k] + a.kc * real[k] : lj[k];
+            if (ok[k]) a.du[tr[k]] = (CH ? lj[k] + a.kc * real[k] : lj[k]) + a.tail_delta;
     }
 }
 

@@ -141,6 +141,13 @@ aleator_status aleator_gpu_framework_num_k(const aleator_framework* fw, uint64_t
  * (framework_energy_grid.cc:207-211). */
 aleator_status aleator_gpu_grid_point_counts(const aleator_lattice* grid_lattice, double spacing,
                                              int32_t n_points[3]);
+/* The analytic LJ tail correction of one insertion, off by default (the reference adds it to
+ * neither its grid nor its Widom loop).  When enabled, the grid builds add to every LJ table
+ * the tail of one site of that table's species, before the cap, and Widom adds to every
+ * trial's dU the tail of the whole molecule: tailEnergyCorrection (lennard_jones.cc:218-257)
+ * of the framework plus the inserted sites minus that of the framework alone (the same
+ * quantity as aleator_lj_tail_insertion_delta). */
+aleator_status aleator_gpu_framework_set_lj_tail(aleator_framework* fw, int32_t enable);
 /* Builds one table per entry of `guest_species` (FrameworkEnergyGrid ctor body,
  * framework_energy_grid.cc:221-248): E(node) = min(sum_f pairEnergy(r, mix(guest,f)), cap).
  * When `coulomb` is non-zero and the framework is charged, one more table follows: the
