@@ -482,7 +482,8 @@ aleator_status aleator_gpu_framework_destroy(aleator_framework* fw) {
         if (!fw) return;
         (void)hipStreamSynchronize(fw->ctx->stream);
         for (void* p : {(void*)fw->d_fx, (void*)fw->d_fy, (void*)fw->d_fz, (void*)fw->d_q, (void*)fw->d_type,
-                        (void*)fw->d_kn, (void*)fw->d_kw, (void*)fw->d_kseg, (void*)fw->d_sre, (void*)fw->d_sim})
+                        (void*)fw->d_kn, (void*)fw->d_kw, (void*)fw->d_kseg, (void*)fw->d_sre, (void*)fw->d_sim,
+                        (void*)fw->d_widom_tab})
             if (p) (void)hipFree(p);
         delete fw;
     });

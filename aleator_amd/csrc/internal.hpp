@@ -144,6 +144,9 @@ struct Framework {
     // add the analytic LJ tail of one insertion to grid tables and Widom energies (off: the
     // reference's outputs)
     bool lj_tail = false;
+    // Widom's LJ constants on the device, kept while the inserted site species stay the same
+    double* d_widom_tab = nullptr;
+    std::vector<uint32_t> widom_tab_species;
 };
 
 struct EwaldState {

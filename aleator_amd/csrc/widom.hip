@@ -237,6 +237,14 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
             a.m[3 * i + j] = fw.lat.m[i][j];
             a.inv[3 * i + j] = fw.lat.inv[i][j];
         }
+    // orthorhombic cells (rows along the axes) take the Cartesian nearest image in the direct kernel
+    if (fw.lat.m[0][1] == 0.0 && fw.lat.m[0][2] == 0.0 && fw.lat.m[1][0] == 0.0 && fw.lat.m[1][2] == 0.0 &&
+        fw.lat.m[2][0] == 0.0 && fw.lat.m[2][1] == 0.0 && fw.lat.m[0][0] > 0.0 && fw.lat.m[1][1] > 0.0 &&
+        fw.lat.m[2][2] > 0.0)
+        for (int i = 0; i < 3; ++i) {
+            a.box[i] = fw.lat.m[i][i];
+            a.inv_box[i] = 1.0 / fw.lat.m[i][i];
+        }
     a.fx = fw.d_fx;
     a.fy = fw.d_fy;
     a.fz = fw.d_fz;
@@ -342,9 +350,17 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
                    255) & ~size_t(255)
                 : 0;
     char* base = static_cast<char*>(ctx.ensure_scratch(tab_bytes + parts * sizeof(double) + du_bytes + sort_bytes));
-    ALEATOR_HIP_CHECK(hipMemcpyAsync(base, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
-                                     ctx.stream));
-    a.ljtab = reinterpret_cast<const double*>(base);
+    // the LJ constants depend only on the framework and the site species: uploaded once per species list
+    std::vector<uint32_t> species_key;
+    for (const auto& site : req.sites) species_key.push_back(site.lj_species);
+    if (!fw.d_widom_tab) ALEATOR_HIP_CHECK(hipMalloc(&fw.d_widom_tab, tab.size() * sizeof(double)));
+    if (species_key != fw.widom_tab_species) {
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(fw.d_widom_tab, tab.data(), tab.size() * sizeof(double),
+                                         hipMemcpyHostToDevice, ctx.stream));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(ctx.stream));
+        fw.widom_tab_species = species_key;
+    }
+    a.ljtab = fw.d_widom_tab;
     a.block_sums = reinterpret_cast<double*>(base + tab_bytes);
     a.du = req.du_device ? req.du_device
                          : reinterpret_cast<double*>(base + tab_bytes + parts * sizeof(double));

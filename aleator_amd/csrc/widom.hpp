@@ -63,6 +63,7 @@ struct WidomArgs {
     double poly[kErfPolyMax + 1];  // erf(alpha r)/r in x = r^2 poly_xs - 1 (pair.hpp)
     double poly_xs;
     double tail_delta;  // analytic LJ tail of one insertion (0 unless the framework asks for it)
+    double box[3], inv_box[3];  // orthorhombic cells: edge lengths and their inverses
 };
 
 // Reference point and rotated sites of trial t, Cartesian, in the reference's
@@ -182,7 +183,9 @@ __device__ inline void widom_pair(double r2, const double4& cg, double qq, doubl
 // a batch of 1e5 trials still fills the chip.  Atoms are staged through LDS in chunks of 256 as
 // fractional coordinates; each pair folds the fractional difference to the nearest image (exact
 // for pairs within a cutoff <= L_perp/2).
-template <bool EXACT, int NS, int DEG, bool LFS>
+// ORTHO: orthorhombic cell; atoms and sites stay Cartesian and the nearest image is taken per
+// axis, d - L rint(d / L) (exact below L_perp/2, as the fractional fold).
+template <bool EXACT, int NS, int DEG, bool LFS, bool ORTHO>
 __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs a) {
     constexpr int NR = EXACT ? 2 : 1;
     constexpr bool CH = DEG > 0;
@@ -202,17 +205,39 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
     const int j = threadIdx.x % tpb, part = threadIdx.x / tpb;
     const uint64_t t = (uint64_t)blockIdx.x * tpb + j;
     const bool valid = t < a.n_trials;
+    // the trial's sites (fractional, or Cartesian with ORTHO): drawn by the first of its split
+    // threads and shared through LDS
+    __shared__ double s_fs[kWThreads / 2][NS][3];
     double fs[NS][3];
     double qs[NS];
-    {
+    auto draw = [&](double (&out)[NS][3]) {
         double cart[kMaxSites][3];
-        if (valid) trial_sites(a, t, cart);
+        trial_sites(a, t, cart);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            if (valid) to_frac(a, cart[s], fs[s]);
-            else fs[s][0] = fs[s][1] = fs[s][2] = 0.0;
-            qs[s] = a.charge[s];
+            if constexpr (ORTHO) {
+                out[s][0] = cart[s][0];
+                out[s][1] = cart[s][1];
+                out[s][2] = cart[s][2];
+            } else {
+                to_frac(a, cart[s], out[s]);
+            }
         }
+    };
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        fs[s][0] = fs[s][1] = fs[s][2] = 0.0;
+        qs[s] = a.charge[s];
+    }
+    if (split == 1) {
+        if (valid) draw(fs);
+    } else {
+        if (valid && part == 0) draw(s_fs[j]);
+        __syncthreads();
+        if (valid)
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                for (int k = 0; k < 3; ++k) fs[s][k] = s_fs[j][s][k];
     }
     double lj = 0.0, real = 0.0;
     const double rc2 = a.rc2_lj, rcr2 = a.rc2_real;
@@ -221,9 +246,15 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
         __syncthreads();
         const int at = base + threadIdx.x;
         if (at < a.natoms) {
-            sfx[threadIdx.x] = a.fx[at];
-            sfy[threadIdx.x] = a.fy[at];
-            sfz[threadIdx.x] = a.fz[at];
+            if constexpr (ORTHO) {
+                sfx[threadIdx.x] = a.fx[at] * a.box[0];
+                sfy[threadIdx.x] = a.fy[at] * a.box[1];
+                sfz[threadIdx.x] = a.fz[at] * a.box[2];
+            } else {
+                sfx[threadIdx.x] = a.fx[at];
+                sfy[threadIdx.x] = a.fy[at];
+                sfz[threadIdx.x] = a.fz[at];
+            }
             sq[threadIdx.x] = a.q[at];
             st[threadIdx.x] = a.type[at];
         }
@@ -235,13 +266,23 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
             const int ty = st[e];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                double d0 = ax - fs[s][0], d1 = ay - fs[s][1], d2 = az - fs[s][2];
-                d0 -= rint(d0);
-                d1 -= rint(d1);
-                d2 -= rint(d2);
-                const double dx = d0 * a.m[0] + d1 * a.m[3] + d2 * a.m[6];
-                const double dy = d0 * a.m[1] + d1 * a.m[4] + d2 * a.m[7];
-                const double dz = d0 * a.m[2] + d1 * a.m[5] + d2 * a.m[8];
+                double dx, dy, dz;
+                if constexpr (ORTHO) {
+                    dx = ax - fs[s][0];
+                    dy = ay - fs[s][1];
+                    dz = az - fs[s][2];
+                    dx = fma(-a.box[0], rint(dx * a.inv_box[0]), dx);
+                    dy = fma(-a.box[1], rint(dy * a.inv_box[1]), dy);
+                    dz = fma(-a.box[2], rint(dz * a.inv_box[2]), dz);
+                } else {
+                    double d0 = ax - fs[s][0], d1 = ay - fs[s][1], d2 = az - fs[s][2];
+                    d0 -= rint(d0);
+                    d1 -= rint(d1);
+                    d2 -= rint(d2);
+                    dx = d0 * a.m[0] + d1 * a.m[3] + d2 * a.m[6];
+                    dy = d0 * a.m[1] + d1 * a.m[4] + d2 * a.m[7];
+                    dz = d0 * a.m[2] + d1 * a.m[5] + d2 * a.m[8];
+                }
                 const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
                 if (one)
                     widom_pair<NR, DEG, LFS, true>(r2, tab[s][ty], qs[s] * qa, rc2, rcr2, pc, pxs, lj, real);
@@ -446,11 +487,16 @@ void launch_cells_deg(const WidomArgs& a, const CellArgs& c, unsigned n_cells, b
 }
 
 template <bool EXACT, int DEG, bool LFS>
-void launch_split_deg(const WidomArgs& a, unsigned blocks, hipStream_t s) {
+void launch_split_deg(const WidomArgs& a, unsigned blocks, bool ortho, hipStream_t s) {
     switch (a.nsites) {
-#define ALEATOR_WSPLIT(N)                                                                                   \
-    case N:                                                                                                 \
-        hipLaunchKernelGGL((widom_split_kernel<EXACT, N, DEG, LFS>), dim3(blocks), dim3(kWThreads), 0, s, a); \
+#define ALEATOR_WSPLIT(N)                                                                                         \
+    case N:                                                                                                       \
+        if (ortho)                                                                                                \
+            hipLaunchKernelGGL((widom_split_kernel<EXACT, N, DEG, LFS, true>), dim3(blocks), dim3(kWThreads), 0, s, \
+                               a);                                                                                \
+        else                                                                                                      \
+            hipLaunchKernelGGL((widom_split_kernel<EXACT, N, DEG, LFS, false>), dim3(blocks), dim3(kWThreads), 0, \
+                               s, a);                                                                             \
         return;
         ALEATOR_WSPLIT(1) ALEATOR_WSPLIT(2) ALEATOR_WSPLIT(3) ALEATOR_WSPLIT(4) ALEATOR_WSPLIT(5)
 #undef ALEATOR_WSPLIT
@@ -465,7 +511,7 @@ void launch_direct(const WidomArgs& a, const CellArgs* c, unsigned n_cells, bool
     auto go = [&](auto deg_tag) {
         constexpr int D = decltype(deg_tag)::value;
         if (c) launch_cells_deg<EXACT, D, LFS>(a, *c, n_cells, wide, s);
-        else launch_split_deg<EXACT, D, LFS>(a, blocks, s);
+        else launch_split_deg<EXACT, D, LFS>(a, blocks, a.box[0] > 0.0, s);
     };
     switch (deg) {
         case 0: return go(std::integral_constant<int, 0>{});
