@@ -32,15 +32,17 @@ long double g_kernel(long double u) {
 
 }  // namespace
 
-int erf_kernel_poly(double alpha, double rmax, double tol, int min_deg, double* coef, double* xs) {
+namespace {
+
+// Chebyshev coefficients of G(alpha^2 s) for s in [smin, smax] (64 points, long double)
+void cheb_fit(double alpha, long double smin, long double smax, long double* cheb) {
     constexpr int kN = 64;
     constexpr long double kPiL = 3.141592653589793238462643383279502884L;
-    const long double smax = (long double)rmax * rmax;
     const long double a2 = (long double)alpha * alpha;
-    long double f[kN], cheb[kN];
+    long double f[kN];
     for (int k = 0; k < kN; ++k) {
         const long double x = cosl(kPiL * (k + 0.5L) / kN);
-        f[k] = g_kernel(a2 * (x + 1.0L) * 0.5L * smax);
+        f[k] = g_kernel(a2 * (smin + (x + 1.0L) * 0.5L * (smax - smin)));
     }
     for (int j = 0; j < kN; ++j) {
         long double s = 0.0L;
@@ -48,24 +50,16 @@ int erf_kernel_poly(double alpha, double rmax, double tol, int min_deg, double* 
         cheb[j] = 2.0L * s / kN;
     }
     cheb[0] *= 0.5L;
-    // smallest allowed degree (at least min_deg) whose dropped coefficients sum below tol
-    static const int kDegrees[] = {18, 24, kErfPolyMax};
-    int deg = kErfPolyMax;
-    for (int d : kDegrees) {
-        if (d < min_deg) continue;
-        long double tail = 0.0L;
-        for (int j = d + 1; j < kN; ++j) tail += fabsl(cheb[j]);
-        if (tail <= tol) {
-            deg = d;
-            break;
-        }
-    }
-    // Chebyshev -> monomials: T_{j+1} = 2x T_j - T_{j-1}
+}
+
+// monomial coefficients (lowest first, times 2 alpha/sqrt(pi)) of the degree-deg truncation
+void to_monomials(double alpha, const long double* cheb, int deg, double* coef) {
+    constexpr long double kPiL = 3.141592653589793238462643383279502884L;
     std::vector<long double> tprev(deg + 1, 0.0L), tcur(deg + 1, 0.0L), mono(deg + 1, 0.0L);
-    tprev[0] = 1.0L;  // T_0
+    tprev[0] = 1.0L;
     mono[0] += cheb[0];
     if (deg >= 1) {
-        tcur[1] = 1.0L;  // T_1
+        tcur[1] = 1.0L;
         mono[1] += cheb[1];
     }
     for (int j = 2; j <= deg; ++j) {
@@ -78,6 +72,42 @@ int erf_kernel_poly(double alpha, double rmax, double tol, int min_deg, double* 
     }
     const long double pref = 2.0L * alpha / sqrtl(kPiL);
     for (int i = 0; i <= deg; ++i) coef[i] = (double)(pref * mono[i]);
+}
+
+long double tail_after(const long double* cheb, int deg) {
+    long double tail = 0.0L;
+    for (int j = deg + 1; j < 64; ++j) tail += fabsl(cheb[j]);
+    return tail;
+}
+
+}  // namespace
+
+bool erf_kernel_poly_range(double alpha, double rmin, double rmax, double tol, int deg, double* coef, double* xs,
+                           double* xo) {
+    const long double smin = (long double)rmin * rmin, smax = (long double)rmax * rmax;
+    long double cheb[64];
+    cheb_fit(alpha, smin, smax, cheb);
+    to_monomials(alpha, cheb, deg, coef);
+    *xs = (double)(2.0L / (smax - smin));
+    *xo = (double)(-(smax + smin) / (smax - smin));
+    return tail_after(cheb, deg) <= tol;
+}
+
+int erf_kernel_poly(double alpha, double rmax, double tol, int min_deg, double* coef, double* xs) {
+    const long double smax = (long double)rmax * rmax;
+    long double cheb[64];
+    cheb_fit(alpha, 0.0L, smax, cheb);
+    // smallest allowed degree (at least min_deg) whose dropped coefficients sum below tol
+    static const int kDegrees[] = {18, 24, kErfPolyMax};
+    int deg = kErfPolyMax;
+    for (int d : kDegrees) {
+        if (d < min_deg) continue;
+        if (tail_after(cheb, d) <= tol) {
+            deg = d;
+            break;
+        }
+    }
+    to_monomials(alpha, cheb, deg, coef);
     *xs = (double)(2.0L / smax);
     return deg;
 }

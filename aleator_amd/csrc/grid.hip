@@ -194,10 +194,7 @@ void GridJob::prepare(Framework& fw, const GridRequest& req) {
 
 void GridJob::fit_poly(int min_deg) {
     if (!coulomb) return;
-    // FP64: the dropped part of the erf kernel below the last bits of a term; MIXED: below 1e-12
-    // of it (a term of 3e4 K then carries under 3e-8 K)
-    const double tol = precision == ALEATOR_PRECISION_FP64 ? 1e-17 : 1e-12;
-    deg = erf_kernel_poly(args.alpha, std::sqrt(args.rc2_real), tol, min_deg, args.poly, &args.poly_xs);
+    deg = erf_kernel_poly(args.alpha, std::sqrt(args.rc2_real), poly_tolerance(), min_deg, args.poly, &args.poly_xs);
 }
 
 double GridJob::tile_radius(int shape) const {
@@ -235,6 +232,15 @@ void GridJob::set_shape(int s, int chunk) {
     if (coulomb) rmin = std::min(rmin, std::sqrt(args.rc2_real));
     const double inner = rmin - tile_radius(s) - 1e-9 * (1.0 + rmin);
     args.inner2 = inner > 0.0 ? inner * inner : -1.0;
+    // shell entries lie beyond `inner` from the tile centre, so every node sees them at r >= r_far:
+    // their erf polynomial is fitted over [r_far, rc] at a lower degree, when that meets the tolerance
+    args.far_ok = 0;
+    const double r_far = inner - tile_radius(s);
+    if (coulomb && deg > kFarDrop && r_far > 0.0)
+        args.far_ok = erf_kernel_poly_range(args.alpha, r_far, std::sqrt(args.rc2_real), poly_tolerance(),
+                                            deg - kFarDrop, args.poly_far, &args.poly_far_xs, &args.poly_far_xo)
+                          ? 1
+                          : 0;
     // below half the smallest perpendicular width at most one image of an atom, the nearest,
     // is within the search radius (the fractional rounding image is then the nearest one)
     args.single_image = (2.0 * sr < std::min(lperp[0], std::min(lperp[1], lperp[2]))) ? 1 : 0;

@@ -7,7 +7,8 @@
 
 namespace aleator_gpu {
 
-constexpr int kNumTileShapes = 4;  // 8x8x4, 8x8x8 (2/thread), 8x8x16 (4/thread), 4x8x16 (2/thread)
+constexpr int kNumTileShapes = 4;
+constexpr int kFarDrop = 6;  // the shell entries' erf polynomial is this many degrees lower  // 8x8x4, 8x8x8 (2/thread), 8x8x16 (4/thread), 4x8x16 (2/thread)
 
 // Everything one framework's tile kernel reads; stored in device memory, one per framework.
 struct GridArgs {
@@ -39,6 +40,11 @@ struct GridArgs {
                                       // every node's cutoffs (-1: none)
     double poly[kErfPolyMax + 1];     // erf(alpha r)/r in x = r^2 poly_xs - 1 (pair.hpp)
     double poly_xs;
+    // the same kernel fitted over the distances the shell entries can have (r >= sqrt(inner2) -
+    // tile radius), six degrees lower, in x = r^2 poly_far_xs + poly_far_xo; used when far_ok
+    double poly_far[kErfPolyMax + 1];
+    double poly_far_xs, poly_far_xo;
+    int far_ok;
     double tail[kMaxGuests];          // analytic LJ tail of one guest site, added before the cap
 };
 
@@ -60,6 +66,9 @@ struct GridJob {
 
     void prepare(Framework& fw, const GridRequest& req);
     void fit_poly(int min_deg);                 // erf polynomial of degree >= min_deg (Coulomb only)
+    // FP64: the dropped part of the erf kernel below the last bits of a term; MIXED: below 1e-12
+    // of it (a term of 3e4 K then carries under 3e-8 K)
+    double poly_tolerance() const { return precision == ALEATOR_PRECISION_FP64 ? 1e-17 : 1e-12; }
     double tile_radius(int shape) const;
     bool shape_fits(int shape) const;
     void set_shape(int s, int chunk);           // chunk: atoms per list-build pass

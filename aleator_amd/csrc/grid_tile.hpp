@@ -72,6 +72,7 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
     using namespace tile_detail;
     static_assert(TX * TY * TZ == kThreads, "one thread per (x, y, z-group)");
     constexpr bool COUL = DEG > 0;
+    constexpr int DEGF = DEG > kFarDrop ? DEG - kFarDrop : DEG;  // the shell entries' erf degree
     constexpr bool LFS = LJK == 2;
     constexpr int NR = EXACT ? 2 : 1;
     constexpr int NZ = TZ * NPT;  // nodes per tile along z
@@ -84,6 +85,7 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
     __shared__ double4 tab[NGA][kMaxTypes];
     __shared__ int scan_tmp[kThreads / 64 + 1];
     __shared__ double s_poly[DEG + 1];
+    __shared__ double s_polyf[DEGF + 1];
 
     // framework and tile of this workgroup
     const int b = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -105,14 +107,11 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
         const double slope = a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED ? a.ljtab[3 * kPairs + i] : 0.0;
         (&tab[0][0])[i] = make_double4(e4 * s6 * s6, e4 * s6, vrc - a.rc_lj * slope, slope);
     }
-    // the erf polynomial goes through LDS so that each thread holds it in vector registers (as
-    // scalar registers it would not fit beside the descriptor and be spilled)
+    // the erf polynomials go through LDS: each pair loop loads its own into vector registers (as
+    // scalar registers they would not fit beside the descriptor and be spilled)
     if (tid <= DEG) s_poly[tid] = COUL ? a.poly[tid] : 0.0;
-    __syncthreads();
-    double pc[DEG + 1];
-#pragma unroll
-    for (int k = 0; k <= DEG; ++k) pc[k] = s_poly[k];
-    const double pxs = a.poly_xs;
+    if (tid <= DEGF) s_polyf[tid] = COUL ? a.poly_far[tid] : 0.0;
+    const bool far_ok = COUL && a.far_ok;
 
     const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2];
     const int t0 = (n0 + TX - 1) / TX, t1 = (n1 + TY - 1) / TY;
@@ -165,9 +164,17 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
     const bool one_cut = NG == 0 || !COUL || rc2 == rcr2;
     const double cut2 = NG > 0 ? rc2 : rcr2;
 
-    // entries [lo_e, hi_e); INNER: every node of the tile is within both cutoffs
-    auto run = [&](auto inner_tag, int lo_e, int hi_e) {
+    // entries [lo_e, hi_e); INNER: every node of the tile is within both cutoffs; FAR: the shell's
+    // lower-degree erf polynomial
+    auto run = [&](auto inner_tag, auto far_tag, int lo_e, int hi_e) {
         constexpr bool INNER = decltype(inner_tag)::value;
+        constexpr bool FAR = decltype(far_tag)::value;
+        constexpr int D = FAR ? DEGF : DEG;
+        double pc[D + 1];
+#pragma unroll
+        for (int k = 0; k <= D; ++k) pc[k] = FAR ? s_polyf[k] : s_poly[k];
+        const double pxs = FAR ? a.poly_far_xs : a.poly_xs;
+        const double pxo = FAR ? a.poly_far_xo : -1.0;
 #pragma unroll 2
         for (int e = lo_e; e < hi_e; ++e) {
             const double2 xy = e_xy[e], zq = e_zq[e];
@@ -192,7 +199,7 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
                     }
                 };
                 auto coul = [&](double ir) {
-                    if constexpr (COUL) accq[p] = fma(zq.y, ir - erf_over_r_poly<DEG>(r2, pc, pxs), accq[p]);
+                    if constexpr (COUL) accq[p] = fma(zq.y, ir - erf_over_r_poly<D>(r2, pc, pxs, pxo), accq[p]);
                 };
                 if (INNER || one_cut) {
                     if (INNER || r2 <= cut2) {
@@ -215,8 +222,9 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
     auto compute = [&](int n_in, int n_out) {
         __syncthreads();
         if (any_valid) {
-            run(std::true_type{}, 0, n_in);
-            run(std::false_type{}, kCap - n_out, kCap);
+            run(std::true_type{}, std::false_type{}, 0, n_in);
+            if (far_ok) run(std::false_type{}, std::true_type{}, kCap - n_out, kCap);
+            else run(std::false_type{}, std::false_type{}, kCap - n_out, kCap);
         }
         __syncthreads();
     };

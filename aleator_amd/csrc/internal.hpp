@@ -70,6 +70,10 @@ constexpr int kErfPolyMax = 28;     // highest degree of the erf kernel polynomi
 // whose dropped Chebyshev coefficients of int_0^1 exp(-alpha^2 r^2 t^2) dt sum below tol (28
 // when none does); it is returned, and coef receives degree + 1 values.
 int erf_kernel_poly(double alpha, double rmax, double tol, int min_deg, double* coef, double* xs);
+// The same kernel over r in [rmin, rmax] at a given degree, in x = r^2 xs + xo; true when the
+// dropped Chebyshev coefficients sum below tol.
+bool erf_kernel_poly_range(double alpha, double rmin, double rmax, double tol, int deg, double* coef, double* xs,
+                           double* xo);
 
 // ---------------------------------------------------------------- host lattice
 // Row-vector lattice (core/geometry/lattice.hpp:21-30) with the reciprocal-vector

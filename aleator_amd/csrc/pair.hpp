@@ -43,10 +43,10 @@ __device__ inline double rcp_refined(double x) {
 }
 
 // erf(alpha r)/r from s = r^2: Horner's rule over the DEG + 1 coefficients c (lowest first) in
-// x = s xs - 1.
+// x = s xs + xo (xo = -1 for a fit over r in [0, rmax]).
 template <int DEG>
-__device__ inline double erf_over_r_poly(double s, const double (&c)[DEG + 1], double xs) {
-    const double x = fma(s, xs, -1.0);
+__device__ inline double erf_over_r_poly(double s, const double (&c)[DEG + 1], double xs, double xo = -1.0) {
+    const double x = fma(s, xs, xo);
     double p = c[DEG];
 #pragma unroll
     for (int k = DEG - 1; k >= 0; --k) p = fma(p, x, c[k]);
