@@ -33,6 +33,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X published dense FP32 vector
 PEAK_FP64_TFLOPS = 78.6
+# tile shapes of the grid kernel by index: nodes per tile along x, y, z and nodes per thread along z
+TILE_SHAPES = {0: (8, 8, 4, 1), 1: (8, 8, 4, 2), 2: (8, 8, 4, 4), 3: (4, 8, 8, 2)}
 # algorithmic FLOP per (node, atom within the cutoff) pair
 F_LJ = 20.0
 F_REAL = 30.0
@@ -95,7 +97,8 @@ def c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=1_000_000, learned_max
     pressure over the last segment of its run (plain_steps) with that mean's standard error sigma.
     A method's time is the GPU-seconds until its segment-mean loading first reaches
     (plain final - sigma) at every pressure; the learned time includes the density's construction
-    (two LJ grid tables, 2e6-sample training set, MLP training, bin table)."""
+    (two LJ grid tables, the host-side draw of the 2e6-sample training set, MLP training, bin
+    table)."""
     from aleator_amd import learned, workloads as wl
 
     chains = wl.c3_chains()
@@ -109,13 +112,14 @@ def c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=1_000_000, learned_max
         return r, time.perf_counter() - t0
 
     tabs, t_grid = once(lambda: fw.grid([4, 5], w["counts"], precision=aa.MIXED, grid_lattice=w["prim"]))
-    frac, e = learned.sample_grid(tabs, [1.0, 2.0], w["counts"], 2_000_000, low_fraction=0.5, low_cut=1000.0)
+    (frac, e), t_sample = once(lambda: learned.sample_grid(tabs, [1.0, 2.0], w["counts"], 2_000_000,
+                                                           low_fraction=0.5, low_cut=1000.0))
     ld = learned.LearnedDensity(ctx, max_rows=8192)
     losses, t_train = once(lambda: ld.fit(frac, e, e_clip=1000.0, n_iters=3000, batch=4096, lr=1e-3))
     wts, t_table = once(lambda: ld.bin_weights(nb, 298.0))
-    t_density = t_grid + t_train + t_table
+    t_density = t_grid + t_sample + t_train + t_table
 
-    def trajectory(weights, max_steps, target=None, sigma=None):
+    def trajectory(weights, max_steps, target=None, sigma=None, keep=False):
         g = aa.Gcmc(fw, wl.CO2_SITES, chains, max_molecules=1024)
         if weights is not None:
             g.set_proposal(weights, nb, w["reps"], 0.05)
@@ -132,19 +136,31 @@ def c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=1_000_000, learned_max
             if target is not None and np.all(segs[-1][1] >= target - sigma):
                 break
         errors = sum(s["error"] for s in g.stats())
+        if keep:
+            return segs, acc / max(att, 1), errors, g
         g.close()
         return segs, acc / max(att, 1), errors
 
     plain, plain_acc, e1 = trajectory(None, plain_steps)
     target, sigma = plain[-1][1], plain[-1][2]
     t_plain = next(t for t, m, _ in plain if np.all(m >= target - sigma))
-    lrn, lrn_acc, e2 = trajectory(wts, learned_max, target, sigma)
+    lrn, lrn_acc, e2, g_filled = trajectory(wts, learned_max, target, sigma, keep=True)
     reached = bool(np.all(lrn[-1][1] >= target - sigma))
+    # the c3 chains at that loading (the filled cells), continued with uniform insertion: the
+    # per-step cost of the c3 workload at a representative occupancy
+    g_filled.set_proposal(None, nb, w["reps"])
+    g_filled.reset_stats()
+    _, dt_filled = once(lambda: g_filled.run(5000))
+    st_f = g_filled.stats()
+    c3_filled = {"c3_filled_chain_steps_per_s": len(chains) * 5000 / dt_filled,
+                 "c3_filled_mean_N": float(np.mean([s_["sum_molecules"] / s_["steps"] for s_ in st_f])),
+                 "c3_filled_us_per_step": dt_filled / 5000 * 1e6}
+    g_filled.close()
     t_learned = lrn[-1][0] + t_density if reached else None
     return {
         "c4_density": {"train_samples": 2_000_000, "adam_steps": 3000, "batch": 4096,
                        "final_loss_std_units": float(losses[-100:].mean()), "bins": list(nb),
-                       "gpu_s_grid_train_table": [t_grid, t_train, t_table]},
+                       "s_grid_sample_train_table": [t_grid, t_sample, t_train, t_table]},
         "c4_plain_steps": len(plain) * seg, "c4_plain_gpu_s": plain[-1][0],
         "c4_plain_final_loading": target.round(3).tolist(), "c4_plain_final_sigma": sigma.round(3).tolist(),
         "c4_plain_gpu_s_to_loading": t_plain,
@@ -153,6 +169,7 @@ def c4_learned_vs_plain(aa, ctx, fw, w, sync, plain_steps=1_000_000, learned_max
         "c4_speedup_gpu_s": (t_plain / t_learned) if t_learned else None,
         "c4_insertion_acceptance": {"plain": plain_acc, "learned": lrn_acc},
         "c4_errors": e1 + e2,
+        **c3_filled,
     }
 
 
@@ -656,22 +673,25 @@ def main():
     # ---------------- extras
     extra = {"nodes_per_s": nodes / t_step, "tile_kernel_ms": k_avg * 1e3, "pairs_per_node": nbar,
              "tile_shape": tile, "tile_atoms_per_pass": plan.atoms_per_pass(), "autotune_ms_per_shape": tuning}
-    # the same grid in the other precision (FP64 beside the MIXED headline), same plan machinery
-    other = aa.FP64 if prec == aa.MIXED else aa.MIXED
-    other_name = "fp64" if prec == aa.MIXED else "mixed"
-    out2 = torch.empty(3 * nodes, dtype=torch.float64, device="cuda")
-    plan_o = aa.GridPlan(ctx, [fw], [4, 5], [w["counts"]], [out2.data_ptr()], coulomb=True, precision=other,
-                         grid_lattices=[w["prim"]], tile=-1, use_graph=not args.no_graph)
-    ctx.timing(True)
-    t_o = timed(plan_o.execute, max(3, args.steps // 2), 1, sync)
-    ko_ms, ko_n = ctx.timing(False)
-    extra[f"c1_grid_{other_name}_pts_per_s"] = 3 * nodes / t_o
-    extra[f"c1_grid_{other_name}_tile_kernel_ms"] = ko_ms / max(ko_n, 1)
-    extra[f"c1_grid_{other_name}_tile_shape"] = plan_o.tile()[0]
-    tables_other = out2.cpu().numpy().reshape(3, nodes)
-    plan_o.close()
-    del out2
-    tables_mixed, tables_fp64 = (tables_main, tables_other) if prec == aa.MIXED else (tables_other, tables_main)
+    # the same grid in the other precision (FP64 beside the MIXED headline), same plan machinery;
+    # profiling runs (--headline-only) leave it out so that their counters cover the headline alone
+    tables_mixed = tables_fp64 = None
+    if not args.headline_only:
+        other = aa.FP64 if prec == aa.MIXED else aa.MIXED
+        other_name = "fp64" if prec == aa.MIXED else "mixed"
+        out2 = torch.empty(3 * nodes, dtype=torch.float64, device="cuda")
+        plan_o = aa.GridPlan(ctx, [fw], [4, 5], [w["counts"]], [out2.data_ptr()], coulomb=True, precision=other,
+                             grid_lattices=[w["prim"]], tile=-1, use_graph=not args.no_graph)
+        ctx.timing(True)
+        t_o = timed(plan_o.execute, max(3, args.steps // 2), 1, sync)
+        ko_ms, ko_n = ctx.timing(False)
+        extra[f"c1_grid_{other_name}_pts_per_s"] = 3 * nodes / t_o
+        extra[f"c1_grid_{other_name}_tile_kernel_ms"] = ko_ms / max(ko_n, 1)
+        extra[f"c1_grid_{other_name}_tile_shape"] = plan_o.tile()[0]
+        tables_other = out2.cpu().numpy().reshape(3, nodes)
+        plan_o.close()
+        del out2
+        tables_mixed, tables_fp64 = (tables_main, tables_other) if prec == aa.MIXED else (tables_other, tables_main)
     if args.copy_calibration:
         # a plain device copy of known size, for scaling HBM byte counters (FETCH_SIZE, WRITE_SIZE)
         src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
@@ -716,7 +736,7 @@ def main():
     from aleator_amd import workloads as wl
 
     if "c2" in legs:
-        # c2: 256 frameworks, one batched (graph-captured, autotuned) launch per step, FP32 pair terms
+        # c2: 256 frameworks, one batched (graph-captured, autotuned) launch per step, MIXED pair terms
         try:
             fws2, eps2, sig2 = wl.c2(256)
             devs = [aa.Framework(ctx, f["sup"], f["x"], f["y"], f["z"], f["sp"], eps2, sig2, f["rc"]) for f in fws2]
@@ -844,7 +864,8 @@ def main():
                    "nodes": nodes, "tables": 3, "precision": args.precision},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(tile),
-                     "kernel": f"grid_tile_kernel<{prec == aa.FP64}, 2, {erf_deg}, false, shape {tile}>",
+                     "kernel": "grid_tile_kernel<{}, 2, {}, 0, {}, {}, {}, {}>".format(
+                         "true" if prec == aa.FP64 else "false", erf_deg, *TILE_SHAPES[tile]),
                      "flop_per_launch": flop,
                      "peak_note": "FP64 vector peak: the pair arithmetic is FP64 (FP32 peak would be 157.3)"},
         "cpu_baseline": cpu,
