@@ -295,6 +295,34 @@ void* Context::ensure_scratch(size_t bytes) {
     return scratch;
 }
 
+void retain(Context* c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void release(Context* c) {
+    if (c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (c->scratch) (void)hipFree(c->scratch);
+    (void)hipStreamDestroy(c->stream);
+    delete static_cast<aleator_ctx*>(c);
+}
+
+void retain(Framework* f) { f->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void release(Framework* f) {
+    if (f->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    Context* c = f->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    for (void* p : {(void*)f->d_fx, (void*)f->d_fy, (void*)f->d_fz, (void*)f->d_q, (void*)f->d_type,
+                    (void*)f->d_kn, (void*)f->d_kw, (void*)f->d_kseg, (void*)f->d_sre, (void*)f->d_sim,
+                    (void*)f->d_widom_tab})
+        if (p) (void)hipFree(p);
+    delete static_cast<aleator_framework*>(f);
+    release(c);
+}
+
 }  // namespace aleator_gpu
 
 using namespace aleator_gpu;
@@ -338,15 +366,7 @@ aleator_status aleator_gpu_timing(aleator_ctx* ctx, int32_t enable, double* tota
 
 aleator_status aleator_gpu_destroy(aleator_ctx* ctx) {
     return guarded([&] {
-        if (!ctx) return;
-        (void)hipStreamSynchronize(ctx->stream);
-        for (auto& e : ctx->events) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
-        if (ctx->scratch) (void)hipFree(ctx->scratch);
-        (void)hipStreamDestroy(ctx->stream);
-        delete ctx;
+        if (ctx) release(ctx);
     });
 }
 
@@ -473,19 +493,14 @@ aleator_status aleator_gpu_framework_create(aleator_ctx* ctx, const aleator_latt
             fw->self_energy = -kCoulomb * (fw->alpha / std::sqrt(kPi)) * q2;
         }
         ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+        retain(ctx);
         *out = fw.release();
     });
 }
 
 aleator_status aleator_gpu_framework_destroy(aleator_framework* fw) {
     return guarded([&] {
-        if (!fw) return;
-        (void)hipStreamSynchronize(fw->ctx->stream);
-        for (void* p : {(void*)fw->d_fx, (void*)fw->d_fy, (void*)fw->d_fz, (void*)fw->d_q, (void*)fw->d_type,
-                        (void*)fw->d_kn, (void*)fw->d_kw, (void*)fw->d_kseg, (void*)fw->d_sre, (void*)fw->d_sim,
-                        (void*)fw->d_widom_tab})
-            if (p) (void)hipFree(p);
-        delete fw;
+        if (fw) release(fw);
     });
 }
 
@@ -648,6 +663,7 @@ aleator_status aleator_gpu_ewald_state_create(aleator_framework* fw, aleator_ewa
         ALEATOR_HIP_CHECK(hipStreamSynchronize(fw->ctx->stream));
         st->self_energy = fw->self_energy;
         st->exclusion_energy = 0.0;  // the framework carries no exclusion (ewald_incremental_state.cc:82)
+        retain(fw);
         *out = st.release();
     });
 }
@@ -659,7 +675,9 @@ aleator_status aleator_gpu_ewald_state_destroy(aleator_ewald_state* st) {
         (void)hipFree(st->d_sre);
         (void)hipFree(st->d_sim);
         (void)hipFree(st->d_out);
+        Framework* fw = st->fw;
         delete st;
+        release(fw);
     });
 }
 

@@ -315,6 +315,7 @@ aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_m
         a.mol_pos = g->d_pos;
         a.sre = g->d_sre;
         a.sim = g->d_sim;
+        retain(fw);
         *out = g.release();
     });
 }
@@ -537,7 +538,9 @@ aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* g) {
                         (void*)g->d_sre, (void*)g->d_sim, (void*)g->d_prop, (void*)g->d_bin_start,
                         (void*)g->d_bin_pos, (void*)g->d_bin_type, (void*)g->d_grid})
             if (p) (void)hipFree(p);
+        Framework* fw = g->fw;
         delete g;
+        release(fw);
     });
 }
 

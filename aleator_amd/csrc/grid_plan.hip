@@ -19,6 +19,7 @@
 
 struct aleator_grid_plan {
     aleator_gpu::Context* ctx = nullptr;
+    std::vector<aleator_gpu::Framework*> fws;  // one reference held on each framework
     std::vector<aleator_gpu::GridJob> jobs;
     int shape = 0;
     int chunk = 128;  // atoms per list-build pass
@@ -208,6 +209,11 @@ aleator_status aleator_gpu_grid_plan_create(aleator_ctx* ctx_in, int32_t n_frame
             plan_free(p.get());
             throw;
         }
+        for (int f = 0; f < n_frameworks; ++f) {
+            auto* fw = reinterpret_cast<Framework*>(fws[f]);
+            retain(fw);
+            p->fws.push_back(fw);
+        }
         *out = p.release();
     });
 }
@@ -248,7 +254,9 @@ aleator_status aleator_gpu_grid_plan_destroy(aleator_grid_plan* p) {
         if (!p) return;
         (void)hipStreamSynchronize(p->ctx->stream);
         plan_free(p);
+        const std::vector<Framework*> fws = std::move(p->fws);
         delete p;
+        for (Framework* fw : fws) release(fw);
     });
 }
 

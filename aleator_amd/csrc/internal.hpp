@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -97,7 +98,12 @@ struct KSet {
 KSet make_kset(const HostLattice& lat, double alpha, int kmax);
 
 // ---------------------------------------------------------------- objects
+// A context and a framework are reference counted.  The caller's destroy call drops the
+// caller's reference, and every object created on a context or a framework holds one more, so
+// what a child object uses stays valid until the child itself is destroyed, in whatever order
+// the destroy calls come (a garbage collector finalises unreachable objects in no set order).
 struct Context {
+    std::atomic<int> refs{1};
     int device = 0;
     hipStream_t stream = nullptr;
     // reusable device scratch
@@ -115,6 +121,7 @@ struct Context {
 };
 
 struct Framework {
+    std::atomic<int> refs{1};
     Context* ctx = nullptr;
     HostLattice lat{};
     uint64_t n = 0;
@@ -152,6 +159,11 @@ struct Framework {
     double* d_widom_tab = nullptr;
     std::vector<uint32_t> widom_tab_species;
 };
+
+void retain(Context* c);
+void release(Context* c);  // frees the context with its last reference
+void retain(Framework* f);
+void release(Framework* f);  // frees the framework with its last reference, then drops its context
 
 struct EwaldState {
     Framework* fw = nullptr;

@@ -413,12 +413,14 @@ Mlp::Mlp(Context* c, int max_rows) : ctx(c), cap(max_rows) {
         for (void* p : buffers) (void)hipFree(p);
         throw;
     }
+    retain(ctx);
 }
 
 Mlp::~Mlp() {
     (void)hipStreamSynchronize(ctx->stream);
     drop_graph();
     for (void* p : buffers) (void)hipFree(p);
+    release(ctx);
 }
 
 void Mlp::set_params(const float* host) {
