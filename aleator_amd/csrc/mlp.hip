@@ -1,5 +1,5 @@
 // mlp.hip -- the learned insertion proposal: a Fourier-feature MLP trained and evaluated on the
-// GPU with hand-written BF16 MFMA GEMMs (FP32 accumulation).  No reference counterpart.
+// GPU with hand-written BF16 MFMA kernels (FP32 accumulation).  No reference counterpart.
 //
 // Network: 64 Fourier features of the fractional position -> 4 hidden layers of width 256
 // with SiLU -> 1 output (214,273 parameters).  Features: sin and cos of 2*pi*(k . f) for 32
@@ -7,15 +7,28 @@
 // a = 0..2; 18 vectors), 2^j (e_a + e_b) (j = 0..3, three axis pairs; 12), 2^j (1,1,1)
 // (j = 0..1; 2).
 //
-// GEMM: C[M x N] = A[M x K] * B[K x N] with A, B in BF16 (arbitrary strides, staged through
-// LDS) and FP32 accumulation on v_mfma_f32_16x16x16_bf16: a 256-thread workgroup computes a
-// 64 x 64 block of C, each wave a 32 x 32 quarter as 2 x 2 MFMA tiles, K in steps of 16.
-// Operand layout of the 16x16x16 instruction (64 lanes): A lane l holds A[l%16][4(l/16)..+3],
-// B lane l holds B[4(l/16)..+3][l%16], D lane l holds D[4(l/16)..+3][l%16].
-// Epilogues fuse bias + SiLU (forward), the SiLU derivative (backward) and FP32 stores.
-// Training is Adam on FP32 master weights; BF16 copies feed the MFMA.
+// A training iteration is three kernels:
+//  1. mlp_rows_kernel, one workgroup of 8 waves per block of 32 rows: draws the rows, computes
+//     their features, runs the 4 hidden layers forward, the head and the loss, and propagates
+//     dL/dz back to the first hidden layer, with the block's activations in LDS between layers.
+//     Wave w owns output columns [32 w, 32 w + 32) of every layer (2 x 2 tiles of
+//     v_mfma_f32_16x16x32_bf16) and keeps their pre-activations in registers from the forward
+//     to the backward pass.  Weights are the B operands, read from global memory in MFMA
+//     fragment order (one 16-byte load per lane and fragment).  The block's features, hidden
+//     activations and dL/dz go to global memory block-transposed (the block's 32 rows of a
+//     column contiguous), with per-block sums for the bias and output-layer gradients.
+//  2. mlp_wgrad_kernel: dW_l = H_{l-1}^T dZ_l in 64 x 64 tiles over ranges of row blocks; a
+//     32-row block is one MFMA K step whose operands are 16-byte loads of the block-transposed
+//     arrays.
+//  3. mlp_update_kernel: sums the partial gradients, then Adam on the FP32 master weights,
+//     refreshing the bf16 fragment copies.
+// Every sum runs in a fixed order, so results are reproducible bit for bit.
+//
+// Operand layout of v_mfma_f32_16x16x32_bf16 (64 lanes): A lane l holds A[l % 16][8 (l / 16) ..
+// + 7], B lane l holds B[8 (l / 16) .. + 7][l % 16], D lane l holds D[4 (l / 16) .. + 3][l % 16].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -28,7 +41,7 @@ namespace aleator_gpu {
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline uint16_t f2bf(float f) {
@@ -40,300 +53,553 @@ __device__ inline uint16_t f2bf(float f) {
 }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
-__device__ inline float silu_grad(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
-    return s * (1.0f + z * (1.0f - s));
+// bf16 of two floats, round to nearest even (v_cvt_pk_bf16_f32), packed low | high << 16
+__device__ inline uint32_t f2bf2(float lo, float hi) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2{lo, hi}), bf16x2));
 }
 
-constexpr int kBM = 64, kBN = 64, kBK = 16, kKC = 256;  // tile, MFMA K step, staged K chunk
+// sigmoid(z) = 1 / (1 + exp(-z)) from the hardware exponential and reciprocal (about 1 ulp each);
+// silu(z) = z sigmoid(z) and silu'(z) = sigmoid(z) (1 + z (1 - sigmoid(z)))
+__device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
-enum Epilogue : int {
-    kStoreF32 = 0,   // C (fp32)
-    kBiasSilu = 1,   // Z = C + bias (fp32), H = silu(Z) (bf16)
-    kSiluGrad = 2,   // dZ = C * silu'(Zprev) (bf16)
+constexpr int kRB = kMlpBlockRows;
+constexpr int kRowThreads = 512;             // 8 waves
+constexpr int kActStride = kMlpHidden + 8;   // LDS row stride (bf16) of an activation block
+
+enum Mode : int { kPredict = 0, kGrad = 1, kTrain = 2 };
+
+// Position of W_l[i][j] (i < fan_in, j < 256) in the forward copy, the B operand of H W_l:
+// fragment (kt = i / 32, nt = j / 16) of 64 lanes x 8 values, lane (i % 32) / 8 * 16 + j % 16,
+// value i % 8; fragments nt-major.
+__host__ __device__ inline int frag_fwd(int fan_in, int i, int j) {
+    const int kt = i >> 5, nt = j >> 4, lane = ((i & 31) >> 3) * 16 + (j & 15);
+    return ((nt * (fan_in >> 5) + kt) * 64 + lane) * 8 + (i & 7);
+}
+// Position of W_l[i][j] in the backward copy, the B operand of dZ W_l^T (K index j, N index i).
+__host__ __device__ inline int frag_bwd(int i, int j) {
+    const int kt = j >> 5, nt = i >> 4, lane = ((j & 31) >> 3) * 16 + (i & 15);
+    return ((nt * 8 + kt) * 64 + lane) * 8 + (j & 7);
+}
+__host__ __device__ inline int wf_offset(int l) {
+    return l == 0 ? 0 : kMlpIn * kMlpHidden + (l - 1) * kMlpHidden * kMlpHidden;
+}
+__host__ __device__ inline int wb_offset(int l) { return (l - 1) * kMlpHidden * kMlpHidden; }  // l = 1..3
+// block-transposed layer inputs of nb blocks: X (64 columns), then H1, H2, H3 (256 each)
+__host__ __device__ inline size_t ht_offset(int l, int nb) {
+    return l == 0 ? 0 : (size_t)nb * kMlpIn * kRB + (size_t)(l - 1) * nb * kMlpHidden * kRB;
+}
+constexpr size_t kHtPerBlock = (size_t)(kMlpIn + 3 * kMlpHidden) * kRB;
+constexpr size_t kDztPerBlock = (size_t)4 * kMlpHidden * kRB;
+
+__device__ inline bf16x8 ld8(const uint16_t* p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+
+struct RowArgs {
+    const double* frac;
+    const double* target;
+    uint64_t n_samples;     // kTrain: rows are drawn from frac[0 .. n_samples)
+    uint64_t seed, stream;  // kTrain: row r of iteration c is floor(u n) with u = Philox(seed, stream) at c rows + r
+    int rows;
+    int mode;
+    const int8_t* kvec;
+    const uint16_t* wf;
+    const uint16_t* wb;
+    const float* w;  // fp32 master parameters (biases, w5, b5)
+    uint64_t* counters;
+    float* y_out;     // kPredict
+    uint16_t* ht;     // block-transposed X, H1..H3
+    uint16_t* dzt;    // block-transposed dZ1..dZ4
+    float* blockpart; // [block][kMlpBlockPart]
 };
 
-struct GemmArgs {
-    const uint16_t* a;
-    long long sam, sak;
-    const uint16_t* b;
-    long long sbk, sbn;
-    int m, n, k;
-    int epilogue;
-    float* c;          // fp32 out [m][n] (kStoreF32, kBiasSilu: Z); split-K: partials [split][m][n]
-    const float* bias; // [n]
-    uint16_t* h;       // bf16 out [m][n] (kBiasSilu: H, kSiluGrad: dZ)
-    const float* zref; // [m][n] (kSiluGrad)
-    int ones_row;      // row of A that reads as 1 (bias gradients as one more GEMM row), or -1
-    int kchunk;        // K per split (a multiple of kBK); gridDim.z splits
-};
-
-// Load one element of A / B (bounds and the ones row handled).
-__device__ inline uint16_t a_at(const GemmArgs& g, int m, int k, int kend) {
-    if (k >= kend) return 0;
-    if (m == g.ones_row) return 0x3f80;  // 1.0
-    return m < g.m ? g.a[m * g.sam + k * g.sak] : (uint16_t)0;
-}
-__device__ inline uint16_t b_at(const GemmArgs& g, int k, int n, int kend) {
-    return (k < kend && n < g.n) ? g.b[k * g.sbk + n * g.sbn] : (uint16_t)0;
-}
-
-typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
-
-// Stage a kBM x kc chunk of A as As[m][k] and a kc x kBN chunk of B as Bs[n][k] (k contiguous in
-// both, the MFMA operand order), with 16-byte global loads along whichever of the two indices
-// is contiguous in memory; groups that cross a bound are loaded element by element.
-__device__ inline void stage(const GemmArgs& g, int m0, int n0, int k0, int kc, int kend, uint16_t (*As)[kKC + 8],
-                             uint16_t (*Bs)[kKC + 8]) {
-    const int tid = threadIdx.x;
-    // A
-    if (g.sak == 1) {  // rows contiguous in k
-        for (int i = tid; i < kBM * (kc / 8); i += 256) {
-            const int r = i / (kc / 8), c8 = (i % (kc / 8)) * 8;
-            const int gm = m0 + r, gk = k0 + c8;
-            u16x8 v;
-            if (gm < g.m && gm != g.ones_row && gk + 8 <= kend) {
-                v = *reinterpret_cast<const u16x8*>(g.a + gm * g.sam + gk);
-            } else {
-                for (int t = 0; t < 8; ++t) v[t] = a_at(g, gm, gk + t, kend);
-            }
-            *reinterpret_cast<u16x8*>(&As[r][c8]) = v;
-        }
-    } else if (g.sam != 1) {  // neither index contiguous: element by element
-        for (int i = tid; i < kBM * kc; i += 256) As[i / kc][i % kc] = a_at(g, m0 + i / kc, k0 + i % kc, kend);
-    } else {  // columns contiguous in m
-        for (int i = tid; i < (kBM / 8) * kc; i += 256) {
-            const int k = i / (kBM / 8), r8 = (i % (kBM / 8)) * 8;
-            const int gm = m0 + r8, gk = k0 + k;
-            u16x8 v;
-            if (gm + 8 <= g.m && gk < kend && (g.ones_row < gm || g.ones_row >= gm + 8)) {
-                v = *reinterpret_cast<const u16x8*>(g.a + gm + gk * g.sak);
-            } else {
-                for (int t = 0; t < 8; ++t) v[t] = a_at(g, gm + t, gk, kend);
-            }
-            for (int t = 0; t < 8; ++t) As[r8 + t][k] = v[t];
-        }
-    }
-    // B
-    if (g.sbk == 1) {  // columns of B contiguous in k
-        for (int i = tid; i < kBN * (kc / 8); i += 256) {
-            const int n = i / (kc / 8), c8 = (i % (kc / 8)) * 8;
-            const int gn = n0 + n, gk = k0 + c8;
-            u16x8 v;
-            if (gn < g.n && gk + 8 <= kend) {
-                v = *reinterpret_cast<const u16x8*>(g.b + gk + gn * g.sbn);
-            } else {
-                for (int t = 0; t < 8; ++t) v[t] = b_at(g, gk + t, gn, kend);
-            }
-            *reinterpret_cast<u16x8*>(&Bs[n][c8]) = v;
-        }
-    } else if (g.sbn != 1) {
-        for (int i = tid; i < kBN * kc; i += 256) Bs[i / kc][i % kc] = b_at(g, k0 + i % kc, n0 + i / kc, kend);
-    } else {  // rows of B contiguous in n
-        for (int i = tid; i < (kBN / 8) * kc; i += 256) {
-            const int k = i / (kBN / 8), n8 = (i % (kBN / 8)) * 8;
-            const int gn = n0 + n8, gk = k0 + k;
-            u16x8 v;
-            if (gn + 8 <= g.n && gk < kend) {
-                v = *reinterpret_cast<const u16x8*>(g.b + gk * g.sbk + gn);
-            } else {
-                for (int t = 0; t < 8; ++t) v[t] = b_at(g, gk, gn + t, kend);
-            }
-            for (int t = 0; t < 8; ++t) Bs[n8 + t][k] = v[t];
-        }
-    }
-}
-
-// One workgroup: a kBM x kBN block of C over the K range of its split, staged kKC at a time.
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[kBM][kKC + 8];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[kBN][kKC + 8];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int kbeg = blockIdx.z * g.kchunk;
-    const int kend = min(g.k, kbeg + g.kchunk);
-    f32x4 acc[2][2];
+// acc[i][j] = act[16 i .. + 15][0 .. 32 KT) x W[0 .. 32 KT)[columns of n tile 2 wave + j], the
+// W columns from fragments frag (KT K steps per n tile).  All B fragments are loaded first.
+template <int KT>
+__device__ inline void block_gemm(const uint16_t (*act)[kActStride], const uint16_t* frag, int wave, int lane,
+                                  f32x4 (&acc)[2][2]) {
+    bf16x8 b[KT][2];
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[kt][j] = ld8(frag + (((2 * wave + j) * KT + kt) * 64 + lane) * 8);
+    // every load in flight before the first MFMA (the scheduler would otherwise interleave them
+    // and wait on one round trip per K step)
+    __builtin_amdgcn_sched_barrier(0);
+    const int ar = lane & 15, ak = 8 * (lane >> 4);
+#pragma unroll
     for (int i = 0; i < 2; ++i)
+#pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int kq = 4 * (lane >> 4), li = lane & 15;
-    for (int k0 = kbeg; k0 < kend; k0 += kKC) {
-        const int kc = min(kKC, (kend - k0 + 15) / 16 * 16);  // whole MFMA steps; the tail is zero
-        stage(g, m0, n0, k0, kc, kend, As, Bs);
-        __syncthreads();
-        for (int kk = 0; kk < kc; kk += kBK) {
-            bf16x4 af[2], bfr[2];
-            for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x4*>(&As[wm + 16 * i + li][kk + kq]);
-            for (int j = 0; j < 2; ++j) bfr[j] = *reinterpret_cast<const bf16x4*>(&Bs[wn + 16 * j + li][kk + kq]);
-            for (int i = 0; i < 2; ++i)
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt) {
+        bf16x8 a[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = ld8(&act[16 * i + ar][32 * kt + ak]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[kt][j], acc[i][j], 0, 0, 0);
     }
-    const int mrows = g.ones_row >= 0 ? g.ones_row + 1 : g.m;
-    const int rq = 4 * (lane >> 4), cl = lane & 15;
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
+}
+
+
+// 4 consecutive rows of one column (two packed bf16 pairs) into the LDS block and, when dst is
+// set, into a block-transposed array with nontemporal stores (read by the next kernel, not by
+// this workgroup)
+__device__ inline void put4(uint16_t (*out)[kActStride], int row, int col, uint32_t p01, uint32_t p23, uint16_t* dst) {
+    out[row][col] = (uint16_t)p01;
+    out[row + 1][col] = (uint16_t)(p01 >> 16);
+    out[row + 2][col] = (uint16_t)p23;
+    out[row + 3][col] = (uint16_t)(p23 >> 16);
+    if (dst) {
+        __builtin_nontemporal_store(p01, reinterpret_cast<uint32_t*>(dst));
+        __builtin_nontemporal_store(p23, reinterpret_cast<uint32_t*>(dst) + 1);
+    }
+}
+
+// Forward epilogue: Z = acc + bias, H = silu(Z) in bf16 to the LDS block `out` and, when ht is
+// set, to the block-transposed array; when training, gz keeps silu'(Z) for the backward pass.
+template <bool kKeep>
+__device__ inline void forward_out(const f32x4 (&acc)[2][2], f32x4 (&gz)[2][2], const float* bias,
+                                   uint16_t (*out)[kActStride], uint16_t* ht, int wave, int lane) {
+    const int rq = 4 * (lane >> 4);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int col = 32 * wave + 16 * j + (lane & 15);
+        const float b = bias[col];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            float h[4];
+#pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm + 16 * i + rq + r, gn = n0 + wn + 16 * j + cl;
-                if (gm >= mrows || gn >= g.n) continue;
-                const float v = acc[i][j][r];
-                const long long o = (long long)gm * g.n + gn;
-                if (g.epilogue == kStoreF32) {
-                    g.c[(long long)blockIdx.z * mrows * g.n + o] = v;
-                } else if (g.epilogue == kBiasSilu) {
-                    const float z = v + g.bias[gn];
-                    g.c[o] = z;
-                    g.h[o] = f2bf(silu(z));
-                } else {
-                    g.h[o] = f2bf(v * silu_grad(g.zref[o]));
+                const float z = acc[i][j][r] + b;
+                const float sg = sigmoid(z);
+                h[r] = z * sg;
+                if (kKeep) gz[i][j][r] = sg * (1.0f + z * (1.0f - sg));
+            }
+            put4(out, 16 * i + rq, col, f2bf2(h[0], h[1]), f2bf2(h[2], h[3]), ht ? ht + col * kRB + 16 * i + rq : nullptr);
+        }
+    }
+}
+
+// Backward epilogue: dZ = acc * silu'(Z) in bf16 to the LDS block `out` and the block-transposed
+// array dzt.
+__device__ inline void backward_out(const f32x4 (&acc)[2][2], const f32x4 (&gz)[2][2], uint16_t (*out)[kActStride],
+                                    uint16_t* dzt, int wave, int lane) {
+    const int rq = 4 * (lane >> 4);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int col = 32 * wave + 16 * j + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const f32x4 v = acc[i][j] * gz[i][j];
+            put4(out, 16 * i + rq, col, f2bf2(v[0], v[1]), f2bf2(v[2], v[3]), dzt + col * kRB + 16 * i + rq);
+        }
+    }
+}
+
+// column sums of a 32-row LDS block of dL/dz (bias gradients of the block): four interleaved
+// partial sums, combined in a fixed order
+__device__ inline void column_sums(const uint16_t (*blk)[kActStride], float* out) {
+    const int c = threadIdx.x;
+    if (c >= kMlpHidden) return;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < kRB; ++r) s[r & 3] += bf2f(blk[r][c]);
+    out[c] = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+// One block of 32 rows: features, forward, head; when training also the loss, dL/dz of every
+// hidden layer and the block's bias and output-layer gradient sums.
+template <bool kTrainPath>
+__global__ __launch_bounds__(kRowThreads) void mlp_rows_kernel(const RowArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t act[2][kRB][kActStride];
+    __shared__ double s_pos[kRB][3];
+    __shared__ float s_target[kRB];
+    __shared__ float s_dy[kRB];
+    __shared__ float s_err2[kRB];
+    __shared__ float s_bias[4][kMlpHidden];
+    __shared__ float s_w5[kMlpHidden + 1];  // w5, then b5
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int blk = blockIdx.x, row0 = blk * kRB, nb = gridDim.x;
+
+    // Warm this XCD's L2 with every weight fragment the layers will read, so that each layer's
+    // B operands hit in L2 instead of going to memory one layer at a time.  Workgroups are
+    // dispatched to the 8 XCDs round robin; the blocks of one XCD split its 128-byte lines.
+    uint32_t sink = 0;
+    {
+        constexpr int kFwdLines = kMlpFwdFrag * 2 / 128;
+        constexpr int kLines = kFwdLines + (kTrainPath ? kMlpBwdFrag * 2 / 128 : 0);
+        const int xcd = blk & 7, rank = blk >> 3, per_xcd = (nb - xcd + 7) >> 3;
+        const int per = (kLines + per_xcd - 1) / per_xcd;
+        const int end = min(kLines, (rank + 1) * per);
+        for (int line = rank * per + tid; line < end; line += kRowThreads) {
+            const uint16_t* p = line < kFwdLines ? a.wf + line * 64 : a.wb + (line - kFwdLines) * 64;
+            sink ^= *reinterpret_cast<const uint32_t*>(p);
+        }
+    }
+    if (tid < kMlpHidden) {
+#pragma unroll
+        for (int l = 0; l < 4; ++l) s_bias[l][tid] = a.w[mlp_offset(2 * l + 1) + tid];
+        s_w5[tid] = a.w[mlp_offset(8) + tid];
+    } else if (tid == kMlpHidden) {
+        s_w5[kMlpHidden] = a.w[mlp_offset(9)];
+    }
+
+    if (tid < kRB) {
+        const int g = row0 + tid;
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+        float t = 0.f;
+        if (g < a.rows) {
+            uint64_t p = (uint64_t)g;
+            if (a.mode == kTrain) {
+                const double u = philox_uniform(a.seed, a.stream, a.counters[0] * (uint64_t)a.rows + (uint64_t)g);
+                const uint64_t v = (uint64_t)(u * (double)a.n_samples);
+                p = v < a.n_samples ? v : a.n_samples - 1;
+            }
+            f0 = a.frac[3 * p];
+            f1 = a.frac[3 * p + 1];
+            f2 = a.frac[3 * p + 2];
+            if (kTrainPath) t = (float)a.target[p];
+        }
+        s_pos[tid][0] = f0;
+        s_pos[tid][1] = f1;
+        s_pos[tid][2] = f2;
+        s_target[tid] = t;
+    }
+    // one more Adam step (read by the update kernel; nothing in this launch reads it)
+    if (kTrainPath && a.mode == kTrain && blk == 0 && tid == 0) a.counters[1] += 1;
+    __syncthreads();
+    // features; rows past the end read zero
+    for (int e = tid; e < kRB * kMlpFreqs; e += kRowThreads) {
+        const int r = e / kMlpFreqs, c = e % kMlpFreqs;
+        uint16_t hs = 0, hc = 0;
+        if (row0 + r < a.rows) {
+            const double ph = a.kvec[3 * c] * s_pos[r][0] + a.kvec[3 * c + 1] * s_pos[r][1] + a.kvec[3 * c + 2] * s_pos[r][2];
+            double s, co;
+            sincospi(2.0 * ph, &s, &co);
+            hs = f2bf((float)s);
+            hc = f2bf((float)co);
+        }
+        act[0][r][2 * c] = hs;
+        act[0][r][2 * c + 1] = hc;
+    }
+    __syncthreads();
+    if (kTrainPath && tid < kMlpIn * 4) {  // X, block-transposed: 8 rows of one column per thread
+        const int col = tid >> 2, q = (tid & 3) * 8;
+        uint16_t v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) v[t] = act[0][q + t][col];
+        uint4 u;
+        u.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+        u.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+        u.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
+        u.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
+        *reinterpret_cast<uint4*>(a.ht + ((size_t)blk * kMlpIn + col) * kRB + q) = u;
+    }
+    auto ht_block = [&](int l) -> uint16_t* {
+        return kTrainPath ? a.ht + ht_offset(l, nb) + (size_t)blk * kMlpHidden * kRB : nullptr;
+    };
+    f32x4 acc[2][2];
+    f32x4 z[4][2][2];  // silu'(Z) of each hidden layer (training)
+    block_gemm<kMlpIn / 32>(act[0], a.wf + wf_offset(0), wave, lane, acc);
+    forward_out<kTrainPath>(acc, z[0], s_bias[0], act[1], ht_block(1), wave, lane);
+    __syncthreads();
+    block_gemm<kMlpHidden / 32>(act[1], a.wf + wf_offset(1), wave, lane, acc);
+    forward_out<kTrainPath>(acc, z[1], s_bias[1], act[0], ht_block(2), wave, lane);
+    __syncthreads();
+    block_gemm<kMlpHidden / 32>(act[0], a.wf + wf_offset(2), wave, lane, acc);
+    forward_out<kTrainPath>(acc, z[2], s_bias[2], act[1], ht_block(3), wave, lane);
+    __syncthreads();
+    block_gemm<kMlpHidden / 32>(act[1], a.wf + wf_offset(3), wave, lane, acc);
+    forward_out<kTrainPath>(acc, z[3], s_bias[3], act[0], nullptr, wave, lane);
+    __syncthreads();
+
+    // head: y = H4 . w5 + b5, one wave per 4 rows (lane-strided partial sums, then a butterfly)
+    const float* w5 = s_w5;
+    for (int q = 0; q < kRB / 8; ++q) {
+        const int r = wave * (kRB / 8) + q;
+        float s = 0.f;
+        for (int k = lane; k < kMlpHidden; k += 64) s += bf2f(act[0][r][k]) * w5[k];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        const float y = s + s_w5[kMlpHidden];
+        if (lane == 0) {
+            const bool valid = row0 + r < a.rows;
+            if (!kTrainPath) {
+                if (valid) a.y_out[row0 + r] = y;
+            } else {
+                const float d = valid ? y - s_target[r] : 0.f;
+                s_dy[r] = 2.0f * d / (float)a.rows;
+                s_err2[r] = d * d;
+            }
+        }
+    }
+    if constexpr (!kTrainPath) {
+        asm volatile("" ::"v"(sink));  // consumes the warm-up loads
+    } else {
+        __syncthreads();
+        float* part = a.blockpart + (size_t)blk * kMlpBlockPart;
+        // output-layer gradient sums of the block: dw5, db5, squared error
+        if (tid < kMlpHidden) {
+            float s[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int r = 0; r < kRB; ++r) s[r & 3] += bf2f(act[0][r][tid]) * s_dy[r];
+            part[4 * kMlpHidden + tid] = (s[0] + s[1]) + (s[2] + s[3]);
+        } else if (tid == kMlpHidden) {
+            float s = 0.f;
+            for (int r = 0; r < kRB; ++r) s += s_dy[r];
+            part[5 * kMlpHidden] = s;
+        } else if (tid == kMlpHidden + 1) {
+            float s = 0.f;
+            for (int r = 0; r < kRB; ++r) s += s_err2[r];
+            part[5 * kMlpHidden + 1] = s;
+        }
+        // dZ4 = dy w5 silu'(Z4)
+        {
+            uint16_t* dzt4 = a.dzt + ((size_t)3 * nb + blk) * kMlpHidden * kRB;
+            const int rq = 4 * (lane >> 4);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int col = 32 * wave + 16 * j + (lane & 15);
+                const float wc = w5[col];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    float v[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] = s_dy[16 * i + rq + r] * wc * z[3][i][j][r];
+                    put4(act[1], 16 * i + rq, col, f2bf2(v[0], v[1]), f2bf2(v[2], v[3]), dzt4 + col * kRB + 16 * i + rq);
                 }
             }
-}
-
-// out[o] = sum over splits of partial[split][o], in split order
-__global__ void split_reduce_kernel(const float* partial, int splits, long long n, float* out) {
-    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= n) return;
-    float s = 0.f;
-    for (int z = 0; z < splits; ++z) s += partial[z * n + o];
-    out[o] = s;
-}
-
-// Fourier features of fractional positions (optionally gathered through idx), bf16 [rows][64]
-__global__ void features_kernel(const double* frac, const uint32_t* idx, int rows, const int8_t* kv,
-                                uint16_t* x) {
-    const int r = blockIdx.x * blockDim.y + threadIdx.y;
-    const int c = threadIdx.x;  // 0..31 frequency
-    if (r >= rows) return;
-    const long long p = idx ? (long long)idx[r] : r;
-    const double f0 = frac[3 * p], f1 = frac[3 * p + 1], f2 = frac[3 * p + 2];
-    const double ph = kv[3 * c] * f0 + kv[3 * c + 1] * f1 + kv[3 * c + 2] * f2;
-    double s, co;
-    sincospi(2.0 * ph, &s, &co);
-    x[(long long)r * kMlpIn + 2 * c] = f2bf((float)s);
-    x[(long long)r * kMlpIn + 2 * c + 1] = f2bf((float)co);
-}
-
-// Output layer and loss: y = H4 . w5 + b5; dy = 2 (y - t) / B; dZ4 = dy w5 silu'(Z4).
-// One wave per row.
-__global__ void head_kernel(const uint16_t* h4, const float* z4, const float* w5, const float* b5, const double* target,
-                            const uint32_t* idx, int rows, float* y_out, float* dy_out, uint16_t* dz4, float* loss_rows,
-                            int backward) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + wave;
-    if (r >= rows) return;
-    float s = 0.f;
-    for (int k = lane; k < kMlpHidden; k += 64) s += bf2f(h4[(long long)r * kMlpHidden + k]) * w5[k];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    const float y = s + b5[0];
-    if (lane == 0 && y_out) y_out[r] = y;
-    if (!backward) return;
-    const long long p = idx ? (long long)idx[r] : r;
-    const float d = y - (float)target[p];
-    const float dy = 2.0f * d / (float)rows;
-    if (lane == 0) {
-        dy_out[r] = dy;
-        loss_rows[r] = d * d;
-    }
-    for (int k = lane; k < kMlpHidden; k += 64) {
-        const long long o = (long long)r * kMlpHidden + k;
-        dz4[o] = f2bf(dy * w5[k] * silu_grad(z4[o]));
-    }
-}
-
-// Output-layer gradients over one chunk of kHeadChunk rows (one block per chunk): partial[chunk][k] =
-// sum_r H4[r][k] dy[r] for k < 256, then sum_r dy[r] and sum_r (y - t)^2.
-constexpr int kHeadChunk = 32;
-__global__ void head_grad_partial_kernel(const uint16_t* h4, const float* dy, const float* loss_rows, int rows,
-                                         float* partial) {
-    const int k = threadIdx.x, r0 = blockIdx.x * kHeadChunk;
-    const int r1 = min(rows, r0 + kHeadChunk);
-    float s = 0.f;
-#pragma unroll 8
-    for (int r = r0; r < r1; ++r) s += bf2f(h4[(long long)r * kMlpHidden + k]) * dy[r];
-    float* out = partial + (long long)blockIdx.x * (kMlpHidden + 2);
-    out[k] = s;
-    if (k == 0) {
-        float sd = 0.f, sl = 0.f;
-        for (int r = r0; r < r1; ++r) {
-            sd += dy[r];
-            sl += loss_rows[r];
         }
-        out[kMlpHidden] = sd;
-        out[kMlpHidden + 1] = sl;
+        __syncthreads();
+        // dZ_{l-1} = (dZ_l W_l^T) * silu'(Z_{l-1}), l = 3, 2, 1 (0-based), with the bias sums of dZ_l
+        column_sums(act[1], part + 3 * kMlpHidden);
+        block_gemm<kMlpHidden / 32>(act[1], a.wb + wb_offset(3), wave, lane, acc);
+        backward_out(acc, z[2], act[0], a.dzt + ((size_t)2 * nb + blk) * kMlpHidden * kRB, wave, lane);
+        __syncthreads();
+        column_sums(act[0], part + 2 * kMlpHidden);
+        block_gemm<kMlpHidden / 32>(act[0], a.wb + wb_offset(2), wave, lane, acc);
+        backward_out(acc, z[1], act[1], a.dzt + ((size_t)1 * nb + blk) * kMlpHidden * kRB, wave, lane);
+        __syncthreads();
+        column_sums(act[1], part + 1 * kMlpHidden);
+        block_gemm<kMlpHidden / 32>(act[1], a.wb + wb_offset(1), wave, lane, acc);
+        backward_out(acc, z[0], act[0], a.dzt + (size_t)blk * kMlpHidden * kRB, wave, lane);
+        __syncthreads();
+        column_sums(act[0], part);
+        asm volatile("" ::"v"(sink));  // consumes the warm-up loads
     }
 }
 
-// dW5, db5 and the mean loss from the chunk partials
-// The mean loss goes to loss_hist[iteration mod kLossChunk] (the iteration counter counters[0]).
-// One wave per output k (258 blocks of 64): lanes take chunks l, l + 64, ..., then a fixed-order
-// wave reduction.
-__global__ void head_grad_reduce_kernel(const float* partial, int chunks, int rows, float* dw5, float* db5,
-                                        float* loss_hist, const uint64_t* counters) {
-    const int k = blockIdx.x, lane = threadIdx.x;  // k: 0 .. 257
-    float s = 0.f;
-    for (int c = lane; c < chunks; c += 64) s += partial[(long long)c * (kMlpHidden + 2) + k];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane != 0) return;
-    if (k < kMlpHidden)
-        dw5[k] = s;
-    else if (k == kMlpHidden)
-        db5[0] = s;
-    else
-        loss_hist[counters[0] % kLossChunk] = s / (float)rows;
+struct WgradArgs {
+    const uint16_t* ht;
+    const uint16_t* dzt;
+    float* partial;    // [split][kMlpParams] (weight entries only)
+    int nb;            // row blocks
+    int per_split;     // row blocks per split
+    int splits;
+    const float* blockpart;
+    float* colsum;     // [kMlpBlockPart]: blockpart summed over the blocks
+    uint64_t* counters;
+    int bump;          // training: advance the iteration counter (read by the next row kernel)
+};
+
+constexpr int kWgradTiles = 2 + 3 * 4;    // 128 x 128 tiles: W1 (64 rows: 2 tiles), then 4 each of W2, W3, W4
+constexpr int kWgradThreads = 512;        // 8 waves, a 32 x 64 quarter-of-a-half each
+constexpr int kWgradChunk = 8;            // row blocks staged in LDS at once (128 KB: one round trip per split)
+constexpr int kColsumCols = 16;           // columns per column-sum workgroup
+constexpr int kColsumGroups = kWgradThreads / kColsumCols;
+constexpr int kColsumBlocks = (kMlpBlockPart + kColsumCols - 1) / kColsumCols;
+
+// Workgroups [0, kWgradTiles * splits8) with splits8 = splits rounded up to a multiple of 8: the
+// 128 x 128 weight-gradient tiles of each split.  The workgroup stages kWgradChunk row blocks of
+// its 128 input columns (H, block-transposed) and 128 output columns (dL/dz) in LDS, 16 KB per
+// block, read from memory once; wave w then sums the 32 x 64 quarter (rows 32 (w / 2), columns
+// 64 (w % 2)) over the blocks, one MFMA K step per block.  Workgroups are dispatched to the 8 XCDs
+// round robin, so workgroup x runs on XCD x % 8; the tiles of split s all go to XCD s % 8, where
+// they share the split's activations in L2.  The kColsumBlocks after them: the per-block partial
+// sums (bias and output-layer gradients, squared error) summed over the blocks, 16 columns per
+// workgroup: thread (g, c) sums blocks g, g + 32, ... of column c, then the group sums are added
+// in order.
+__global__ __launch_bounds__(kWgradThreads) void mlp_wgrad_kernel(const WgradArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_op[kWgradChunk][2][128][kRB];  // [block][A, B][column][row]
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int splits8 = (a.splits + 7) & ~7;
+    if ((int)blockIdx.x >= kWgradTiles * splits8) {
+        float(*s_part)[kColsumCols] = reinterpret_cast<float(*)[kColsumCols]>(&s_op[0][0][0][0]);
+        const int g = tid / kColsumCols, c = tid % kColsumCols;
+        const int col = kColsumCols * (blockIdx.x - kWgradTiles * splits8) + c;
+        float s = 0.f;
+        if (col < kMlpBlockPart) {
+#pragma unroll 4
+            for (int b = g; b < a.nb; b += kColsumGroups) s += a.blockpart[(size_t)b * kMlpBlockPart + col];
+        }
+        s_part[g][c] = s;
+        __syncthreads();
+        if (g == 0 && col < kMlpBlockPart) {
+            float t = 0.f;
+#pragma unroll
+            for (int k = 0; k < kColsumGroups; ++k) t += s_part[k][c];
+            a.colsum[col] = t;
+        }
+        return;
+    }
+    const int r = blockIdx.x >> 3;
+    const int tile = r % kWgradTiles, split = (blockIdx.x & 7) + 8 * (r / kWgradTiles);
+    if (split >= a.splits) return;
+    int l, mt, nt;
+    if (tile < 2) {
+        l = 0;
+        mt = 0;
+        nt = tile;
+    } else {
+        l = 1 + (tile - 2) / 4;
+        mt = ((tile - 2) % 4) / 2;
+        nt = (tile - 2) % 2;
+    }
+    const int cols = l == 0 ? kMlpIn : kMlpHidden;
+    const int mrows = min(128, cols);  // input columns in this tile (64 for W1)
+    const uint16_t* A = a.ht + ht_offset(l, a.nb) + (size_t)mt * 128 * kRB;
+    const uint16_t* B = a.dzt + (size_t)l * a.nb * kMlpHidden * kRB + (size_t)nt * 128 * kRB;
+    const int wm = wave >> 1, wn = wave & 1;
+    const bool active = 32 * wm < mrows;
+    const int k8 = 8 * (lane >> 4), li = lane & 15;
+    const int b0 = split * a.per_split, b1 = min(a.nb, b0 + a.per_split);
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int base = b0; base < b1; base += kWgradChunk) {
+        // stage: per block 128 columns x 32 rows of A and of B (each 8 KB contiguous in memory)
+        uint4 v[kWgradChunk * 2];
+#pragma unroll
+        for (int u = 0; u < kWgradChunk; ++u) {
+            const int blk = base + u;
+            const bool ok = blk < b1;
+#pragma unroll
+            for (int o = 0; o < 2; ++o) {
+                const int e = tid * 8;  // 512 threads x 8 values = 128 columns x 32 rows
+                const bool in = ok && (o == 1 || e < mrows * kRB);
+                const uint16_t* src = o == 0 ? A + (size_t)blk * cols * kRB : B + (size_t)blk * kMlpHidden * kRB;
+                v[2 * u + o] = in ? *reinterpret_cast<const uint4*>(src + e) : make_uint4(0, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kWgradChunk; ++u)
+#pragma unroll
+            for (int o = 0; o < 2; ++o) *reinterpret_cast<uint4*>(&s_op[u][o][0][0] + tid * 8) = v[2 * u + o];
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < kWgradChunk; ++u) {
+                bf16x8 af[2], bfr[4];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) af[i] = ld8(&s_op[u][0][32 * wm + 16 * i + li][k8]);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bfr[j] = ld8(&s_op[u][1][64 * wn + 16 * j + li][k8]);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (active) {
+        float* out = a.partial + (size_t)split * kMlpParams + mlp_offset(2 * l);
+        const int rq = 4 * (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    out[(128 * mt + 32 * wm + 16 * i + rq + q) * kMlpHidden + 128 * nt + 64 * wn + 16 * j + li] = acc[i][j][q];
+    }
+    if (a.bump && tile == 0 && split == 0 && tid == 0) a.counters[0] += 1;
 }
 
-// Adam on fp32 master weights, step t = counters[1] + 1 (bias corrections 1 - b^t); refreshes
-// the bf16 copies of the GEMM weights
-__global__ void adam_kernel(float* w, const float* grad, float* m1, float* m2, uint16_t* wbf, int n, int n_bf,
-                            float lr, float b1, float b2, float eps, const uint64_t* counters) {
+struct UpdateArgs {
+    float* w;
+    float* grad;
+    float* m1;
+    float* m2;
+    uint16_t* wf;
+    uint16_t* wb;
+    const float* partial;
+    int splits;
+    const float* colsum;
+    int rows;
+    float lr;
+    int adam;         // 0: gradient and loss only
+    const uint64_t* counters;
+    float* loss_hist;
+};
+
+// region of parameter i: 0..9 = W1 b1 W2 b2 W3 b3 W4 b4 w5 b5
+__device__ inline int param_region(int i) {
+    int g = 0;
+#pragma unroll
+    for (int k = 1; k < 10; ++k) g += i >= mlp_offset(k);
+    return g;
+}
+
+// bf16 fragment copies of weight i (region g, even and below 8)
+__device__ inline void store_bf16_copies(uint16_t* wf, uint16_t* wb, int g, int i, uint16_t v) {
+    const int l = g >> 1, e = i - mlp_offset(g);
+    const int ii = e / kMlpHidden, jj = e % kMlpHidden;
+    wf[wf_offset(l) + frag_fwd(l == 0 ? kMlpIn : kMlpHidden, ii, jj)] = v;
+    if (l > 0) wb[wb_offset(l) + frag_bwd(ii, jj)] = v;
+}
+
+// Gradient of parameter i (weights: the split partials summed in split order; biases and the
+// output layer: the block sums), the mean loss, and with adam the update: Adam step
+// t = counters[1] (bias corrections 1 - b^t).
+__global__ void mlp_update_kernel(const UpdateArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float t = (float)(counters[1] + 1);
+    if (i >= kMlpParams) return;
+    const int g = param_region(i);
+    float gr = 0.f;
+    if (g < 8 && (g & 1) == 0) {
+        float v[kMlpMaxSplits];  // every load in flight at once; splits past the count read as 0
+#pragma unroll
+        for (int s = 0; s < kMlpMaxSplits; ++s) v[s] = s < a.splits ? a.partial[(size_t)s * kMlpParams + i] : 0.f;
+#pragma unroll
+        for (int s = 0; s < kMlpMaxSplits; ++s) gr += v[s];
+    } else {
+        const int j = i - mlp_offset(g);
+        gr = a.colsum[g < 8 ? (g >> 1) * kMlpHidden + j : (g == 8 ? 4 * kMlpHidden + j : 5 * kMlpHidden)];
+    }
+    a.grad[i] = gr;
+    if (i == 0) {
+        // with adam the iteration counter has already advanced (mlp_wgrad_kernel)
+        a.loss_hist[(a.counters[0] - (a.adam ? 1 : 0)) % kLossChunk] = a.colsum[5 * kMlpHidden + 1] / (float)a.rows;
+    }
+    if (!a.adam) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float t = (float)a.counters[1];
     const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
-    const float g = grad[i];
-    const float a = b1 * m1[i] + (1.f - b1) * g;
-    const float v = b2 * m2[i] + (1.f - b2) * g * g;
-    m1[i] = a;
-    m2[i] = v;
-    const float upd = lr * (a / c1) / (sqrtf(v / c2) + eps);
-    const float nw = w[i] - upd;
-    w[i] = nw;
-    if (i < n_bf) wbf[i] = f2bf(nw);
+    const float m = b1 * a.m1[i] + (1.f - b1) * gr;
+    const float v = b2 * a.m2[i] + (1.f - b2) * gr * gr;
+    a.m1[i] = m;
+    a.m2[i] = v;
+    const float upd = a.lr * (m / c1) / (sqrtf(v / c2) + eps);
+    const float nw = a.w[i] - upd;
+    a.w[i] = nw;
+    if (g < 8 && (g & 1) == 0) store_bf16_copies(a.wf, a.wb, g, i, f2bf(nw));
 }
 
-// end of a training iteration: next iteration, one more Adam step
-__global__ void step_kernel(uint64_t* counters) {
-    counters[0] += 1;
-    counters[1] += 1;
-}
-
-__global__ void to_bf16_kernel(const float* w, uint16_t* wbf, int n) {
+// bf16 fragment copies of every weight of W1..W4
+__global__ void weights_to_bf16_kernel(const float* w, uint16_t* wf, uint16_t* wb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) wbf[i] = f2bf(w[i]);
-}
-
-// batch indices: idx[r] = floor(u * n) from Philox(seed, stream) at call counters[0] * rows + r
-__global__ void sample_kernel(uint64_t seed, uint64_t stream, const uint64_t* counters, int rows, uint64_t n,
-                              uint32_t* idx) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    const double u = philox_uniform(seed, stream, counters[0] * (uint64_t)rows + r);
-    uint64_t v = (uint64_t)(u * (double)n);
-    idx[r] = (uint32_t)(v < n ? v : n - 1);
-}
-
-// Launches C = A B split into at most `splits` K ranges (multiples of kBK); returns the count used.
-int gemm(GemmArgs g, hipStream_t s, int splits = 1) {
-    const int mrows = g.ones_row >= 0 ? g.ones_row + 1 : g.m;
-    g.kchunk = ((g.k + splits - 1) / splits + kBK - 1) / kBK * kBK;
-    splits = (g.k + g.kchunk - 1) / g.kchunk;
-    dim3 grid((g.n + kBN - 1) / kBN, (mrows + kBM - 1) / kBM, splits);
-    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, s, g);
-    return splits;
+    if (i >= mlp_offset(8)) return;
+    const int g = param_region(i);
+    if ((g & 1) == 0) store_bf16_copies(wf, wb, g, i, f2bf(w[i]));
 }
 
 }  // namespace
@@ -362,18 +628,20 @@ std::vector<int8_t> mlp_frequencies() {
 }
 
 // parameter offsets (floats) in the packed vector
-const int kMlpOffsets[11] = {
-    0,                                      // W1 [64][256]
-    kMlpIn * kMlpHidden,                    // b1
-    kMlpIn * kMlpHidden + kMlpHidden,       // W2 [256][256]
-    kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden,  // b2
-    kMlpIn * kMlpHidden + 2 * kMlpHidden + kMlpHidden * kMlpHidden,                            // W3
-    kMlpIn * kMlpHidden + 2 * kMlpHidden + 2 * kMlpHidden * kMlpHidden,                        // b3
-    kMlpIn * kMlpHidden + 3 * kMlpHidden + 2 * kMlpHidden * kMlpHidden,                        // W4
-    kMlpIn * kMlpHidden + 3 * kMlpHidden + 3 * kMlpHidden * kMlpHidden,                        // b4
-    kMlpIn * kMlpHidden + 4 * kMlpHidden + 3 * kMlpHidden * kMlpHidden,                        // W5 [256]
-    kMlpIn * kMlpHidden + 5 * kMlpHidden + 3 * kMlpHidden * kMlpHidden,                        // b5
-    kMlpParams};
+const int kMlpOffsets[11] = {mlp_offset(0), mlp_offset(1), mlp_offset(2), mlp_offset(3), mlp_offset(4), mlp_offset(5),
+                             mlp_offset(6), mlp_offset(7), mlp_offset(8), mlp_offset(9), mlp_offset(10)};
+
+namespace {
+int blocks_of(int rows) { return (rows + kRB - 1) / kRB; }
+
+// splits of the weight-gradient sums over nb row blocks: kWgradChunk blocks each (one LDS stage),
+// at most kMlpMaxSplits
+void wgrad_split(int nb, int& splits, int& per_split) {
+    splits = std::min(kMlpMaxSplits, std::max(1, nb / kWgradChunk));
+    per_split = (nb + splits - 1) / splits;
+    splits = (nb + per_split - 1) / per_split;
+}
+}  // namespace
 
 Mlp::Mlp(Context* c, int max_rows) : ctx(c), cap(max_rows) {
     const auto kv = mlp_frequencies();
@@ -381,33 +649,30 @@ Mlp::Mlp(Context* c, int max_rows) : ctx(c), cap(max_rows) {
         ALEATOR_HIP_CHECK(hipMalloc(p, std::max<size_t>(n, 1) * sizeof(**p)));
         buffers.push_back(*p);
     };
+    const size_t nb = (size_t)blocks_of(cap);
     try {
         alloc(&w, kMlpParams);
         alloc(&grad, kMlpParams);
         alloc(&m1, kMlpParams);
         alloc(&m2, kMlpParams);
-        alloc(&wbf, kMlpOffsets[8]);
+        alloc(&wf, kMlpFwdFrag);
+        alloc(&wb, kMlpBwdFrag);
         alloc(&kvec, kv.size());
-        alloc(&x, (size_t)cap * kMlpIn);
-        for (int l = 0; l < 4; ++l) {
-            alloc(&z[l], (size_t)cap * kMlpHidden);
-            alloc(&h[l], (size_t)cap * kMlpHidden);
-            alloc(&dz[l], (size_t)cap * kMlpHidden);
-        }
-        alloc(&y, cap);
-        alloc(&dy, cap);
-        alloc(&loss_rows, cap);
-        alloc(&idx, cap);
+        alloc(&ht, nb * kHtPerBlock);
+        alloc(&dzt, nb * kDztPerBlock);
+        alloc(&blockpart, nb * kMlpBlockPart);
+        alloc(&colsum, kMlpBlockPart);
+        alloc(&partial, (size_t)kMlpMaxSplits * kMlpParams);
         alloc(&loss_hist, kLossChunk);
         alloc(&counters, 2);
-        alloc(&partial, (size_t)kMaxSplits * (kMlpHidden + 1) * kMlpHidden);
-        alloc(&head_partial, (size_t)((cap + kHeadChunk - 1) / kHeadChunk) * (kMlpHidden + 2));
         ALEATOR_HIP_CHECK(hipMemcpyAsync(kvec, kv.data(), kv.size(), hipMemcpyHostToDevice, ctx->stream));
         ALEATOR_HIP_CHECK(hipMemsetAsync(w, 0, kMlpParams * sizeof(float), ctx->stream));
         ALEATOR_HIP_CHECK(hipMemsetAsync(grad, 0, kMlpParams * sizeof(float), ctx->stream));
-        ALEATOR_HIP_CHECK(hipMemsetAsync(wbf, 0, kMlpOffsets[8] * sizeof(uint16_t), ctx->stream));
+        ALEATOR_HIP_CHECK(hipMemsetAsync(wf, 0, kMlpFwdFrag * sizeof(uint16_t), ctx->stream));
+        ALEATOR_HIP_CHECK(hipMemsetAsync(wb, 0, kMlpBwdFrag * sizeof(uint16_t), ctx->stream));
         ALEATOR_HIP_CHECK(hipMemsetAsync(m1, 0, kMlpParams * sizeof(float), ctx->stream));
         ALEATOR_HIP_CHECK(hipMemsetAsync(m2, 0, kMlpParams * sizeof(float), ctx->stream));
+        ALEATOR_HIP_CHECK(hipMemsetAsync(counters, 0, 2 * sizeof(uint64_t), ctx->stream));
         ALEATOR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     } catch (...) {
         for (void* p : buffers) (void)hipFree(p);
@@ -425,8 +690,8 @@ Mlp::~Mlp() {
 
 void Mlp::set_params(const float* host) {
     ALEATOR_HIP_CHECK(hipMemcpyAsync(w, host, kMlpParams * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(to_bf16_kernel, dim3((kMlpOffsets[8] + 255) / 256), dim3(256), 0, ctx->stream, w, wbf,
-                       kMlpOffsets[8]);
+    hipLaunchKernelGGL(weights_to_bf16_kernel, dim3((kMlpOffsets[8] + 255) / 256), dim3(256), 0, ctx->stream, w, wf,
+                       wb);
     ALEATOR_HIP_CHECK(hipMemsetAsync(m1, 0, kMlpParams * sizeof(float), ctx->stream));
     ALEATOR_HIP_CHECK(hipMemsetAsync(m2, 0, kMlpParams * sizeof(float), ctx->stream));
     ALEATOR_HIP_CHECK(hipMemsetAsync(counters, 0, 2 * sizeof(uint64_t), ctx->stream));
@@ -438,100 +703,70 @@ void Mlp::get(float* host, const float* src) const {
     ALEATOR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
-void Mlp::forward(const double* frac, const uint32_t* rows_idx, int rows) {
-    hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(features_kernel, dim3((rows + 7) / 8), dim3(32, 8), 0, s, frac, rows_idx, rows, kvec, x);
-    const uint16_t* in = x;
-    int kin = kMlpIn;
-    for (int l = 0; l < 4; ++l) {
-        GemmArgs g{};
-        g.a = in;
-        g.sam = kin;
-        g.sak = 1;
-        g.b = wbf + kMlpOffsets[2 * l];
-        g.sbk = kMlpHidden;
-        g.sbn = 1;
-        g.m = rows;
-        g.n = kMlpHidden;
-        g.k = kin;
-        g.epilogue = kBiasSilu;
-        g.c = z[l];
-        g.bias = w + kMlpOffsets[2 * l + 1];
-        g.h = h[l];
-        g.ones_row = -1;
-        gemm(g, s);
-        in = h[l];
-        kin = kMlpHidden;
-    }
+namespace {
+// row kernel, weight gradients and the update (with or without Adam) of one pass over `rows`
+void training_pass(Mlp& net, const double* frac, const double* target, uint64_t n_samples, int rows, int mode,
+                   float lr, uint64_t seed, uint64_t stream) {
+    hipStream_t s = net.ctx->stream;
+    const int nb = blocks_of(rows);
+    RowArgs ra{};
+    ra.frac = frac;
+    ra.target = target;
+    ra.n_samples = n_samples;
+    ra.seed = seed;
+    ra.stream = stream;
+    ra.rows = rows;
+    ra.mode = mode;
+    ra.kvec = net.kvec;
+    ra.wf = net.wf;
+    ra.wb = net.wb;
+    ra.w = net.w;
+    ra.counters = net.counters;
+    ra.ht = net.ht;
+    ra.dzt = net.dzt;
+    ra.blockpart = net.blockpart;
+    hipLaunchKernelGGL(mlp_rows_kernel<true>, dim3(nb), dim3(kRowThreads), 0, s, ra);
+    const bool train = mode == kTrain;
+    WgradArgs wa{};
+    wa.ht = net.ht;
+    wa.dzt = net.dzt;
+    wa.partial = net.partial;
+    wa.nb = nb;
+    int splits = 1;
+    wgrad_split(nb, splits, wa.per_split);
+    wa.splits = splits;
+    wa.blockpart = net.blockpart;
+    wa.colsum = net.colsum;
+    wa.counters = net.counters;
+    wa.bump = train ? 1 : 0;
+    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(kWgradTiles * ((splits + 7) & ~7) + kColsumBlocks), dim3(kWgradThreads), 0,
+                       s, wa);
+    UpdateArgs ua{};
+    ua.w = net.w;
+    ua.grad = net.grad;
+    ua.m1 = net.m1;
+    ua.m2 = net.m2;
+    ua.wf = net.wf;
+    ua.wb = net.wb;
+    ua.partial = net.partial;
+    ua.splits = splits;
+    ua.colsum = net.colsum;
+    ua.rows = rows;
+    ua.lr = lr;
+    ua.adam = train ? 1 : 0;
+    ua.counters = net.counters;
+    ua.loss_hist = net.loss_hist;
+    hipLaunchKernelGGL(mlp_update_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, ua);
 }
+}  // namespace
 
-void Mlp::backward(const double* target, const uint32_t* rows_idx, int rows) {
-    hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(head_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, h[3], z[3], w + kMlpOffsets[8],
-                       w + kMlpOffsets[9], target, rows_idx, rows, y, dy, dz[3], loss_rows, 1);
-    const int chunks = (rows + kHeadChunk - 1) / kHeadChunk;
-    hipLaunchKernelGGL(head_grad_partial_kernel, dim3(chunks), dim3(kMlpHidden), 0, s, h[3], dy, loss_rows, rows,
-                       head_partial);
-    hipLaunchKernelGGL(head_grad_reduce_kernel, dim3(kMlpHidden + 2), dim3(64), 0, s, head_partial, chunks, rows,
-                       grad + kMlpOffsets[8], grad + kMlpOffsets[9], loss_hist, counters);
-    const int splits = std::max(1, std::min(kMaxSplits, rows / 256));
-    for (int l = 3; l >= 0; --l) {
-        // [dW_l; db_l] = [H_{l-1}^T; 1^T] dZ_l   ([kin + 1 x rows] x [rows x 256]), split over rows;
-        // W_l and b_l are adjacent in the parameter vector, so the result lands on both
-        const uint16_t* hin = l == 0 ? x : h[l - 1];
-        const int kin = l == 0 ? kMlpIn : kMlpHidden;
-        GemmArgs gw{};
-        gw.a = hin;
-        gw.sam = 1;
-        gw.sak = kin;
-        gw.b = dz[l];
-        gw.sbk = kMlpHidden;
-        gw.sbn = 1;
-        gw.m = kin;
-        gw.n = kMlpHidden;
-        gw.k = rows;
-        gw.epilogue = kStoreF32;
-        gw.ones_row = kin;
-        const long long n_out = (long long)(kin + 1) * kMlpHidden;
-        if (splits == 1) {
-            gw.c = grad + kMlpOffsets[2 * l];
-            gemm(gw, s);
-        } else {
-            gw.c = partial;
-            const int used = gemm(gw, s, splits);
-            hipLaunchKernelGGL(split_reduce_kernel, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, s, partial,
-                               used, n_out, grad + kMlpOffsets[2 * l]);
-        }
-        if (l == 0) break;
-        // dZ_{l-1} = (dZ_l W_l^T) * silu'(Z_{l-1})   ([rows x 256] x [256 x 256])
-        GemmArgs gd{};
-        gd.a = dz[l];
-        gd.sam = kMlpHidden;
-        gd.sak = 1;
-        gd.b = wbf + kMlpOffsets[2 * l];
-        gd.sbk = 1;
-        gd.sbn = kMlpHidden;
-        gd.m = rows;
-        gd.n = kMlpHidden;
-        gd.k = kMlpHidden;
-        gd.epilogue = kSiluGrad;
-        gd.h = dz[l - 1];
-        gd.zref = z[l - 1];
-        gd.ones_row = -1;
-        gemm(gd, s);
-    }
+void Mlp::loss_grad(const double* frac, const double* target, int rows) {
+    training_pass(*this, frac, target, (uint64_t)rows, rows, kGrad, 0.0f, 0, 0);
 }
 
 void Mlp::iteration(const double* frac, const double* target, uint64_t n_samples, int batch, float lr,
                     uint64_t seed, uint64_t stream) {
-    hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(sample_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, seed, stream, counters, batch,
-                       n_samples, idx);
-    forward(frac, idx, batch);
-    backward(target, idx, batch);
-    hipLaunchKernelGGL(adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, w, grad, m1, m2, wbf, kMlpParams,
-                       kMlpOffsets[8], lr, 0.9f, 0.999f, 1e-8f, counters);
-    hipLaunchKernelGGL(step_kernel, dim3(1), dim3(1), 0, s, counters);
+    training_pass(*this, frac, target, n_samples, batch, kTrain, lr, seed, stream);
 }
 
 void Mlp::drop_graph() {
@@ -573,9 +808,17 @@ void Mlp::train(const double* frac, const double* target, uint64_t n_samples, in
 }
 
 void Mlp::predict(const double* frac, int rows, float* out) {
-    forward(frac, nullptr, rows);
-    hipLaunchKernelGGL(head_kernel, dim3((rows + 3) / 4), dim3(256), 0, ctx->stream, h[3], z[3], w + kMlpOffsets[8],
-                       w + kMlpOffsets[9], nullptr, nullptr, rows, out, nullptr, nullptr, nullptr, 0);
+    RowArgs ra{};
+    ra.frac = frac;
+    ra.rows = rows;
+    ra.mode = kPredict;
+    ra.kvec = kvec;
+    ra.wf = wf;
+    ra.wb = wb;
+    ra.w = w;
+    ra.counters = counters;
+    ra.y_out = out;
+    hipLaunchKernelGGL(mlp_rows_kernel<false>, dim3(blocks_of(rows)), dim3(kRowThreads), 0, ctx->stream, ra);
 }
 
 }  // namespace aleator_gpu
@@ -674,8 +917,7 @@ aleator_status aleator_gpu_mlp_loss_grad(aleator_mlp* m, const double* frac, con
         Mlp& net = m->net;
         require(n >= 1 && n <= net.cap, ALEATOR_INVALID_ARGUMENT, "mlp: rows must be in [1, max_rows]");
         ALEATOR_HIP_CHECK(hipMemsetAsync(net.counters, 0, sizeof(uint64_t), net.ctx->stream));  // loss slot 0
-        net.forward(frac, nullptr, n);
-        net.backward(target, nullptr, n);
+        net.loss_grad(frac, target, n);
         check_launch();
         float l = 0.0f;
         ALEATOR_HIP_CHECK(hipMemcpyAsync(&l, net.loss_hist, sizeof(float), hipMemcpyDeviceToHost, net.ctx->stream));

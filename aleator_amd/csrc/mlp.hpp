@@ -11,13 +11,29 @@ namespace aleator_gpu {
 constexpr int kMlpIn = 64;       // 32 frequency vectors x (sin, cos)
 constexpr int kMlpFreqs = 32;
 constexpr int kMlpHidden = 256;  // 4 hidden layers
-constexpr int kMaxSplits = 32;      // row splits of the weight-gradient GEMMs
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + 3 * (kMlpHidden * kMlpHidden + kMlpHidden) + kMlpHidden + 1;
+constexpr int kMlpBlockRows = 32;  // rows per workgroup of the row kernel (two 16-row MFMA tiles)
+constexpr int kMlpMaxSplits = 16;  // row-block ranges of the weight-gradient kernel
 constexpr int kLossChunk = 1 << 16;  // iterations whose losses are kept on the device at once
-constexpr int kMlpParams =kMlpIn * kMlpHidden + kMlpHidden + 3 * (kMlpHidden * kMlpHidden + kMlpHidden) + kMlpHidden + 1;
+// per row block: bias gradients of the 4 hidden layers, then dw5 (256), db5 and the summed
+// squared error
+constexpr int kMlpBlockPart = 4 * kMlpHidden + kMlpHidden + 2;
+// bf16 weights in MFMA fragment order: forward copies of W1..W4, backward copies of W2..W4
+constexpr int kMlpFwdFrag = kMlpIn * kMlpHidden + 3 * kMlpHidden * kMlpHidden;
+constexpr int kMlpBwdFrag = 3 * kMlpHidden * kMlpHidden;
 
-// offsets (floats) of W1 b1 W2 b2 W3 b3 W4 b4 w5 b5 in the packed parameter vector, then the total;
-// W_l is stored [fan_in][fan_out] row-major
-extern const int kMlpOffsets[11];
+// offset (floats) of part k = 0..9 of the packed parameter vector, W1 b1 W2 b2 W3 b3 W4 b4 w5 b5,
+// and the total for k = 10; W_l is stored [fan_in][fan_out] row-major
+__host__ __device__ constexpr int mlp_offset(int k) {
+    constexpr int first = kMlpIn * kMlpHidden + kMlpHidden;            // W1, b1
+    constexpr int layer = kMlpHidden * kMlpHidden + kMlpHidden;        // W_l, b_l (l = 2..4)
+    return k == 0 ? 0
+         : k == 1 ? kMlpIn * kMlpHidden
+         : k < 8  ? first + ((k - 2) >> 1) * layer + ((k - 2) & 1) * kMlpHidden * kMlpHidden
+         : first + 3 * layer + (k - 8) * kMlpHidden - (k == 10 ? kMlpHidden - 1 : 0);
+}
+static_assert(mlp_offset(10) == kMlpParams && mlp_offset(9) == kMlpParams - 1 && mlp_offset(7) == mlp_offset(6) + 65536);
+extern const int kMlpOffsets[11];  // mlp_offset(0 .. 10)
 
 std::vector<int8_t> mlp_frequencies();
 
@@ -29,19 +45,17 @@ struct Mlp {
     float* grad = nullptr;
     float* m1 = nullptr;
     float* m2 = nullptr;
-    uint16_t* wbf = nullptr; // bf16 copies of W1..W4 (and b1..b4, unused)
+    uint16_t* wf = nullptr;  // bf16 W1..W4 as B operands of H W (fragment order)
+    uint16_t* wb = nullptr;  // bf16 W2..W4 as B operands of dZ W^T (fragment order)
     int8_t* kvec = nullptr;
-    uint16_t* x = nullptr;   // features [cap][64]
-    float* z[4] = {};        // pre-activations [cap][256]
-    uint16_t* h[4] = {};     // silu(z) in bf16
-    uint16_t* dz[4] = {};    // dL/dz in bf16
-    float* y = nullptr;
-    float* dy = nullptr;
-    float* loss_rows = nullptr;
-    uint32_t* idx = nullptr;
+    // activations of the last pass, per 32-row block and column, the block's rows contiguous:
+    // X (64 columns), H1, H2, H3 (256 each)
+    uint16_t* ht = nullptr;
+    uint16_t* dzt = nullptr;     // dL/dz of the 4 hidden layers, same layout (256 columns each)
+    float* blockpart = nullptr;  // [block][kMlpBlockPart]
+    float* colsum = nullptr;     // [kMlpBlockPart]: blockpart summed over the blocks
+    float* partial = nullptr;    // weight-gradient partial sums [split][kMlpParams]
     float* loss_hist = nullptr;  // per-iteration loss of the current train call chunk
-    float* partial = nullptr;    // split-K partial weight gradients [kMaxSplits][257][256]
-    float* head_partial = nullptr;  // output-layer partial sums [row chunk][258]
     uint64_t* counters = nullptr;   // device: [0] iteration of the current train call, [1] Adam steps taken
     // a training iteration captured as a hipGraph (kernels only), replayed for every iteration
     // of train calls with the same arguments
@@ -62,11 +76,9 @@ struct Mlp {
     ~Mlp();
     void set_params(const float* host);
     void get(float* host, const float* src) const;
-    // forward of rows (gathered through rows_idx when not null) up to the last hidden layer
-    void forward(const double* frac, const uint32_t* rows_idx, int rows);
-    // head, loss and gradients of the last forward; the mean loss goes to
-    // loss_hist[counters[0] mod kLossChunk]
-    void backward(const double* target, const uint32_t* rows_idx, int rows);
+    // loss and gradient of rows 0..rows-1 (mean squared error against target); the mean loss goes
+    // to loss_hist[counters[0] mod kLossChunk]
+    void loss_grad(const double* frac, const double* target, int rows);
     // one training iteration: sample a minibatch, forward, backward, Adam, advance the counters
     void iteration(const double* frac, const double* target, uint64_t n_samples, int batch, float lr, uint64_t seed,
                    uint64_t stream);
