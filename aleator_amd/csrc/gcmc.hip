@@ -22,6 +22,7 @@
 #include "device.hpp"
 #include "gcmc.hpp"
 #include "internal.hpp"
+#include "pair.hpp"
 #include "recip.hpp"
 
 namespace aleator_gpu {
@@ -97,6 +98,27 @@ __device__ inline double lj_term(const GcmcArgs& a, const double* tab4, int stri
     else if (a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED)
         v = v - tab4[2 * stride + idx] - (sqrt(r2) - a.rc_lj) * tab4[3 * stride + idx];
     return v;
+}
+
+// erfc(alpha r)/r from r^2 as 1/r - erf(alpha r)/r: 1/r from the hardware reciprocal square root
+// and two Newton steps, erf(alpha r)/r by Horner's rule over the polynomial coefficients in LDS.
+// (The library erfc, inlined into the pair loops, needed more registers than the kernel has.)
+template <int DEG>
+__device__ inline double erf_horner(const double* poly, double x) {
+    double c[DEG + 1];
+#pragma unroll
+    for (int k = 0; k <= DEG; ++k) c[k] = poly[k];  // independent LDS reads, issued together
+    double p = c[DEG];
+#pragma unroll
+    for (int k = DEG - 1; k >= 0; --k) p = fma(p, x, c[k]);
+    return p;
+}
+__device__ inline double erfc_over_r(const GcmcArgs& a, const double* poly, double r2) {
+    const double x = fma(r2, a.erf_xs, -1.0);
+    const double p = a.erf_deg == 18 ? erf_horner<18>(poly, x)
+                   : a.erf_deg == 24 ? erf_horner<24>(poly, x)
+                                     : erf_horner<kErfPolyMax>(poly, x);
+    return rsq_refined<2>(r2) - p;
 }
 
 // Reduce four doubles over the workgroup of NW waves, in a fixed order; every thread receives
@@ -293,8 +315,11 @@ __device__ void trial_positions(const GcmcArgs& a, Draws& uni, int move, int pro
 // thread 0 applies the acceptance rule and publishes the decision.
 // CHARGED: the framework carries Ewald charges (CHARGED); the uncharged instances drop the S(k)
 // arrays and code.
+// The arguments are read through a pointer to a device copy: as a by-value kernel argument the
+// compiler kept its many loop-invariant fields in scalar registers and spilled them.
 template <int NT, bool CHARGED, bool CELLS>
-__global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
+__global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs* __restrict__ args) {
+    const GcmcArgs& a = *args;
     constexpr int NW = NT / 64;
     __shared__ double s_re[kMaxKHalf], s_im[kMaxKHalf];
     // k vectors of the half set, (nx + kmax) | (ny + kmax) << 8 | (nz + kmax) << 16, and weights
@@ -311,6 +336,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
     __shared__ uint64_t s_call;                   // Philox call index of the next step's first draw
     __shared__ int s_rbeg[2 * kMaxSites], s_rlen[2 * kMaxSites];  // framework list of each site row
     __shared__ double s_lj[4 * kMaxSites * kMaxTypes];             // site vs framework type LJ constants
+    __shared__ double s_erf[CHARGED ? kErfPolyMax + 1 : 1];        // real-space kernel polynomial
     __shared__ int s_cand[CELLS ? kMaxGuestCand : 1];  // guest cell filter: candidate sites, ascending
     __shared__ int s_rowcell[2 * kMaxSites][3];  // guest cell of each site row
     __shared__ int s_wtot[NW];
@@ -341,6 +367,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
             s_kw[t] = a.kw[t];
         }
     for (int t = tid; t < 4 * kMaxSites * kMaxTypes; t += NT) s_lj[t] = a.ljtab[t];
+    if (CHARGED && tid <= a.erf_deg) s_erf[tid] = a.erf_poly[tid];
     __syncthreads();
 
     // tracked by every thread
@@ -462,10 +489,7 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                 const double r2 = dist2(a, at.x - site4[4 * row], at.y - site4[4 * row + 1], at.z - site4[4 * row + 2]);
                 if (!a.grid && r2 <= a.rc2_lj)
                     e4[is_old ? 0 : 1] += lj_term(a, s_lj, kMaxSites * kMaxTypes, s * kMaxTypes + ty, r2);
-                if (CHARGED && r2 <= a.rc2_real) {
-                    const double r = sqrt(r2);
-                    e4[is_old ? 2 : 3] += a.charge[s] * at.w * (erfc(a.alpha * r) / r);
-                }
+                if (CHARGED && r2 <= a.rc2_real) e4[is_old ? 2 : 3] += a.charge[s] * at.w * erfc_over_r(a, s_erf, r2);
             }
         }
         // other guests: every (site row, candidate guest site) pair in one flattened loop over
@@ -529,10 +553,8 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs a) {
                 const double r2 =
                     dist2(a, gf[0] - site4[4 * row], gf[1] - site4[4 * row + 1], gf[2] - site4[4 * row + 2]);
                 if (r2 <= a.rc2_lj) e4[is_old ? 0 : 1] += lj_term(a, a.ljgg, kMaxSites * kMaxSites, idx, r2);
-                if (CHARGED && r2 <= a.rc2_real) {
-                    const double r = sqrt(r2);
-                    e4[is_old ? 2 : 3] += a.charge[s] * a.charge[u] * (erfc(a.alpha * r) / r);
-                }
+                if (CHARGED && r2 <= a.rc2_real)
+                    e4[is_old ? 2 : 3] += a.charge[s] * a.charge[u] * erfc_over_r(a, s_erf, r2);
             }
         }
         // framework LJ from the energy grid: one site row per thread
@@ -803,16 +825,16 @@ void build_framework_bins(const GcmcArgs& a, double reach, int32_t** start, doub
     ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
 }
 
-void launch_gcmc(const GcmcArgs& a, int n_chains, hipStream_t s) {
+void launch_gcmc(const GcmcArgs& a, const GcmcArgs* d_args, int n_chains, hipStream_t s) {
     if (a.nk_half > kMaxKHalf) fail(ALEATOR_NOT_IMPLEMENTED, "gcmc: more than 4096 reciprocal terms");
     // guest cells compile in only where used, so the default full scan keeps its registers
     const bool cells = a.gc[0] > 0;
 #define LAUNCH(NTH, CH)                                                                                     \
     do {                                                                                                    \
         if (cells)                                                                                          \
-            hipLaunchKernelGGL((gcmc_chain_kernel<NTH, CH, true>), dim3(n_chains), dim3(NTH), 0, s, a);  \
+            hipLaunchKernelGGL((gcmc_chain_kernel<NTH, CH, true>), dim3(n_chains), dim3(NTH), 0, s, d_args);  \
         else                                                                                                \
-            hipLaunchKernelGGL((gcmc_chain_kernel<NTH, CH, false>), dim3(n_chains), dim3(NTH), 0, s, a); \
+            hipLaunchKernelGGL((gcmc_chain_kernel<NTH, CH, false>), dim3(n_chains), dim3(NTH), 0, s, d_args); \
     } while (0)
     if (a.charged)
         LAUNCH(kGThreads, true);

@@ -46,6 +46,11 @@ struct GcmcArgs {
     // Ewald
     int charged;
     double alpha, rc2_real, kc, recip_scale, self_pref, fw_self;
+    // real-space pair kernel erfc(alpha r)/r = 1/r - erf(alpha r)/r, erf(alpha r)/r as a polynomial
+    // of degree erf_deg (18, 24 or 28) in x = r^2 erf_xs - 1 over r in [0, rc_real]
+    // (erf_kernel_poly, cut at 1e-17)
+    double erf_poly[kErfPolyMax + 1], erf_xs;
+    int erf_deg;
     const int4* kn;
     const double* kw;
     const double* fw_sre;
@@ -94,6 +99,8 @@ struct GcmcArgs {
 void build_framework_bins(const GcmcArgs& a, double reach, int32_t** start, double4** pos, int32_t** type,
                           hipStream_t s);
 
-void launch_gcmc(const GcmcArgs& a, int n_chains, hipStream_t s);
+// Launches the chains with `a`, which must already be in the device copy d_args (the kernel reads
+// its arguments from there).
+void launch_gcmc(const GcmcArgs& a, const GcmcArgs* d_args, int n_chains, hipStream_t s);
 
 }  // namespace aleator_gpu

@@ -22,6 +22,7 @@ struct aleator_gcmc {
     uint32_t site_species[aleator_gpu::kMaxSites] = {};  // LJ species of each site
     aleator_gpu::GcmcArgs args{};
     aleator_gpu::ChainParams* d_params = nullptr;
+    aleator_gpu::GcmcArgs* d_args = nullptr;  // device copy of the launch arguments
     aleator_gpu::ChainState* d_state = nullptr;
     double* d_pos = nullptr;
     double* d_sre = nullptr;
@@ -215,6 +216,8 @@ aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_m
         a.charged = fw->charged ? 1 : 0;
         a.alpha = fw->alpha;
         a.rc2_real = fw->rc_real * fw->rc_real;
+        if (fw->charged)
+            a.erf_deg = erf_kernel_poly(fw->alpha, fw->rc_real, 1e-17, 0, a.erf_poly, &a.erf_xs);
         a.kc = kCoulomb;
         a.recip_scale = 2.0 * kCoulomb * (2.0 * kPi / std::abs(fw->lat.volume));
         a.self_pref = -kCoulomb * (fw->alpha / std::sqrt(kPi));
@@ -259,6 +262,7 @@ aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_m
             upload(&g->d_ljtab, tab.data(), tab.size());
             upload(&g->d_ljgg, gg.data(), gg.size());
             upload(&g->d_params, params.data(), params.size());
+            ALEATOR_HIP_CHECK(hipMalloc(&g->d_args, sizeof(GcmcArgs)));
             std::vector<ChainState> st(n_chains);
             std::memset(st.data(), 0, st.size() * sizeof(ChainState));
             for (auto& c : st) c.self_energy = fw->charged ? fw->self_energy : 0.0;
@@ -299,9 +303,9 @@ aleator_status aleator_gpu_gcmc_create(aleator_framework* fw_in, const aleator_m
                 build_framework_bins(a, reach, &g->d_bin_start, &g->d_bin_pos, &g->d_bin_type, s);
             }
         } catch (...) {
-            for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_state,
-                            (void*)g->d_pos, (void*)g->d_sre, (void*)g->d_sim, (void*)g->d_bin_start,
-                            (void*)g->d_bin_pos, (void*)g->d_bin_type})
+            for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_args,
+                            (void*)g->d_state, (void*)g->d_pos, (void*)g->d_sre, (void*)g->d_sim,
+                            (void*)g->d_bin_start, (void*)g->d_bin_pos, (void*)g->d_bin_type})
                 if (p) (void)hipFree(p);
             throw;
         }
@@ -332,6 +336,9 @@ aleator_status aleator_gpu_gcmc_run(aleator_gcmc* g, uint64_t n_steps, int32_t* 
         if (trace_device)
             ALEATOR_HIP_CHECK(hipMemsetAsync(trace_device, 0xff, (size_t)g->n_chains * n_steps * sizeof(int32_t),
                                              ctx.stream));
+        // the kernel reads its arguments from the device copy, refreshed before every launch (a
+        // pageable copy: `a` is read before the call returns), never from inside a captured graph
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(g->d_args, &a, sizeof(a), hipMemcpyHostToDevice, ctx.stream));
         if (g->use_graph && !trace_device) {
             if (!g->graph_exec || std::memcmp(&g->graph_args, &a, sizeof(a)) != 0) {
                 if (g->graph_exec) ALEATOR_HIP_CHECK(hipGraphExecDestroy(g->graph_exec));
@@ -340,7 +347,7 @@ aleator_status aleator_gpu_gcmc_run(aleator_gcmc* g, uint64_t n_steps, int32_t* 
                 g->graph = nullptr;
                 ALEATOR_HIP_CHECK(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
                 try {
-                    launch_gcmc(a, g->n_chains, ctx.stream);
+                    launch_gcmc(a, g->d_args, g->n_chains, ctx.stream);
                 } catch (...) {
                     hipGraph_t dead = nullptr;
                     (void)hipStreamEndCapture(ctx.stream, &dead);
@@ -357,7 +364,7 @@ aleator_status aleator_gpu_gcmc_run(aleator_gcmc* g, uint64_t n_steps, int32_t* 
             return;
         }
         ctx.time_begin();
-        launch_gcmc(a, g->n_chains, ctx.stream);
+        launch_gcmc(a, g->d_args, g->n_chains, ctx.stream);
         ctx.time_end();
     });
 }
@@ -534,8 +541,8 @@ aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* g) {
         (void)hipStreamSynchronize(g->fw->ctx->stream);
         if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
         if (g->graph) (void)hipGraphDestroy(g->graph);
-        for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_state, (void*)g->d_pos,
-                        (void*)g->d_sre, (void*)g->d_sim, (void*)g->d_prop, (void*)g->d_bin_start,
+        for (void* p : {(void*)g->d_ljtab, (void*)g->d_ljgg, (void*)g->d_params, (void*)g->d_args, (void*)g->d_state,
+                        (void*)g->d_pos, (void*)g->d_sre, (void*)g->d_sim, (void*)g->d_prop, (void*)g->d_bin_start,
                         (void*)g->d_bin_pos, (void*)g->d_bin_type, (void*)g->d_grid})
             if (p) (void)hipFree(p);
         Framework* fw = g->fw;
