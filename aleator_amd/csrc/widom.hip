@@ -245,6 +245,8 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
             a.box[i] = fw.lat.m[i][i];
             a.inv_box[i] = 1.0 / fw.lat.m[i][i];
         }
+    a.geo = a.box[0] <= 0.0 ? kGeoTriclinic
+          : (a.box[0] == a.box[1] && a.box[1] == a.box[2] ? kGeoCubic : kGeoOrtho);
     a.fx = fw.d_fx;
     a.fy = fw.d_fy;
     a.fz = fw.d_fz;
@@ -332,10 +334,12 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
     const unsigned pair_blocks = (unsigned)((req.n_trials + kWThreads - 1) / kWThreads);
     // split kernel: the fewest threads per trial (1 to 16) that give at least 2048 workgroups
     // (8 per CU), so that small batches still fill the chip
+    const uint64_t split_tpt = (uint64_t)split_trials_per_thread(ns);
     int split = 1;
-    while (split < 16 && (req.n_trials * split + kWThreads - 1) / kWThreads < 2048) split *= 2;
+    while (split < 16 && (req.n_trials * split + kWThreads * split_tpt - 1) / (kWThreads * split_tpt) < 2048) split *= 2;
     a.split = split;
-    const unsigned split_blocks = (unsigned)((req.n_trials * split + kWThreads - 1) / kWThreads);
+    const unsigned split_blocks =
+        (unsigned)((req.n_trials * split + kWThreads * split_tpt - 1) / (kWThreads * split_tpt));
     const int rec_rt = charged ? runs_molecules_per_block(ns, fw.kmax) : 1;
     if (rec_rt == 0) fail(ALEATOR_NOT_IMPLEMENTED, "widom: phase tables exceed the LDS budget");
     const unsigned rec_blocks = (unsigned)((req.n_trials + rec_rt - 1) / rec_rt);
@@ -369,10 +373,13 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
     // erf kernel polynomial of the real-space term (as in the grid build: FP64 to the last bits of
     // a term, MIXED to 1e-12 of it)
     const int deg = charged ? erf_kernel_poly(fw.alpha, fw.rc_real, fp64 ? 1e-17 : 1e-12, 0, a.poly, &a.poly_xs) : 0;
-    const bool lfs = a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED;
+    const int ljk = a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED ? 2 : (a.trunc == ALEATOR_LJ_SHIFTED ? 1 : 0);
     auto direct = [&](const CellArgs* c, unsigned blocks) {
-        if (fp64) widom_direct_fp64(a, c, n_cells, wide_cells, blocks, deg, lfs, ctx.stream);
-        else widom_direct_mixed(a, c, n_cells, wide_cells, blocks, deg, lfs, ctx.stream);
+        using Direct = void (*)(const WidomArgs&, const CellArgs*, unsigned, bool, unsigned, int, hipStream_t);
+        static constexpr Direct kDirect[2][3] = {
+            {widom_direct_mixed_lj0, widom_direct_mixed_lj1, widom_direct_mixed_lj2},
+            {widom_direct_fp64_lj0, widom_direct_fp64_lj1, widom_direct_fp64_lj2}};
+        kDirect[fp64 ? 1 : 0][ljk](a, c, n_cells, wide_cells, blocks, deg, ctx.stream);
     };
     ctx.time_begin();
     if (n_cells) {

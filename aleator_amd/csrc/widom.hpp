@@ -64,7 +64,13 @@ struct WidomArgs {
     double poly_xs;
     double tail_delta;  // analytic LJ tail of one insertion (0 unless the framework asks for it)
     double box[3], inv_box[3];  // orthorhombic cells: edge lengths and their inverses
+    int geo;  // split kernel: 0 triclinic, 1 orthorhombic, 2 cubic (kGeo*)
 };
+
+// Cell geometry of the split kernel's minimum image: triclinic cells fold fractional differences
+// and map them through the lattice; orthorhombic cells take d - L rint(d / L) per Cartesian axis;
+// cubic cells fold fractional differences and scale |f|^2 by L^2 (two instructions fewer).
+constexpr int kGeoTriclinic = 0, kGeoOrtho = 1, kGeoCubic = 2;
 
 // Reference point and rotated sites of trial t, Cartesian, in the reference's
 // arithmetic order (no contraction), and their fractional coordinates.
@@ -148,16 +154,19 @@ __device__ inline void stage_constants(const WidomArgs& a, double4 (&tab)[NS][kM
 }
 
 // One (site, atom image) pair at squared distance r2: LJ into lj, q_site q_atom erfc(alpha r)/r
-// into real.  ONE: the LJ and real-space cutoffs are equal (or the molecule is uncharged).
-template <int NR, int DEG, bool LFS, bool ONE>
+// into real.  ONE: the LJ and real-space cutoffs are equal (or the molecule is uncharged).  LJK:
+// LJ truncation kind (0 truncated, 1 shifted, 2 linear-force-shifted); V = r^-6 (A r^-6 - B)
+// [- shift [- slope r]] from cg = (A, B, shift, slope).
+template <int NR, int DEG, int LJK, bool ONE>
 __device__ inline void widom_pair(double r2, const double4& cg, double qq, double rc2, double rcr2,
                                   const double (&pc)[DEG + 1], double pxs, double& lj, double& real) {
     constexpr bool CH = DEG > 0;
+    constexpr bool LFS = LJK == 2;
     auto lj_term = [&](double ir2, double r) {
         const double ir6 = ir2 * ir2 * ir2;
-        double w = fma(cg.y, ir6, cg.z);
-        if constexpr (LFS) w = fma(r, cg.w, w);
-        lj += fma(cg.x, ir6 * ir6, -w);
+        lj = fma(ir6, fma(cg.x, ir6, -cg.y), lj);
+        if constexpr (LJK >= 1) lj -= cg.z;
+        if constexpr (LFS) lj = fma(-r, cg.w, lj);
     };
     if constexpr (ONE) {
         if (r2 <= rc2) {
@@ -177,23 +186,30 @@ __device__ inline void widom_pair(double r2, const double4& cg, double qq, doubl
     }
 }
 
+// Trials per thread of the split kernel: each staged atom read from LDS is paired with this many
+// trials held in registers.  One: at 1e5 CH4 trials in c0, 2 and 4 per thread measured 5 % and
+// 9 % slower (the fewer, longer workgroups leave a larger partly filled last round on the chip).
+constexpr int split_trials_per_thread(int) { return 1; }
+
 // Direct sum over every framework atom (small batches, or no usable cube grid).  The 256 threads
-// of a workgroup take 256/split trials; the split threads of a trial share its atoms (atom j of
-// a staged chunk to thread j mod split) and their partial sums meet in LDS in a fixed order, so
-// a batch of 1e5 trials still fills the chip.  Atoms are staged through LDS in chunks of 256 as
-// fractional coordinates; each pair folds the fractional difference to the nearest image (exact
-// for pairs within a cutoff <= L_perp/2).
-// ORTHO: orthorhombic cell; atoms and sites stay Cartesian and the nearest image is taken per
-// axis, d - L rint(d / L) (exact below L_perp/2, as the fractional fold).
-template <bool EXACT, int NS, int DEG, bool LFS, bool ORTHO>
+// of a workgroup form 256/split groups of `split` threads, each group taking TPT trials; the split
+// threads of a group share the atoms (atom j of a staged chunk to thread j mod split) and their
+// partial sums meet in LDS in a fixed order, so a batch of 1e5 trials still fills the chip.  Atoms
+// are staged through LDS in chunks of 256, with each site's LJ constants for the atom's type
+// beside them (no per-pair type lookup); each atom read from LDS is paired with the thread's TPT
+// trials.  Each pair takes the nearest image as a.geo says (exact for pairs within a cutoff <=
+// L_perp/2).
+template <bool EXACT, int NS, int DEG, int LJK>
 __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs a) {
     constexpr int NR = EXACT ? 2 : 1;
     constexpr bool CH = DEG > 0;
+    constexpr int TPT = split_trials_per_thread(NS);
     __shared__ double sfx[kWThreads], sfy[kWThreads], sfz[kWThreads], sq[kWThreads];
-    __shared__ int st[kWThreads];
+    __shared__ double2 sab[NS][kWThreads];                    // A, B of (site, atom)
+    __shared__ double2 scd[LJK > 0 ? NS : 1][kWThreads];      // shift, slope of (site, atom)
     __shared__ double4 tab[NS][kMaxTypes];
     __shared__ double s_poly[DEG + 1];
-    __shared__ double part_lj[kWThreads], part_re[kWThreads];
+    __shared__ double part_lj[TPT][kWThreads], part_re[CH ? TPT : 1][kWThreads];
     __shared__ double red[kWThreads / 64];
     stage_constants<NS, DEG>(a, tab, s_poly);
     __syncthreads();
@@ -203,19 +219,20 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
     const double pxs = a.poly_xs;
     const int split = a.split, tpb = kWThreads / split;
     const int j = threadIdx.x % tpb, part = threadIdx.x / tpb;
-    const uint64_t t = (uint64_t)blockIdx.x * tpb + j;
-    const bool valid = t < a.n_trials;
-    // the trial's sites (fractional, or Cartesian with ORTHO): drawn by the first of its split
-    // threads and shared through LDS
-    __shared__ double s_fs[kWThreads / 2][NS][3];
-    double fs[NS][3];
+    // trial of slot k of this group
+    auto trial = [&](int k) { return ((uint64_t)blockIdx.x * TPT + k) * tpb + j; };
+    const int geo = a.geo;
+    // the group's trial sites (fractional, or Cartesian for orthorhombic cells): drawn by its
+    // first thread and shared through LDS
+    __shared__ double s_fs[kWThreads / 2][TPT][NS][3];
+    double fs[TPT][NS][3];
     double qs[NS];
-    auto draw = [&](double (&out)[NS][3]) {
+    auto draw = [&](int k, double (&out)[NS][3]) {
         double cart[kMaxSites][3];
-        trial_sites(a, t, cart);
+        trial_sites(a, trial(k), cart);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            if constexpr (ORTHO) {
+            if (geo == kGeoOrtho) {
                 out[s][0] = cart[s][0];
                 out[s][1] = cart[s][1];
                 out[s][2] = cart[s][2];
@@ -225,85 +242,130 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
         }
     };
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        fs[s][0] = fs[s][1] = fs[s][2] = 0.0;
-        qs[s] = a.charge[s];
-    }
-    if (split == 1) {
-        if (valid) draw(fs);
-    } else {
-        if (valid && part == 0) draw(s_fs[j]);
-        __syncthreads();
-        if (valid)
+    for (int s = 0; s < NS; ++s) qs[s] = a.charge[s];
 #pragma unroll
-            for (int s = 0; s < NS; ++s)
-                for (int k = 0; k < 3; ++k) fs[s][k] = s_fs[j][s][k];
+    for (int k = 0; k < TPT; ++k)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) fs[k][s][0] = fs[k][s][1] = fs[k][s][2] = 0.0;
+    const bool any_valid = trial(0) < a.n_trials;
+    if (split == 1) {
+#pragma unroll
+        for (int k = 0; k < TPT; ++k)
+            if (trial(k) < a.n_trials) draw(k, fs[k]);
+    } else {
+        if (part == 0)
+#pragma unroll
+            for (int k = 0; k < TPT; ++k)
+                if (trial(k) < a.n_trials) draw(k, s_fs[j][k]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TPT; ++k)
+            if (trial(k) < a.n_trials)
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    for (int c = 0; c < 3; ++c) fs[k][s][c] = s_fs[j][k][s][c];
     }
-    double lj = 0.0, real = 0.0;
+    double lj[TPT], real[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) lj[k] = real[k] = 0.0;
     const double rc2 = a.rc2_lj, rcr2 = a.rc2_real;
     const bool one = !CH || rc2 == rcr2;
+    const double l2 = a.box[0] * a.box[0];  // cubic cells
+    // the pairs of staged atoms part, part + split, ... < cnt with the TPT trials, minimum image
+    // by geometry G
+    auto pairs = [&](auto geo_tag, auto one_tag, int cnt) {
+        constexpr int G = decltype(geo_tag)::value;
+        constexpr bool ONE = decltype(one_tag)::value;
+        for (int e = part; e < cnt; e += split) {
+            const double ax = sfx[e], ay = sfy[e], az = sfz[e], qa = sq[e];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const double2 ab = sab[s][e];
+                double2 cd = make_double2(0.0, 0.0);
+                if constexpr (LJK > 0) cd = scd[s][e];
+                const double4 cg = make_double4(ab.x, ab.y, cd.x, cd.y);
+                const double qq = qs[s] * qa;
+#pragma unroll
+                for (int k = 0; k < TPT; ++k) {
+                    double r2;
+                    double d0 = ax - fs[k][s][0], d1 = ay - fs[k][s][1], d2 = az - fs[k][s][2];
+                    if constexpr (G == kGeoOrtho) {
+                        d0 = fma(-a.box[0], rint(d0 * a.inv_box[0]), d0);
+                        d1 = fma(-a.box[1], rint(d1 * a.inv_box[1]), d1);
+                        d2 = fma(-a.box[2], rint(d2 * a.inv_box[2]), d2);
+                        r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
+                    } else {
+                        d0 -= rint(d0);
+                        d1 -= rint(d1);
+                        d2 -= rint(d2);
+                        if constexpr (G == kGeoCubic) {
+                            r2 = l2 * fma(d2, d2, fma(d1, d1, d0 * d0));
+                        } else {
+                            const double dx = d0 * a.m[0] + d1 * a.m[3] + d2 * a.m[6];
+                            const double dy = d0 * a.m[1] + d1 * a.m[4] + d2 * a.m[7];
+                            const double dz = d0 * a.m[2] + d1 * a.m[5] + d2 * a.m[8];
+                            r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                        }
+                    }
+                    widom_pair<NR, DEG, LJK, ONE>(r2, cg, qq, rc2, rcr2, pc, pxs, lj[k], real[k]);
+                }
+            }
+        }
+    };
     for (int base = 0; base < a.natoms; base += kWThreads) {
         __syncthreads();
         const int at = base + threadIdx.x;
         if (at < a.natoms) {
-            if constexpr (ORTHO) {
-                sfx[threadIdx.x] = a.fx[at] * a.box[0];
-                sfy[threadIdx.x] = a.fy[at] * a.box[1];
-                sfz[threadIdx.x] = a.fz[at] * a.box[2];
-            } else {
-                sfx[threadIdx.x] = a.fx[at];
-                sfy[threadIdx.x] = a.fy[at];
-                sfz[threadIdx.x] = a.fz[at];
-            }
+            const double sc0 = geo == kGeoOrtho ? a.box[0] : 1.0;
+            const double sc1 = geo == kGeoOrtho ? a.box[1] : 1.0;
+            const double sc2 = geo == kGeoOrtho ? a.box[2] : 1.0;
+            sfx[threadIdx.x] = a.fx[at] * sc0;
+            sfy[threadIdx.x] = a.fy[at] * sc1;
+            sfz[threadIdx.x] = a.fz[at] * sc2;
             sq[threadIdx.x] = a.q[at];
-            st[threadIdx.x] = a.type[at];
+            const int ty = a.type[at];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const double4 c = tab[s][ty];
+                sab[s][threadIdx.x] = make_double2(c.x, c.y);
+                if constexpr (LJK > 0) scd[s][threadIdx.x] = make_double2(c.z, c.w);
+            }
         }
         __syncthreads();
         const int cnt = min(kWThreads, a.natoms - base);
-        if (!valid) continue;
-        for (int e = part; e < cnt; e += split) {
-            const double ax = sfx[e], ay = sfy[e], az = sfz[e], qa = sq[e];
-            const int ty = st[e];
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                double dx, dy, dz;
-                if constexpr (ORTHO) {
-                    dx = ax - fs[s][0];
-                    dy = ay - fs[s][1];
-                    dz = az - fs[s][2];
-                    dx = fma(-a.box[0], rint(dx * a.inv_box[0]), dx);
-                    dy = fma(-a.box[1], rint(dy * a.inv_box[1]), dy);
-                    dz = fma(-a.box[2], rint(dz * a.inv_box[2]), dz);
-                } else {
-                    double d0 = ax - fs[s][0], d1 = ay - fs[s][1], d2 = az - fs[s][2];
-                    d0 -= rint(d0);
-                    d1 -= rint(d1);
-                    d2 -= rint(d2);
-                    dx = d0 * a.m[0] + d1 * a.m[3] + d2 * a.m[6];
-                    dy = d0 * a.m[1] + d1 * a.m[4] + d2 * a.m[7];
-                    dz = d0 * a.m[2] + d1 * a.m[5] + d2 * a.m[8];
-                }
-                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                if (one)
-                    widom_pair<NR, DEG, LFS, true>(r2, tab[s][ty], qs[s] * qa, rc2, rcr2, pc, pxs, lj, real);
-                else
-                    widom_pair<NR, DEG, LFS, false>(r2, tab[s][ty], qs[s] * qa, rc2, rcr2, pc, pxs, lj, real);
-            }
+        if (!any_valid) continue;
+        if (geo == kGeoCubic) {
+            if (one) pairs(std::integral_constant<int, kGeoCubic>{}, std::true_type{}, cnt);
+            else pairs(std::integral_constant<int, kGeoCubic>{}, std::false_type{}, cnt);
+        } else if (geo == kGeoOrtho) {
+            if (one) pairs(std::integral_constant<int, kGeoOrtho>{}, std::true_type{}, cnt);
+            else pairs(std::integral_constant<int, kGeoOrtho>{}, std::false_type{}, cnt);
+        } else {
+            if (one) pairs(std::integral_constant<int, kGeoTriclinic>{}, std::true_type{}, cnt);
+            else pairs(std::integral_constant<int, kGeoTriclinic>{}, std::false_type{}, cnt);
         }
     }
-    part_lj[threadIdx.x] = lj;
-    part_re[threadIdx.x] = real;
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        part_lj[k][threadIdx.x] = lj[k];
+        if constexpr (CH) part_re[k][threadIdx.x] = real[k];
+    }
     __syncthreads();
     double boltz = 0.0;
-    if (part == 0 && valid) {
-        double sl = 0.0, sr = 0.0;
-        for (int p = 0; p < split; ++p) {
-            sl += part_lj[j + p * tpb];
-            sr += part_re[j + p * tpb];
+    if (part == 0) {
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            const uint64_t t = trial(k);
+            if (t >= a.n_trials) continue;
+            double sl = 0.0, sr = 0.0;
+            for (int p = 0; p < split; ++p) {
+                sl += part_lj[k][j + p * tpb];
+                if constexpr (CH) sr += part_re[k][j + p * tpb];
+            }
+            const double du = (CH ? sl + a.kc * sr : sl) + a.tail_delta;
+            a.du[t] = du;
+            if (!CH) boltz += exp(-du / a.temperature);
         }
-        const double du = (CH ? sl + a.kc * sr : sl) + a.tail_delta;
-        a.du[t] = du;
-        if (!CH) boltz = exp(-du / a.temperature);
     }
     if constexpr (!CH) {
         const double s = block_sum(boltz, red);
@@ -319,7 +381,7 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
 // sites against it.  Every image within the cutoff of a site lies in the list, and at most one
 // image of an atom does (cutoff <= L_perp/2), so the sum is the reference's minimum-image sum,
 // taken in atom order per trial.
-template <bool EXACT, int NS, int DEG, bool LFS, int TPT>
+template <bool EXACT, int NS, int DEG, int LJK, int TPT>
 __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a, const CellArgs c) {
     constexpr int NR = EXACT ? 2 : 1;
     constexpr bool CH = DEG > 0;
@@ -393,7 +455,7 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
                             for (int k = 0; k < TPT; ++k) {
                                 const double dx = xy.x - ox[k][s], dy = xy.y - oy[k][s], dz = zq.x - oz[k][s];
                                 const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                                widom_pair<NR, DEG, LFS, ONE>(r2, cg, qq, rc2, rcr2, pc, pxs, lj[k], real[k]);
+                                widom_pair<NR, DEG, LJK, ONE>(r2, cg, qq, rc2, rcr2, pc, pxs, lj[k], real[k]);
                             }
                         }
                     }
@@ -472,12 +534,12 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
 // Trials per thread of the cube kernel: 1, or (for large batches) 4 for one site and 2 otherwise.
 inline int max_trials_per_thread(int ns) { return ns == 1 ? 4 : 2; }
 
-template <bool EXACT, int DEG, bool LFS>
+template <bool EXACT, int DEG, int LJK>
 void launch_cells_deg(const WidomArgs& a, const CellArgs& c, unsigned n_cells, bool wide, hipStream_t s) {
     switch (a.nsites * 2 + (wide ? 1 : 0)) {
 #define ALEATOR_WCELL(N, T)                                                                                     \
     case 2 * N + (T > 1 ? 1 : 0):                                                                               \
-        hipLaunchKernelGGL((widom_cell_kernel<EXACT, N, DEG, LFS, T>), dim3(n_cells), dim3(kWThreads), 0, s, a, c); \
+        hipLaunchKernelGGL((widom_cell_kernel<EXACT, N, DEG, LJK, T>), dim3(n_cells), dim3(kWThreads), 0, s, a, c); \
         return;
         ALEATOR_WCELL(1, 1) ALEATOR_WCELL(1, 4) ALEATOR_WCELL(2, 1) ALEATOR_WCELL(2, 2) ALEATOR_WCELL(3, 1)
         ALEATOR_WCELL(3, 2) ALEATOR_WCELL(4, 1) ALEATOR_WCELL(4, 2) ALEATOR_WCELL(5, 1) ALEATOR_WCELL(5, 2)
@@ -486,17 +548,12 @@ void launch_cells_deg(const WidomArgs& a, const CellArgs& c, unsigned n_cells, b
     fail(ALEATOR_INVALID_ARGUMENT, "widom: 1 to 5 sites supported");
 }
 
-template <bool EXACT, int DEG, bool LFS>
-void launch_split_deg(const WidomArgs& a, unsigned blocks, bool ortho, hipStream_t s) {
+template <bool EXACT, int DEG, int LJK>
+void launch_split_deg(const WidomArgs& a, unsigned blocks, hipStream_t s) {
     switch (a.nsites) {
 #define ALEATOR_WSPLIT(N)                                                                                         \
     case N:                                                                                                       \
-        if (ortho)                                                                                                \
-            hipLaunchKernelGGL((widom_split_kernel<EXACT, N, DEG, LFS, true>), dim3(blocks), dim3(kWThreads), 0, s, \
-                               a);                                                                                \
-        else                                                                                                      \
-            hipLaunchKernelGGL((widom_split_kernel<EXACT, N, DEG, LFS, false>), dim3(blocks), dim3(kWThreads), 0, \
-                               s, a);                                                                             \
+        hipLaunchKernelGGL((widom_split_kernel<EXACT, N, DEG, LJK>), dim3(blocks), dim3(kWThreads), 0, s, a);    \
         return;
         ALEATOR_WSPLIT(1) ALEATOR_WSPLIT(2) ALEATOR_WSPLIT(3) ALEATOR_WSPLIT(4) ALEATOR_WSPLIT(5)
 #undef ALEATOR_WSPLIT
@@ -505,13 +562,13 @@ void launch_split_deg(const WidomArgs& a, unsigned blocks, bool ortho, hipStream
 }
 
 // deg: erf polynomial degree (0 when the molecule sees no charges)
-template <bool EXACT, bool LFS>
+template <bool EXACT, int LJK>
 void launch_direct(const WidomArgs& a, const CellArgs* c, unsigned n_cells, bool wide, unsigned blocks, int deg,
                    hipStream_t s) {
     auto go = [&](auto deg_tag) {
         constexpr int D = decltype(deg_tag)::value;
-        if (c) launch_cells_deg<EXACT, D, LFS>(a, *c, n_cells, wide, s);
-        else launch_split_deg<EXACT, D, LFS>(a, blocks, a.box[0] > 0.0, s);
+        if (c) launch_cells_deg<EXACT, D, LJK>(a, *c, n_cells, wide, s);
+        else launch_split_deg<EXACT, D, LJK>(a, blocks, s);
     };
     switch (deg) {
         case 0: return go(std::integral_constant<int, 0>{});
@@ -524,11 +581,17 @@ void launch_direct(const WidomArgs& a, const CellArgs* c, unsigned n_cells, bool
 
 }  // namespace widom_detail
 
-// The direct-sum kernels of each precision (widom_fp64.hip, widom_mixed.hip): the cube kernel
-// when c is given, else the split kernel over `blocks` workgroups.
-void widom_direct_fp64(const widom_detail::WidomArgs& a, const widom_detail::CellArgs* c, unsigned n_cells,
-                       bool wide, unsigned blocks, int deg, bool lfs, hipStream_t s);
-void widom_direct_mixed(const widom_detail::WidomArgs& a, const widom_detail::CellArgs* c, unsigned n_cells,
-                        bool wide, unsigned blocks, int deg, bool lfs, hipStream_t s);
+// The direct-sum kernels of each precision and LJ truncation kind ljk (widom_{fp64,mixed}_lj{0,1,2}
+// .hip): the cube kernel when c is given, else the split kernel over `blocks` workgroups.
+#define ALEATOR_WIDOM_DIRECT(NAME)                                                                       \
+    void NAME(const widom_detail::WidomArgs& a, const widom_detail::CellArgs* c, unsigned n_cells, bool wide, \
+              unsigned blocks, int deg, hipStream_t s);
+ALEATOR_WIDOM_DIRECT(widom_direct_fp64_lj0)
+ALEATOR_WIDOM_DIRECT(widom_direct_fp64_lj1)
+ALEATOR_WIDOM_DIRECT(widom_direct_fp64_lj2)
+ALEATOR_WIDOM_DIRECT(widom_direct_mixed_lj0)
+ALEATOR_WIDOM_DIRECT(widom_direct_mixed_lj1)
+ALEATOR_WIDOM_DIRECT(widom_direct_mixed_lj2)
+#undef ALEATOR_WIDOM_DIRECT
 
 }  // namespace aleator_gpu
