@@ -107,8 +107,9 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
         const double slope = a.trunc == ALEATOR_LJ_LINEAR_FORCE_SHIFTED ? a.ljtab[3 * kPairs + i] : 0.0;
         (&tab[0][0])[i] = make_double4(e4 * s6 * s6, e4 * s6, vrc - a.rc_lj * slope, slope);
     }
-    // the erf polynomials go through LDS: each pair loop loads its own into vector registers (as
-    // scalar registers they would not fit beside the descriptor and be spilled)
+    // the erf polynomials live in LDS and the pair loops read them there (the compiler keeps the
+    // current loop's coefficients in vector registers; held as two full register copies they
+    // took the kernel to 162 VGPRs and 3 waves per SIMD, now 126 and 4)
     if (tid <= DEG) s_poly[tid] = COUL ? a.poly[tid] : 0.0;
     if (tid <= DEGF) s_polyf[tid] = COUL ? a.poly_far[tid] : 0.0;
     const bool far_ok = COUL && a.far_ok;
@@ -118,7 +119,13 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
     const int bx = tile % t0;
     const int by = (tile / t0) % t1;
     const int bz = tile / (t0 * t1);
-    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
+    // threads cover the tile's valid nodes first: in an edge tile (the grid size need not be a
+    // multiple of the tile) the valid x, y columns and z groups are packed into the lowest thread
+    // ids, so its waves are full or skip the pair loop instead of running with idle lanes
+    const int vx = min(TX, n0 - bx * TX), vy = min(TY, n1 - by * TY);
+    const int vz = min(TZ, (n2 - bz * NZ + NPT - 1) / NPT);
+    const bool in_tile = tid < vx * vy * vz;
+    const int tx = in_tile ? tid % vx : 0, ty = in_tile ? (tid / vx) % vy : 0, tz = in_tile ? tid / (vx * vy) : 0;
     const int ix = bx * TX + tx, iy = by * TY + ty, iz0 = bz * NZ + tz * NPT;
 
     // tile centre in grid and image fractional coordinates
@@ -138,7 +145,7 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
     }
     bool valid[NPT];
 #pragma unroll
-    for (int p = 0; p < NPT; ++p) valid[p] = ix < n0 && iy < n1 && iz0 + p < n2;
+    for (int p = 0; p < NPT; ++p) valid[p] = in_tile && iz0 + p < n2;
     const bool any_valid = valid[0];
     // node p of a thread sits p steps s (one grid step along the third lattice row) past node 0,
     // so |e - o_p|^2 = |e - o_0|^2 - 2p (e - o_0).s + p^2 |s|^2: two instructions per extra node
@@ -170,9 +177,7 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
         constexpr bool INNER = decltype(inner_tag)::value;
         constexpr bool FAR = decltype(far_tag)::value;
         constexpr int D = FAR ? DEGF : DEG;
-        double pc[D + 1];
-#pragma unroll
-        for (int k = 0; k <= D; ++k) pc[k] = FAR ? s_polyf[k] : s_poly[k];
+        const double* pc = FAR ? s_polyf : s_poly;
         const double pxs = FAR ? a.poly_far_xs : a.poly_xs;
         const double pxo = FAR ? a.poly_far_xo : -1.0;
 #pragma unroll 2
@@ -199,7 +204,13 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
                     }
                 };
                 auto coul = [&](double ir) {
-                    if constexpr (COUL) accq[p] = fma(zq.y, ir - erf_over_r_poly<D>(r2, pc, pxs, pxo), accq[p]);
+                    if constexpr (COUL) {
+                        const double x = fma(r2, pxs, pxo);
+                        double pp = pc[D];
+#pragma unroll
+                        for (int k = D - 1; k >= 0; --k) pp = fma(pp, x, pc[k]);
+                        accq[p] = fma(zq.y, ir - pp, accq[p]);
+                    }
                 };
                 if (INNER || one_cut) {
                     if (INNER || r2 <= cut2) {
