@@ -7,7 +7,7 @@
 // a = 0..2; 18 vectors), 2^j (e_a + e_b) (j = 0..3, three axis pairs; 12), 2^j (1,1,1)
 // (j = 0..1; 2).
 //
-// A training iteration is three kernels:
+// A training iteration is two kernels:
 //  1. mlp_rows_kernel, one workgroup of 8 waves per block of 32 rows: draws the rows, computes
 //     their features, runs the 4 hidden layers forward, the head and the loss, and propagates
 //     dL/dz back to the first hidden layer, with the block's activations in LDS between layers.
@@ -17,11 +17,12 @@
 //     fragment order (one 16-byte load per lane and fragment).  The block's features, hidden
 //     activations and dL/dz go to global memory block-transposed (the block's 32 rows of a
 //     column contiguous), with per-block sums for the bias and output-layer gradients.
-//  2. mlp_wgrad_kernel: dW_l = H_{l-1}^T dZ_l in 64 x 64 tiles over ranges of row blocks; a
-//     32-row block is one MFMA K step whose operands are 16-byte loads of the block-transposed
-//     arrays.
-//  3. mlp_update_kernel: sums the partial gradients, then Adam on the FP32 master weights,
-//     refreshing the bf16 fragment copies.
+//  2. mlp_grad_adam_kernel: one workgroup per 32 x 32 tile of dW_l = H_{l-1}^T dZ_l over the
+//     whole batch (a 32-row block is one MFMA K step whose operands are 16-byte loads of the
+//     block-transposed arrays; the 8 waves take contiguous eighths of the blocks and their
+//     partial tiles are summed in LDS), followed in the same workgroup by Adam on the tile's
+//     FP32 master weights and the refresh of its bf16 fragment copies; further workgroups sum
+//     the per-block bias and output-layer gradients and update those parameters.
 // Every sum runs in a fixed order, so results are reproducible bit for bit.
 //
 // Operand layout of v_mfma_f32_16x16x32_bf16 (64 lanes): A lane l holds A[l % 16][8 (l / 16) ..
@@ -394,152 +395,31 @@ __global__ __launch_bounds__(kRowThreads) void mlp_rows_kernel(const RowArgs a) 
     }
 }
 
-struct WgradArgs {
-    const uint16_t* ht;
-    const uint16_t* dzt;
-    float* partial;    // [split][kMlpParams] (weight entries only)
-    int nb;            // row blocks
-    int per_split;     // row blocks per split
-    int splits;
-    const float* blockpart;
-    float* colsum;     // [kMlpBlockPart]: blockpart summed over the blocks
-    uint64_t* counters;
-    int bump;          // training: advance the iteration counter (read by the next row kernel)
-};
-
-constexpr int kWgradTiles = 2 + 3 * 4;    // 128 x 128 tiles: W1 (64 rows: 2 tiles), then 4 each of W2, W3, W4
-constexpr int kWgradThreads = 512;        // 8 waves, a 32 x 64 quarter-of-a-half each
-constexpr int kWgradChunk = 8;            // row blocks staged in LDS at once (128 KB: one round trip per split)
-constexpr int kColsumCols = 16;           // columns per column-sum workgroup
-constexpr int kColsumGroups = kWgradThreads / kColsumCols;
-constexpr int kColsumBlocks = (kMlpBlockPart + kColsumCols - 1) / kColsumCols;
-
-// Workgroups [0, kWgradTiles * splits8) with splits8 = splits rounded up to a multiple of 8: the
-// 128 x 128 weight-gradient tiles of each split.  The workgroup stages kWgradChunk row blocks of
-// its 128 input columns (H, block-transposed) and 128 output columns (dL/dz) in LDS, 16 KB per
-// block, read from memory once; wave w then sums the 32 x 64 quarter (rows 32 (w / 2), columns
-// 64 (w % 2)) over the blocks, one MFMA K step per block.  Workgroups are dispatched to the 8 XCDs
-// round robin, so workgroup x runs on XCD x % 8; the tiles of split s all go to XCD s % 8, where
-// they share the split's activations in L2.  The kColsumBlocks after them: the per-block partial
-// sums (bias and output-layer gradients, squared error) summed over the blocks, 16 columns per
-// workgroup: thread (g, c) sums blocks g, g + 32, ... of column c, then the group sums are added
-// in order.
-__global__ __launch_bounds__(kWgradThreads) void mlp_wgrad_kernel(const WgradArgs a) {
-    __shared__ __attribute__((aligned(16))) uint16_t s_op[kWgradChunk][2][128][kRB];  // [block][A, B][column][row]
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int splits8 = (a.splits + 7) & ~7;
-    if ((int)blockIdx.x >= kWgradTiles * splits8) {
-        float(*s_part)[kColsumCols] = reinterpret_cast<float(*)[kColsumCols]>(&s_op[0][0][0][0]);
-        const int g = tid / kColsumCols, c = tid % kColsumCols;
-        const int col = kColsumCols * (blockIdx.x - kWgradTiles * splits8) + c;
-        float s = 0.f;
-        if (col < kMlpBlockPart) {
-#pragma unroll 4
-            for (int b = g; b < a.nb; b += kColsumGroups) s += a.blockpart[(size_t)b * kMlpBlockPart + col];
-        }
-        s_part[g][c] = s;
-        __syncthreads();
-        if (g == 0 && col < kMlpBlockPart) {
-            float t = 0.f;
-#pragma unroll
-            for (int k = 0; k < kColsumGroups; ++k) t += s_part[k][c];
-            a.colsum[col] = t;
-        }
-        return;
-    }
-    const int r = blockIdx.x >> 3;
-    const int tile = r % kWgradTiles, split = (blockIdx.x & 7) + 8 * (r / kWgradTiles);
-    if (split >= a.splits) return;
-    int l, mt, nt;
-    if (tile < 2) {
-        l = 0;
-        mt = 0;
-        nt = tile;
-    } else {
-        l = 1 + (tile - 2) / 4;
-        mt = ((tile - 2) % 4) / 2;
-        nt = (tile - 2) % 2;
-    }
-    const int cols = l == 0 ? kMlpIn : kMlpHidden;
-    const int mrows = min(128, cols);  // input columns in this tile (64 for W1)
-    const uint16_t* A = a.ht + ht_offset(l, a.nb) + (size_t)mt * 128 * kRB;
-    const uint16_t* B = a.dzt + (size_t)l * a.nb * kMlpHidden * kRB + (size_t)nt * 128 * kRB;
-    const int wm = wave >> 1, wn = wave & 1;
-    const bool active = 32 * wm < mrows;
-    const int k8 = 8 * (lane >> 4), li = lane & 15;
-    const int b0 = split * a.per_split, b1 = min(a.nb, b0 + a.per_split);
-    f32x4 acc[2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int base = b0; base < b1; base += kWgradChunk) {
-        // stage: per block 128 columns x 32 rows of A and of B (each 8 KB contiguous in memory)
-        uint4 v[kWgradChunk * 2];
-#pragma unroll
-        for (int u = 0; u < kWgradChunk; ++u) {
-            const int blk = base + u;
-            const bool ok = blk < b1;
-#pragma unroll
-            for (int o = 0; o < 2; ++o) {
-                const int e = tid * 8;  // 512 threads x 8 values = 128 columns x 32 rows
-                const bool in = ok && (o == 1 || e < mrows * kRB);
-                const uint16_t* src = o == 0 ? A + (size_t)blk * cols * kRB : B + (size_t)blk * kMlpHidden * kRB;
-                v[2 * u + o] = in ? *reinterpret_cast<const uint4*>(src + e) : make_uint4(0, 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kWgradChunk; ++u)
-#pragma unroll
-            for (int o = 0; o < 2; ++o) *reinterpret_cast<uint4*>(&s_op[u][o][0][0] + tid * 8) = v[2 * u + o];
-        __syncthreads();
-        if (active) {
-#pragma unroll
-            for (int u = 0; u < kWgradChunk; ++u) {
-                bf16x8 af[2], bfr[4];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) af[i] = ld8(&s_op[u][0][32 * wm + 16 * i + li][k8]);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) bfr[j] = ld8(&s_op[u][1][64 * wn + 16 * j + li][k8]);
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-            }
-        }
-        __syncthreads();
-    }
-    if (active) {
-        float* out = a.partial + (size_t)split * kMlpParams + mlp_offset(2 * l);
-        const int rq = 4 * (lane >> 4);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    out[(128 * mt + 32 * wm + 16 * i + rq + q) * kMlpHidden + 128 * nt + 64 * wn + 16 * j + li] = acc[i][j][q];
-    }
-    if (a.bump && tile == 0 && split == 0 && tid == 0) a.counters[0] += 1;
-}
-
-struct UpdateArgs {
+struct GradArgs {
+    const uint16_t* ht;       // block-transposed layer inputs (X, H1..H3)
+    const uint16_t* dzt;      // block-transposed dL/dz of the 4 hidden layers
+    const float* blockpart;   // [block][kMlpBlockPart]
+    int nb;                   // row blocks
+    int rows;
     float* w;
     float* grad;
     float* m1;
     float* m2;
     uint16_t* wf;
     uint16_t* wb;
-    const float* partial;
-    int splits;
-    const float* colsum;
-    int rows;
     float lr;
-    int adam;         // 0: gradient and loss only
-    const uint64_t* counters;
+    int adam;                 // 0: gradient and loss only
+    uint64_t* counters;
     float* loss_hist;
 };
+
+constexpr int kGradThreads = 512;                          // 8 waves
+constexpr int kGradWaves = kGradThreads / 64;
+constexpr int kGradTiles = (kMlpIn / 32) * 8 + 3 * 8 * 8;  // 32 x 32 tiles: 16 of W1, then 64 each of W2..W4
+constexpr int kGradChunk = 8;                              // row blocks whose operands are loaded at once
+constexpr int kColsumCols = 16;                            // columns per column-sum workgroup
+constexpr int kColsumGroups = kGradThreads / kColsumCols;  // block groups per column
+constexpr int kColsumBlocks = (kMlpBlockPart + kColsumCols - 1) / kColsumCols;
 
 // region of parameter i: 0..9 = W1 b1 W2 b2 W3 b3 W4 b4 w5 b5
 __device__ inline int param_region(int i) {
@@ -557,41 +437,158 @@ __device__ inline void store_bf16_copies(uint16_t* wf, uint16_t* wb, int g, int 
     if (l > 0) wb[wb_offset(l) + frag_bwd(ii, jj)] = v;
 }
 
-// Gradient of parameter i (weights: the split partials summed in split order; biases and the
-// output layer: the block sums), the mean loss, and with adam the update: Adam step
-// t = counters[1] (bias corrections 1 - b^t).
-__global__ void mlp_update_kernel(const UpdateArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kMlpParams) return;
-    const int g = param_region(i);
-    float gr = 0.f;
-    if (g < 8 && (g & 1) == 0) {
-        float v[kMlpMaxSplits];  // every load in flight at once; splits past the count read as 0
-#pragma unroll
-        for (int s = 0; s < kMlpMaxSplits; ++s) v[s] = s < a.splits ? a.partial[(size_t)s * kMlpParams + i] : 0.f;
-#pragma unroll
-        for (int s = 0; s < kMlpMaxSplits; ++s) gr += v[s];
-    } else {
-        const int j = i - mlp_offset(g);
-        gr = a.colsum[g < 8 ? (g >> 1) * kMlpHidden + j : (g == 8 ? 4 * kMlpHidden + j : 5 * kMlpHidden)];
-    }
-    a.grad[i] = gr;
-    if (i == 0) {
-        // with adam the iteration counter has already advanced (mlp_wgrad_kernel)
-        a.loss_hist[(a.counters[0] - (a.adam ? 1 : 0)) % kLossChunk] = a.colsum[5 * kMlpHidden + 1] / (float)a.rows;
-    }
-    if (!a.adam) return;
+// Adam on parameter i with gradient g (step t, bias corrections 1 - b^t); returns the new value
+__device__ inline float adam_update(const GradArgs& a, int i, float g, float t) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const float t = (float)a.counters[1];
     const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
-    const float m = b1 * a.m1[i] + (1.f - b1) * gr;
-    const float v = b2 * a.m2[i] + (1.f - b2) * gr * gr;
+    const float m = b1 * a.m1[i] + (1.f - b1) * g;
+    const float v = b2 * a.m2[i] + (1.f - b2) * g * g;
     a.m1[i] = m;
     a.m2[i] = v;
-    const float upd = a.lr * (m / c1) / (sqrtf(v / c2) + eps);
-    const float nw = a.w[i] - upd;
+    const float nw = a.w[i] - a.lr * (m / c1) / (sqrtf(v / c2) + eps);
     a.w[i] = nw;
-    if (g < 8 && (g & 1) == 0) store_bf16_copies(a.wf, a.wb, g, i, f2bf(nw));
+    return nw;
+}
+
+// Gradients and, with adam, the Adam step, in one launch.  Workgroups [0, kGradTiles): one 32 x 32
+// tile of dW_l = H_{l-1}^T dZ_l over every row: wave w sums an eighth of the row blocks (one MFMA
+// K step per 32-row block, operands 16-byte loads of the block-transposed arrays, kGradChunk
+// blocks' loads in flight at once), the eight parts are added in wave order, and the tile's
+// weights are updated in place, with both bf16 fragment copies written as whole 16-byte
+// entries.  Workgroups after them: the per-block records (bias and output-layer gradients,
+// squared error) summed over the blocks, 16 columns per workgroup (thread (g, c) takes blocks
+// g, g + 32, ... of column c), and the update of those parameters; the workgroup holding the
+// loss also advances the iteration counter when training (nothing else in this launch reads it).
+// Every sum runs in a fixed order.
+__global__ __launch_bounds__(kGradThreads) void mlp_grad_adam_kernel(const GradArgs a) {
+    __shared__ float s_acc[kGradWaves][32][33];
+    __shared__ float s_w[32][33];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const float t_adam = (float)a.counters[1];  // advanced by the row kernel of this iteration
+    if ((int)blockIdx.x >= kGradTiles) {
+        float(*s_part)[kColsumCols] = reinterpret_cast<float(*)[kColsumCols]>(&s_acc[0][0][0]);
+        const int grp = tid / kColsumCols, c = tid % kColsumCols;
+        const int col = kColsumCols * (blockIdx.x - kGradTiles) + c;
+        float s = 0.f;
+        if (col < kMlpBlockPart) {
+#pragma unroll 4
+            for (int b = grp; b < a.nb; b += kColsumGroups) s += a.blockpart[(size_t)b * kMlpBlockPart + col];
+        }
+        s_part[grp][c] = s;
+        __syncthreads();
+        if (grp != 0 || col >= kMlpBlockPart) return;
+        float g = 0.f;
+        for (int k = 0; k < kColsumGroups; ++k) g += s_part[k][c];
+        if (col == 5 * kMlpHidden + 1) {
+            const uint64_t it = a.counters[0];
+            a.loss_hist[it % kLossChunk] = g / (float)a.rows;
+            if (a.adam) a.counters[0] = it + 1;
+            return;
+        }
+        const int i = col < 4 * kMlpHidden ? mlp_offset(2 * (col / kMlpHidden) + 1) + col % kMlpHidden
+                                           : mlp_offset(8) + (col - 4 * kMlpHidden);  // w5, then b5
+        a.grad[i] = g;
+        if (a.adam) adam_update(a, i, g, t_adam);
+        return;
+    }
+    // tile -> layer l, 32-row tile mt of fan_in, 32-column tile nt of the 256 outputs.  Workgroup
+    // b runs on XCD x = b mod 8 (round-robin dispatch); its 26 tiles are layer 0's column tile
+    // nt = x (both row tiles), then per hidden layer the 2 x 4 block of row tiles 2 (x / 2) + {0, 1}
+    // and column tiles 4 (x mod 2) + {0..3}, so an XCD reads a quarter of a layer's inputs and
+    // half of its dL/dz through its L2 instead of nearly all of both
+    int l, mt, nt;
+    {
+        const int x = blockIdx.x % 8, slot = blockIdx.x / 8;
+        if (slot < 2) {
+            l = 0;
+            mt = slot;
+            nt = x;
+        } else {
+            const int r = slot - 2;
+            l = 1 + r / 8;
+            mt = 2 * (x / 2) + (r % 8) / 4;
+            nt = 4 * (x % 2) + r % 4;
+        }
+    }
+    const int cols = l == 0 ? kMlpIn : kMlpHidden;
+    const uint16_t* A = a.ht + ht_offset(l, a.nb) + (size_t)(32 * mt) * kRB;
+    const uint16_t* B = a.dzt + (size_t)l * a.nb * kMlpHidden * kRB + (size_t)(32 * nt) * kRB;
+    const int k8 = 8 * (lane >> 4), li = lane & 15;
+    const int per = (a.nb + kGradWaves - 1) / kGradWaves, b0 = wave * per, b1 = min(a.nb, b0 + per);
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int base = b0; base < b1; base += kGradChunk) {
+        bf16x8 af[kGradChunk][2], bfr[kGradChunk][2];
+#pragma unroll
+        for (int u = 0; u < kGradChunk; ++u) {
+            const int blk = min(base + u, b1 - 1);  // past the end: a repeated block, zeroed below
+#pragma unroll
+            for (int i = 0; i < 2; ++i) af[u][i] = ld8(A + ((size_t)blk * cols + 16 * i + li) * kRB + k8);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bfr[u][j] = ld8(B + ((size_t)blk * kMlpHidden + 16 * j + li) * kRB + k8);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // the chunk's loads all in flight before the first MFMA
+#pragma unroll
+        for (int u = 0; u < kGradChunk; ++u) {
+            if (base + u >= b1) {  // a zero dL/dz operand adds nothing
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bfr[u][j] = bf16x8{};
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[u][i], bfr[u][j], acc[i][j], 0, 0, 0);
+        }
+    }
+    {
+        const int rq = 4 * (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) s_acc[wave][16 * i + rq + q][16 * j + li] = acc[i][j][q];
+    }
+    __syncthreads();
+    // the tile's 1024 parameters, 2 consecutive columns per thread
+    {
+        const int r = tid >> 4, c0 = (tid & 15) * 2;
+        const int base_i = mlp_offset(2 * l) + (32 * mt + r) * kMlpHidden + 32 * nt + c0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int c = c0 + q;
+            float g = 0.f;
+#pragma unroll
+            for (int v = 0; v < kGradWaves; ++v) g += s_acc[v][r][c];
+            a.grad[base_i + q] = g;
+            if (a.adam) s_w[r][c] = adam_update(a, base_i + q, g, t_adam);
+        }
+    }
+    if (!a.adam) return;
+    __syncthreads();
+    // bf16 copies as whole 16-byte fragment entries: threads 0-127 the forward copy (8 consecutive
+    // input rows of one output column), threads 128-255 the backward copy (8 consecutive output
+    // columns of one input row; W2..W4 only)
+    const int f = (tid >> 6) & 1;
+    uint32_t u[4];
+    size_t dst;
+    if (tid < 128) {
+        const int rr = 8 * (lane >> 4), cc = 16 * f + (lane & 15);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) u[e] = f2bf2(s_w[rr + 2 * e][cc], s_w[rr + 2 * e + 1][cc]);
+        dst = (size_t)wf_offset(l) + ((size_t)((2 * nt + f) * (cols / 32) + mt) * 64 + lane) * 8;
+        *reinterpret_cast<uint4*>(a.wf + dst) = make_uint4(u[0], u[1], u[2], u[3]);
+    } else if (l > 0) {
+        const int rr = 16 * f + (lane & 15), cc = 8 * (lane >> 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) u[e] = f2bf2(s_w[rr][cc + 2 * e], s_w[rr][cc + 2 * e + 1]);
+        dst = (size_t)wb_offset(l) + ((size_t)((2 * mt + f) * 8 + nt) * 64 + lane) * 8;
+        *reinterpret_cast<uint4*>(a.wb + dst) = make_uint4(u[0], u[1], u[2], u[3]);
+    }
 }
 
 // bf16 fragment copies of every weight of W1..W4
@@ -634,13 +631,6 @@ const int kMlpOffsets[11] = {mlp_offset(0), mlp_offset(1), mlp_offset(2), mlp_of
 namespace {
 int blocks_of(int rows) { return (rows + kRB - 1) / kRB; }
 
-// splits of the weight-gradient sums over nb row blocks: kWgradChunk blocks each (one LDS stage),
-// at most kMlpMaxSplits
-void wgrad_split(int nb, int& splits, int& per_split) {
-    splits = std::min(kMlpMaxSplits, std::max(1, nb / kWgradChunk));
-    per_split = (nb + splits - 1) / splits;
-    splits = (nb + per_split - 1) / per_split;
-}
 }  // namespace
 
 Mlp::Mlp(Context* c, int max_rows) : ctx(c), cap(max_rows) {
@@ -661,8 +651,6 @@ Mlp::Mlp(Context* c, int max_rows) : ctx(c), cap(max_rows) {
         alloc(&ht, nb * kHtPerBlock);
         alloc(&dzt, nb * kDztPerBlock);
         alloc(&blockpart, nb * kMlpBlockPart);
-        alloc(&colsum, kMlpBlockPart);
-        alloc(&partial, (size_t)kMlpMaxSplits * kMlpParams);
         alloc(&loss_hist, kLossChunk);
         alloc(&counters, 2);
         ALEATOR_HIP_CHECK(hipMemcpyAsync(kvec, kv.data(), kv.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -726,37 +714,23 @@ void training_pass(Mlp& net, const double* frac, const double* target, uint64_t 
     ra.dzt = net.dzt;
     ra.blockpart = net.blockpart;
     hipLaunchKernelGGL(mlp_rows_kernel<true>, dim3(nb), dim3(kRowThreads), 0, s, ra);
-    const bool train = mode == kTrain;
-    WgradArgs wa{};
-    wa.ht = net.ht;
-    wa.dzt = net.dzt;
-    wa.partial = net.partial;
-    wa.nb = nb;
-    int splits = 1;
-    wgrad_split(nb, splits, wa.per_split);
-    wa.splits = splits;
-    wa.blockpart = net.blockpart;
-    wa.colsum = net.colsum;
-    wa.counters = net.counters;
-    wa.bump = train ? 1 : 0;
-    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(kWgradTiles * ((splits + 7) & ~7) + kColsumBlocks), dim3(kWgradThreads), 0,
-                       s, wa);
-    UpdateArgs ua{};
-    ua.w = net.w;
-    ua.grad = net.grad;
-    ua.m1 = net.m1;
-    ua.m2 = net.m2;
-    ua.wf = net.wf;
-    ua.wb = net.wb;
-    ua.partial = net.partial;
-    ua.splits = splits;
-    ua.colsum = net.colsum;
-    ua.rows = rows;
-    ua.lr = lr;
-    ua.adam = train ? 1 : 0;
-    ua.counters = net.counters;
-    ua.loss_hist = net.loss_hist;
-    hipLaunchKernelGGL(mlp_update_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, ua);
+    GradArgs ga{};
+    ga.ht = net.ht;
+    ga.dzt = net.dzt;
+    ga.blockpart = net.blockpart;
+    ga.nb = nb;
+    ga.rows = rows;
+    ga.w = net.w;
+    ga.grad = net.grad;
+    ga.m1 = net.m1;
+    ga.m2 = net.m2;
+    ga.wf = net.wf;
+    ga.wb = net.wb;
+    ga.lr = lr;
+    ga.adam = mode == kTrain ? 1 : 0;
+    ga.counters = net.counters;
+    ga.loss_hist = net.loss_hist;
+    hipLaunchKernelGGL(mlp_grad_adam_kernel, dim3(kGradTiles + kColsumBlocks), dim3(kGradThreads), 0, s, ga);
 }
 }  // namespace
 

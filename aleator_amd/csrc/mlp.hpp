@@ -13,7 +13,6 @@ constexpr int kMlpFreqs = 32;
 constexpr int kMlpHidden = 256;  // 4 hidden layers
 constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + 3 * (kMlpHidden * kMlpHidden + kMlpHidden) + kMlpHidden + 1;
 constexpr int kMlpBlockRows = 32;  // rows per workgroup of the row kernel (two 16-row MFMA tiles)
-constexpr int kMlpMaxSplits = 16;  // row-block ranges of the weight-gradient kernel
 constexpr int kLossChunk = 1 << 16;  // iterations whose losses are kept on the device at once
 // per row block: bias gradients of the 4 hidden layers, then dw5 (256), db5 and the summed
 // squared error
@@ -53,8 +52,6 @@ struct Mlp {
     uint16_t* ht = nullptr;
     uint16_t* dzt = nullptr;     // dL/dz of the 4 hidden layers, same layout (256 columns each)
     float* blockpart = nullptr;  // [block][kMlpBlockPart]
-    float* colsum = nullptr;     // [kMlpBlockPart]: blockpart summed over the blocks
-    float* partial = nullptr;    // weight-gradient partial sums [split][kMlpParams]
     float* loss_hist = nullptr;  // per-iteration loss of the current train call chunk
     uint64_t* counters = nullptr;   // device: [0] iteration of the current train call, [1] Adam steps taken
     // a training iteration captured as a hipGraph (kernels only), replayed for every iteration
