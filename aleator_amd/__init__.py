@@ -514,6 +514,13 @@ class Framework:
         (aleator_gpu_framework_set_lj_tail; off by default, as in the reference)."""
         _check(_load().aleator_gpu_framework_set_lj_tail(self._h, C.c_int32(1 if enable else 0)))
 
+    def nonfinite_count(self) -> int:
+        """Non-finite table values written by this framework's grid builds so far
+        (aleator_gpu_framework_nonfinite_count)."""
+        n = C.c_uint64()
+        _check(_load().aleator_gpu_framework_nonfinite_count(self._h, C.byref(n)))
+        return n.value
+
     def grid(self, guests, n_points, coulomb=False, precision=FP64, cap=1.0e5, grid_lattice=None):
         """Energy tables [len(guests) (+1 Coulomb)][nz][ny][nx] in K (Coulomb: K/e)."""
         g = np.ascontiguousarray(guests, dtype=np.uint32)
@@ -812,6 +819,28 @@ class Gcmc:
         out = C.c_double()
         _check(_load().aleator_gpu_gcmc_electrostatic_energy(self._h, C.c_int32(chain), C.byref(out)))
         return out.value
+
+    def resync_check(self):
+        """Per chain: the largest relative drift |fresh - committed| / max(1, |fresh|) of the Ewald
+        energy seen at a resync (aleator_gpu_gcmc_resync_check), and the number of resyncs."""
+        drift = np.zeros(self.n_chains)
+        count = np.zeros(self.n_chains, dtype=np.uint64)
+        _check(_load().aleator_gpu_gcmc_resync_check(self._h, _p(drift), _p(count, C.POINTER(C.c_uint64))))
+        return drift, count
+
+    def save_chain(self, chain: int) -> bytes:
+        """One chain's state as the checkpoint record of aleator_gpu_gcmc_save_chain."""
+        n = C.c_uint64()
+        _check(_load().aleator_gpu_gcmc_checkpoint_size(self._h, C.c_int32(chain), C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        written = C.c_uint64()
+        _check(_load().aleator_gpu_gcmc_save_chain(self._h, C.c_int32(chain), buf, n, C.byref(written)))
+        return buf.raw[:written.value]
+
+    def load_chain(self, chain: int, record: bytes):
+        """Restores one chain from a save_chain record (aleator_gpu_gcmc_load_chain)."""
+        _check(_load().aleator_gpu_gcmc_load_chain(self._h, C.c_int32(chain), C.c_char_p(bytes(record)),
+                                                   C.c_uint64(len(record))))
 
 
 def grid_interpolate_device(ctx: "Context", lattice, n_points, table_ptr: int, positions_ptr: int, n: int,

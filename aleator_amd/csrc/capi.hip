@@ -317,7 +317,7 @@ void release(Framework* f) {
     (void)hipStreamSynchronize(c->stream);
     for (void* p : {(void*)f->d_fx, (void*)f->d_fy, (void*)f->d_fz, (void*)f->d_q, (void*)f->d_type,
                     (void*)f->d_kn, (void*)f->d_kw, (void*)f->d_kseg, (void*)f->d_sre, (void*)f->d_sim,
-                    (void*)f->d_widom_tab})
+                    (void*)f->d_widom_tab, (void*)f->d_nonfinite})
         if (p) (void)hipFree(p);
     delete static_cast<aleator_framework*>(f);
     release(c);
@@ -434,6 +434,8 @@ aleator_status aleator_gpu_framework_create(aleator_ctx* ctx, const aleator_latt
         fw->d_fz = device_upload(fw->fz.data(), n, s);
         fw->d_q = device_upload(fw->q.data(), n, s);
         fw->d_type = device_upload(fw->type.data(), n, s);
+        ALEATOR_HIP_CHECK(hipMalloc(&fw->d_nonfinite, sizeof(unsigned long long)));
+        ALEATOR_HIP_CHECK(hipMemsetAsync(fw->d_nonfinite, 0, sizeof(unsigned long long), s));
         if (ewald) {
             require(ewald->alpha > 0.0, ALEATOR_INVALID_ARGUMENT, "Ewald: alpha must be positive");
             require(ewald->real_space_cutoff > 0.0, ALEATOR_INVALID_ARGUMENT,
@@ -508,6 +510,16 @@ aleator_status aleator_gpu_framework_num_k(const aleator_framework* fw, uint64_t
     return guarded([&] {
         require(fw && nk, ALEATOR_INVALID_ARGUMENT, "null argument");
         *nk = fw->kset.size();
+    });
+}
+
+aleator_status aleator_gpu_framework_nonfinite_count(aleator_framework* fw, uint64_t* count) {
+    return guarded([&] {
+        require(fw && count, ALEATOR_INVALID_ARGUMENT, "null argument");
+        unsigned long long v = 0;
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(&v, fw->d_nonfinite, sizeof(v), hipMemcpyDeviceToHost, fw->ctx->stream));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(fw->ctx->stream));
+        *count = v;
     });
 }
 

@@ -348,6 +348,8 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs* __restri
         double pend_self, pend_excl;  // self / exclusion deltas of the pending move
         uint64_t att[4], acc[4];
         uint64_t step_call;           // Philox call index of the current step's first draw
+        double max_drift;             // resync consistency check (ChainState::max_drift)
+        uint64_t resyncs;
     } t0;
 
     const int chain = blockIdx.x;
@@ -386,6 +388,8 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs* __restri
             t0.att[i] = st->attempted[i];
             t0.acc[i] = st->accepted[i];
         }
+        t0.max_drift = st->max_drift;
+        t0.resyncs = st->resyncs;
     }
     const double volume = fabs(a.volume);
 
@@ -702,7 +706,10 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs* __restri
             // maybeResyncEwald (:380-403)
             if (CHARGED && ++since_resync >= 100) {
                 since_resync = 0;
+                // the committed reciprocal energy before the recompute, for the consistency check
+                double e_old = 0.0;
                 for (int t = tid; t < a.nk_half; t += NT) {
+                    e_old += s_kw[t] * (s_re[t] * s_re[t] + s_im[t] * s_im[t]);
                     s_re[t] = a.fw_sre[t];
                     s_im[t] = a.fw_sim[t];
                 }
@@ -740,8 +747,14 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs* __restri
                         s_im[t] += dim;
                     }
                 }
+                // each thread re-reads only the terms it updated itself, so no barrier is needed
+                double e_new = 0.0;
+                for (int t = tid; t < a.nk_half; t += NT) e_new += s_kw[t] * (s_re[t] * s_re[t] + s_im[t] * s_im[t]);
+                double r4[4] = {e_old, e_new, 0.0, 0.0};
+                block_sum4<NW>(r4, red);
                 // self and exclusion from scratch (ewald_incremental_state.cc:230-246)
                 if (tid == 0) {
+                    const double committed = a.recip_scale * r4[0] + t0.self_e + t0.excl_e;
                     double q2 = 0.0;
                     for (int s = 0; s < ns; ++s) q2 += a.charge[s] * a.charge[s];
                     t0.self_e = a.fw_self + a.self_pref * (q2 * (double)n_mol);
@@ -760,6 +773,12 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs* __restri
                                       (r < 1e-8 ? 2.0 * a.alpha / sqrt(kPi) : erf(a.alpha * r) / r);
                             }
                     t0.excl_e = -a.kc * ex;
+                    // the test of test_ewald_incremental_state.cc:137-222, at every resync: relative
+                    // drift of the incrementally kept energy from the from-scratch one; a NaN sticks
+                    const double fresh = a.recip_scale * r4[1] + t0.self_e + t0.excl_e;
+                    const double drift = fabs(fresh - committed) / fmax(1.0, fabs(fresh));
+                    if (!isnan(t0.max_drift) && !(drift <= t0.max_drift)) t0.max_drift = drift;
+                    t0.resyncs += 1;
                 }
                 __syncthreads();
             }
@@ -794,6 +813,8 @@ __global__ __launch_bounds__(NT) void gcmc_chain_kernel(const GcmcArgs* __restri
             st->accepted[i] = t0.acc[i];
         }
         st->error = error;
+        st->max_drift = t0.max_drift;
+        st->resyncs = t0.resyncs;
     }
 }
 

@@ -27,6 +27,10 @@ struct ChainState {
     uint64_t attempted[4], accepted[4];  // translate, rotate, insert, delete
     int32_t error;             // 1: molecule capacity exceeded
     int32_t pad;
+    // consistency check at every resync: largest |fresh - committed| / max(1, |fresh|) of the
+    // Ewald energy (reciprocal + self + exclusion), and the number of resyncs it covers
+    double max_drift;
+    uint64_t resyncs;
 };
 
 struct GcmcArgs {

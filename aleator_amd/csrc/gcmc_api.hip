@@ -468,6 +468,8 @@ aleator_status aleator_gpu_gcmc_reset_stats(aleator_gcmc* g) {
             c.steps = 0;
             std::memset(c.attempted, 0, sizeof(c.attempted));
             std::memset(c.accepted, 0, sizeof(c.accepted));
+            c.max_drift = 0.0;
+            c.resyncs = 0;
         }
         ALEATOR_HIP_CHECK(hipMemcpyAsync(g->d_state, st.data(), st.size() * sizeof(ChainState), hipMemcpyHostToDevice,
                                          g->fw->ctx->stream));
@@ -532,6 +534,151 @@ aleator_status aleator_gpu_gcmc_guest_cells(const aleator_gcmc* g, int32_t cells
     return guarded([&] {
         require(g != nullptr && cells != nullptr, ALEATOR_INVALID_ARGUMENT, "null argument");
         for (int j = 0; j < 3; ++j) cells[j] = g->args.gc[j];
+    });
+}
+
+aleator_status aleator_gpu_gcmc_resync_check(aleator_gcmc* g, double* max_drift, uint64_t* resyncs) {
+    return guarded([&] {
+        require(g && max_drift && resyncs, ALEATOR_INVALID_ARGUMENT, "null argument");
+        const auto st = download_states(g);
+        for (int c = 0; c < g->n_chains; ++c) {
+            max_drift[c] = st[c].max_drift;
+            resyncs[c] = st[c].resyncs;
+        }
+    });
+}
+
+}  // extern "C"
+
+namespace aleator_gpu {
+namespace {
+// Chain checkpoint record (layout documented with aleator_gpu_gcmc_save_chain).
+constexpr uint32_t kCheckpointMagic = 0x4b434741u;  // "AGCK"
+constexpr uint32_t kCheckpointVersion = 1;
+constexpr size_t kCheckpointHeader = 144;
+
+size_t checkpoint_bytes(const aleator_gcmc* g, int n_mol) {
+    const size_t nkh = g->fw->charged ? (size_t)g->fw->nk_half : 0;
+    return kCheckpointHeader + 16 * nkh + 24 * (size_t)n_mol * g->args.ns;
+}
+
+template <class T>
+void put(unsigned char* buf, size_t off, T v) {
+    std::memcpy(buf + off, &v, sizeof(T));
+}
+template <class T>
+T get(const unsigned char* buf, size_t off) {
+    T v;
+    std::memcpy(&v, buf + off, sizeof(T));
+    return v;
+}
+}  // namespace
+}  // namespace aleator_gpu
+
+extern "C" {
+
+aleator_status aleator_gpu_gcmc_checkpoint_size(aleator_gcmc* g, int32_t chain, uint64_t* bytes) {
+    return guarded([&] {
+        require(g && bytes, ALEATOR_INVALID_ARGUMENT, "null argument");
+        require(chain >= 0 && chain < g->n_chains, ALEATOR_OUT_OF_RANGE, "chain index out of range");
+        *bytes = checkpoint_bytes(g, download_states(g)[chain].n_mol);
+    });
+}
+
+aleator_status aleator_gpu_gcmc_save_chain(aleator_gcmc* g, int32_t chain, void* buffer, uint64_t capacity,
+                                           uint64_t* written) {
+    return guarded([&] {
+        require(g && buffer && written, ALEATOR_INVALID_ARGUMENT, "null argument");
+        require(chain >= 0 && chain < g->n_chains, ALEATOR_OUT_OF_RANGE, "chain index out of range");
+        const ChainState st = download_states(g)[chain];
+        const size_t need = checkpoint_bytes(g, st.n_mol);
+        require(capacity >= need, ALEATOR_INVALID_ARGUMENT, "gcmc checkpoint: buffer too small");
+        auto* b = static_cast<unsigned char*>(buffer);
+        const int nkh = g->fw->charged ? g->fw->nk_half : 0;
+        const int ns = g->args.ns;
+        put<uint32_t>(b, 0, kCheckpointMagic);
+        put<uint32_t>(b, 4, kCheckpointVersion);
+        put<int32_t>(b, 8, ns);
+        put<int32_t>(b, 12, st.n_mol);
+        put<int32_t>(b, 16, nkh);
+        put<int32_t>(b, 20, st.moves_since_resync);
+        put<uint64_t>(b, 24, st.call);
+        put<double>(b, 32, st.self_energy);
+        put<double>(b, 40, st.exclusion_energy);
+        put<double>(b, 48, st.sum_n);
+        put<uint64_t>(b, 56, st.steps);
+        for (int i = 0; i < 4; ++i) {
+            put<uint64_t>(b, 64 + 8 * i, st.attempted[i]);
+            put<uint64_t>(b, 96 + 8 * i, st.accepted[i]);
+        }
+        put<double>(b, 128, st.max_drift);
+        put<uint64_t>(b, 136, st.resyncs);
+        hipStream_t s = g->fw->ctx->stream;
+        size_t off = kCheckpointHeader;
+        if (nkh) {
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(b + off, g->d_sre + (size_t)chain * nkh, nkh * sizeof(double),
+                                             hipMemcpyDeviceToHost, s));
+            off += nkh * sizeof(double);
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(b + off, g->d_sim + (size_t)chain * nkh, nkh * sizeof(double),
+                                             hipMemcpyDeviceToHost, s));
+            off += nkh * sizeof(double);
+        }
+        if (st.n_mol)
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(b + off, g->d_pos + (size_t)chain * g->args.max_mol * ns * 3,
+                                             (size_t)st.n_mol * ns * 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
+        *written = need;
+    });
+}
+
+aleator_status aleator_gpu_gcmc_load_chain(aleator_gcmc* g, int32_t chain, const void* buffer, uint64_t size) {
+    return guarded([&] {
+        require(g && buffer, ALEATOR_INVALID_ARGUMENT, "null argument");
+        require(chain >= 0 && chain < g->n_chains, ALEATOR_OUT_OF_RANGE, "chain index out of range");
+        const auto* b = static_cast<const unsigned char*>(buffer);
+        require(size >= kCheckpointHeader && get<uint32_t>(b, 0) == kCheckpointMagic, ALEATOR_RUNTIME_ERROR,
+                "gcmc checkpoint: not a chain checkpoint");
+        require(get<uint32_t>(b, 4) == kCheckpointVersion, ALEATOR_RUNTIME_ERROR,
+                "gcmc checkpoint: unsupported version");
+        const int nkh = g->fw->charged ? g->fw->nk_half : 0;
+        const int ns = g->args.ns;
+        const int n_mol = get<int32_t>(b, 12);
+        require(get<int32_t>(b, 8) == ns && get<int32_t>(b, 16) == nkh, ALEATOR_INVALID_ARGUMENT,
+                "gcmc checkpoint: sites or k-set differ from these chains");
+        require(n_mol >= 0 && n_mol <= g->args.max_mol, ALEATOR_INVALID_ARGUMENT,
+                "gcmc checkpoint: more molecules than max_molecules");
+        require(size == checkpoint_bytes(g, n_mol), ALEATOR_INVALID_ARGUMENT, "gcmc checkpoint: wrong size");
+        auto all = download_states(g);
+        ChainState& st = all[chain];
+        st.n_mol = n_mol;
+        st.moves_since_resync = get<int32_t>(b, 20);
+        st.call = get<uint64_t>(b, 24);
+        st.self_energy = get<double>(b, 32);
+        st.exclusion_energy = get<double>(b, 40);
+        st.sum_n = get<double>(b, 48);
+        st.steps = get<uint64_t>(b, 56);
+        for (int i = 0; i < 4; ++i) {
+            st.attempted[i] = get<uint64_t>(b, 64 + 8 * i);
+            st.accepted[i] = get<uint64_t>(b, 96 + 8 * i);
+        }
+        st.max_drift = get<double>(b, 128);
+        st.resyncs = get<uint64_t>(b, 136);
+        st.error = 0;
+        hipStream_t s = g->fw->ctx->stream;
+        size_t off = kCheckpointHeader;
+        if (nkh) {
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(g->d_sre + (size_t)chain * nkh, b + off, nkh * sizeof(double),
+                                             hipMemcpyHostToDevice, s));
+            off += nkh * sizeof(double);
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(g->d_sim + (size_t)chain * nkh, b + off, nkh * sizeof(double),
+                                             hipMemcpyHostToDevice, s));
+            off += nkh * sizeof(double);
+        }
+        if (n_mol)
+            ALEATOR_HIP_CHECK(hipMemcpyAsync(g->d_pos + (size_t)chain * g->args.max_mol * ns * 3, b + off,
+                                             (size_t)n_mol * ns * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+        ALEATOR_HIP_CHECK(hipMemcpyAsync(g->d_state + chain, &st, sizeof(ChainState), hipMemcpyHostToDevice, s));
+        ALEATOR_HIP_CHECK(hipStreamSynchronize(s));
     });
 }
 

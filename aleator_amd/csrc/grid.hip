@@ -120,6 +120,7 @@ void GridJob::prepare(Framework& fw, const GridRequest& req) {
     a.fz = fw.d_fz;
     a.q = fw.d_q;
     a.type = fw.d_type;
+    a.nonfinite = fw.d_nonfinite;
     a.natoms = (int)fw.n;
     a.rc_lj = fw.rc;
     a.rc2_lj = fw.rc * fw.rc;

@@ -46,6 +46,7 @@ struct GridArgs {
     double poly_far_xs, poly_far_xo;
     int far_ok;
     double tail[kMaxGuests];          // analytic LJ tail of one guest site, added before the cap
+    unsigned long long* nonfinite;    // running count of non-finite values written (the framework's)
 };
 
 // One framework's build: host-side constants, their device placement, pre-kernels.

@@ -318,6 +318,7 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
     compute(pend_in, pend_out);
 
     const int nk = 2 * a.kmax + 1;
+    unsigned nonfinite = 0;  // NaN / Inf sentinel: counted, the values are written as computed
 #pragma unroll
     for (int p = 0; p < NPT; ++p) {
         if (!valid[p]) continue;
@@ -326,7 +327,9 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             const double e = acc[p][g] + a.tail[g];
-            a.out[g * a.stride + flat] = (a.cap < e) ? a.cap : e;  // std::min(e, cap), NaN kept
+            const double v = (a.cap < e) ? a.cap : e;  // std::min(e, cap), NaN kept
+            nonfinite += !isfinite(v);
+            a.out[g * a.stride + flat] = v;
         }
         if constexpr (COUL) {
             double rec = 0.0;
@@ -335,9 +338,12 @@ __global__ __launch_bounds__(tile_detail::kThreads) void grid_tile_kernel(const 
                 const double2 ee = a.e0[(long long)m * n0 + ix];
                 rec += tt.x * ee.x - tt.y * ee.y;
             }
-            a.out[NG * a.stride + flat] = a.kc * accq[p] + a.recip_pref * rec;
+            const double v = a.kc * accq[p] + a.recip_pref * rec;
+            nonfinite += !isfinite(v);
+            a.out[NG * a.stride + flat] = v;
         }
     }
+    if (nonfinite) atomicAdd(a.nonfinite, (unsigned long long)nonfinite);
 }
 
 // ---------------------------------------------------------------- instantiation and dispatch

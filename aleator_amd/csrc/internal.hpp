@@ -137,6 +137,8 @@ struct Framework {
     // device atoms
     double *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr, *d_q = nullptr;
     int32_t* d_type = nullptr;
+    // running count of non-finite grid table values written by this framework's builds
+    unsigned long long* d_nonfinite = nullptr;
     // Ewald
     bool charged = false;
     double alpha = 0.0, rc_real = 0.0;

@@ -148,6 +148,13 @@ aleator_status aleator_gpu_grid_point_counts(const aleator_lattice* grid_lattice
  * of the framework plus the inserted sites minus that of the framework alone (the same
  * quantity as aleator_lj_tail_insertion_delta). */
 aleator_status aleator_gpu_framework_set_lj_tail(aleator_framework* fw, int32_t enable);
+/* No reference counterpart (a NaN / Inf sentinel; the reference writes such values silently):
+ * the number of non-finite table values (NaN, +-Inf) that the grid builds of this framework
+ * have written since it was created, over every entry point that builds its tables (per call,
+ * allocation-free and batched plans).  The values themselves are written as computed (an atom
+ * exactly on a node gives an infinite Coulomb value, as in the reference); callers compare the
+ * count before and after a build.  Waits for the context stream. */
+aleator_status aleator_gpu_framework_nonfinite_count(aleator_framework* fw, uint64_t* count);
 /* Builds one table per entry of `guest_species` (FrameworkEnergyGrid ctor body,
  * framework_energy_grid.cc:221-248): E(node) = min(sum_f pairEnergy(r, mix(guest,f)), cap).
  * When `coulomb` is non-zero and the framework is charged, one more table follows: the
@@ -357,6 +364,34 @@ aleator_status aleator_gpu_gcmc_positions(aleator_gcmc* gcmc, int32_t chain, dou
                                           uint64_t capacity, int32_t* n_molecules);
 /* committedEnergy of the chain's Ewald state (ewald_incremental_state.cc:85-93). */
 aleator_status aleator_gpu_gcmc_electrostatic_energy(aleator_gcmc* gcmc, int32_t chain, double* energy);
+/* Consistency check at every Ewald resync, the reference's drift gate
+ * (test_ewald_incremental_state.cc:137-222, tolerance 1e-10) run inside the chains: before each
+ * resync replaces the committed S(k), self and exclusion terms by the from-scratch ones
+ * (ewald_incremental_state.cc:174-247), the kernel records |fresh - committed| /
+ * max(1, |fresh|) of the Ewald energy.  Per chain [n_chains]: the largest such drift (NaN once
+ * a non-finite energy was seen) and the number of resyncs it covers, since creation or
+ * aleator_gpu_gcmc_reset_stats.  Uncharged chains never resync and report 0. */
+aleator_status aleator_gpu_gcmc_resync_check(aleator_gcmc* gcmc, double* max_drift, uint64_t* resyncs);
+/* No reference counterpart (the reference keeps no GCMC checkpoint; its RNG state is resumable
+ * by construction, counter_based_rng.hpp:59-66): one chain's whole state as a flat record, to
+ * resume a run or move a chain between chain sets of the same framework and species.
+ * Little-endian layout, byte offsets:
+ *   0 u32 magic 0x4b434741 ("AGCK")   4 u32 version 1
+ *   8 i32 n_sites                    12 i32 n_molecules
+ *  16 i32 nk_half (0 uncharged)      20 i32 accepted moves since the last resync
+ *  24 u64 Philox call index of the next draw
+ *  32 f64 self energy   40 f64 exclusion energy   48 f64 sum of molecule counts   56 u64 steps
+ *  64 u64 attempted[4]  96 u64 accepted[4]  (translation, rotation, insertion, deletion)
+ * 128 f64 max resync drift          136 u64 resyncs
+ * 144 f64 S_re[nk_half], f64 S_im[nk_half] (the first half of the reference-order k-set),
+ *     then f64 xyz[n_molecules][n_sites][3] (Cartesian, molecule order).
+ * checkpoint_size gives the record's size for the chain's current molecule count.  load_chain
+ * checks magic, version, site count, k-set size and max_molecules, and clears the chain's
+ * capacity error. */
+aleator_status aleator_gpu_gcmc_checkpoint_size(aleator_gcmc* gcmc, int32_t chain, uint64_t* bytes);
+aleator_status aleator_gpu_gcmc_save_chain(aleator_gcmc* gcmc, int32_t chain, void* buffer, uint64_t capacity,
+                                           uint64_t* written);
+aleator_status aleator_gpu_gcmc_load_chain(aleator_gcmc* gcmc, int32_t chain, const void* buffer, uint64_t size);
 aleator_status aleator_gpu_gcmc_destroy(aleator_gcmc* gcmc);
 /* No reference counterpart (learned proposal): makes every later insertion draw its anchor
  * point from a piecewise-constant density instead of uniformly, and corrects both insertion

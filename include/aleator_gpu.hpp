@@ -149,6 +149,12 @@ public:
     Framework& operator=(const Framework&) = delete;
     [[nodiscard]] aleator_framework* handle() const noexcept { return fw_; }
     [[nodiscard]] Device& device() const noexcept { return *dev_; }
+    /// NaN / Inf sentinel: non-finite table values written by this framework's grid builds.
+    [[nodiscard]] std::uint64_t nonfiniteCount() const {
+        std::uint64_t n = 0;
+        check(aleator_gpu_framework_nonfinite_count(fw_, &n));
+        return n;
+    }
 
 private:
     template <class LJ, class P, class Lat>
@@ -417,6 +423,26 @@ public:
         double e = 0.0;
         check(aleator_gpu_gcmc_electrostatic_energy(g_, chain, &e));
         return e;
+    }
+    /// Largest relative Ewald-energy drift seen at a resync, per chain (the reference's 1e-10
+    /// drift gate, test_ewald_incremental_state.cc:137-222, checked inside the chains).
+    [[nodiscard]] std::vector<double> maxResyncDrift() const {
+        std::vector<double> drift(n_chains_);
+        std::vector<std::uint64_t> count(n_chains_);
+        check(aleator_gpu_gcmc_resync_check(g_, drift.data(), count.data()));
+        return drift;
+    }
+    /// One chain's state as a checkpoint record; loadChain restores it.
+    [[nodiscard]] std::vector<unsigned char> saveChain(int chain) const {
+        std::uint64_t n = 0, written = 0;
+        check(aleator_gpu_gcmc_checkpoint_size(g_, chain, &n));
+        std::vector<unsigned char> rec(n);
+        check(aleator_gpu_gcmc_save_chain(g_, chain, rec.data(), rec.size(), &written));
+        rec.resize(written);
+        return rec;
+    }
+    void loadChain(int chain, const std::vector<unsigned char>& rec) {
+        check(aleator_gpu_gcmc_load_chain(g_, chain, rec.data(), rec.size()));
     }
     [[nodiscard]] aleator_gcmc* handle() const noexcept { return g_; }
 
