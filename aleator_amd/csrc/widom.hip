@@ -46,8 +46,8 @@ __global__ __launch_bounds__(256) void widom_recip_runs_kernel(const WidomArgs a
     const uint64_t t = (uint64_t)blockIdx.x * rt + j;
     const bool valid = t < a.n_trials;
     if (g == 0 && valid) {
-        double cart[kMaxSites][3];
-        trial_sites(a, t, cart);
+        double cart[NS][3];
+        trial_sites<NS>(a, t, cart);
         for (int s = 0; s < NS; ++s) to_frac(a, cart[s], s_f[j][s]);
         // intramolecular exclusion: -kC sum_{i<j} q_i q_j erf(alpha r)/r
         // (ewald_incremental_state.cc:32-43); rigid-molecule distances are far below L_perp/2
@@ -133,6 +133,7 @@ __global__ void scatter_kernel(const uint32_t* key, uint64_t n, uint32_t* cursor
 // over blocks gives each (block, cube) its offset among the cube's trials; the block then places
 // its trials, ranked inside (block, cube) by LDS atomics.
 constexpr int kSortThreads = 1024;
+constexpr uint64_t kSplitWaves = 6144;
 constexpr uint32_t kSortMaxCells = 12288;  // 48 KB of LDS counters
 
 __global__ __launch_bounds__(kSortThreads) void cube_hist_kernel(const WidomArgs a, const CellArgs c, uint64_t per,
@@ -332,14 +333,14 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
         }
     }
     const unsigned pair_blocks = (unsigned)((req.n_trials + kWThreads - 1) / kWThreads);
-    // split kernel: the fewest threads per trial (1 to 16) that give at least 2048 workgroups
-    // (8 per CU), so that small batches still fill the chip
-    const uint64_t split_tpt = (uint64_t)split_trials_per_thread(ns);
+    // split kernel: the fewest threads per trial (1 to 16) that give at least kSplitWaves waves
+    // (6 per SIMD), so that small batches still fill the chip; 64 trials per wave group
+    const uint64_t tpl = (uint64_t)split_trials_per_lane(ns);
     int split = 1;
-    while (split < 16 && (req.n_trials * split + kWThreads * split_tpt - 1) / (kWThreads * split_tpt) < 2048) split *= 2;
+    while (split < 16 && req.n_trials * (uint64_t)split < 64ull * tpl * kSplitWaves) split *= 2;
     a.split = split;
-    const unsigned split_blocks =
-        (unsigned)((req.n_trials * split + kWThreads * split_tpt - 1) / (kWThreads * split_tpt));
+    const uint64_t split_trials_per_block = (uint64_t)split_block_threads(split) / split * tpl;
+    const unsigned split_blocks = (unsigned)((req.n_trials + split_trials_per_block - 1) / split_trials_per_block);
     const int rec_rt = charged ? runs_molecules_per_block(ns, fw.kmax) : 1;
     if (rec_rt == 0) fail(ALEATOR_NOT_IMPLEMENTED, "widom: phase tables exceed the LDS budget");
     const unsigned rec_blocks = (unsigned)((req.n_trials + rec_rt - 1) / rec_rt);
@@ -365,6 +366,14 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
         fw.widom_tab_species = species_key;
     }
     a.ljtab = fw.d_widom_tab;
+    // the split kernel of an uncharged batch adds up its own workgroup sums (last workgroup done)
+    const bool fused_sum = !charged && !n_cells;
+    if (fused_sum && !fw.d_widom_ticket) {
+        ALEATOR_HIP_CHECK(hipMalloc(&fw.d_widom_ticket, sizeof(unsigned)));
+        ALEATOR_HIP_CHECK(hipMemsetAsync(fw.d_widom_ticket, 0, sizeof(unsigned), ctx.stream));
+    }
+    a.ticket = fused_sum ? fw.d_widom_ticket : nullptr;
+    a.sum_out = req.sum_device;
     a.block_sums = reinterpret_cast<double*>(base + tab_bytes);
     a.du = req.du_device ? req.du_device
                          : reinterpret_cast<double*>(base + tab_bytes + parts * sizeof(double));
@@ -422,8 +431,9 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
         if (charged) launch_recip(a, fw, rec_blocks, rec_rt, ctx.stream);
     }
     ctx.time_end();
-    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, ctx.stream, a.block_sums, (uint64_t)parts,
-                       req.sum_device);
+    if (!fused_sum)
+        hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, ctx.stream, a.block_sums, (uint64_t)parts,
+                           req.sum_device);
     ALEATOR_HIP_CHECK(hipGetLastError());
 }
 

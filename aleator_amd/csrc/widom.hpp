@@ -16,6 +16,7 @@
 // kernel polynomial of degree DEG (0: uncharged).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -60,6 +61,8 @@ struct WidomArgs {
     double* du;           // [n_trials] (LJ + real in the direct-sum kernel when charged)
     double* block_sums;   // per workgroup sum of exp(-dU/T)
     int split;            // direct-sum kernel: threads sharing one trial's atoms (1, 2, 4, 8 or 16)
+    unsigned* ticket;     // split kernel, uncharged: workgroups finished (0 between calls), or null
+    double* sum_out;      // split kernel with ticket: the batch's sum of exp(-dU/T)
     double poly[kErfPolyMax + 1];  // erf(alpha r)/r in x = r^2 poly_xs - 1 (pair.hpp)
     double poly_xs;
     double tail_delta;  // analytic LJ tail of one insertion (0 unless the framework asks for it)
@@ -74,7 +77,8 @@ constexpr int kGeoTriclinic = 0, kGeoOrtho = 1, kGeoCubic = 2;
 
 // Reference point and rotated sites of trial t, Cartesian, in the reference's
 // arithmetic order (no contraction), and their fractional coordinates.
-__device__ inline void trial_sites(const WidomArgs& a, uint64_t t, double cart[kMaxSites][3]) {
+template <int NS>
+__device__ inline void trial_sites(const WidomArgs& a, uint64_t t, double (&cart)[NS][3]) {
 #pragma clang fp contract(off)
     const uint64_t call = a.first + t * (uint64_t)a.draws;
     const double u0 = philox_uniform(a.seed, a.stream, call);
@@ -82,7 +86,7 @@ __device__ inline void trial_sites(const WidomArgs& a, uint64_t t, double cart[k
     const double u2 = philox_uniform(a.seed, a.stream, call + 2);
     double ref[3];
     for (int j = 0; j < 3; ++j) ref[j] = a.m[j] * u0 + a.m[3 + j] * u1 + a.m[6 + j] * u2;
-    if (a.nsites == 1) {
+    if constexpr (NS == 1) {
         for (int j = 0; j < 3; ++j) cart[0][j] = ref[j] + a.local[0][j];
         return;
     }
@@ -92,7 +96,8 @@ __device__ inline void trial_sites(const WidomArgs& a, uint64_t t, double cart[k
     const double r1 = sqrt(1.0 - v0), r2 = sqrt(v0);
     const double th1 = 2.0 * kPi * v1, th2 = 2.0 * kPi * v2;
     const double qw = r2 * cos(th2), qx = r1 * sin(th1), qy = r1 * cos(th1), qz = r2 * sin(th2);
-    for (int s = 0; s < a.nsites; ++s) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
         const double vx = a.local[s][0], vy = a.local[s][1], vz = a.local[s][2];
         // t = 2 (qv x v); v' = v + w t + qv x t
         const double tx = (qy * vz - qz * vy) * 2.0;
@@ -186,93 +191,110 @@ __device__ inline void widom_pair(double r2, const double4& cg, double qq, doubl
     }
 }
 
-// Trials per thread of the split kernel: each staged atom read from LDS is paired with this many
-// trials held in registers.  One: at 1e5 CH4 trials in c0, 2 and 4 per thread measured 5 % and
-// 9 % slower (the fewer, longer workgroups leave a larger partly filled last round on the chip).
-constexpr int split_trials_per_thread(int) { return 1; }
+// Direct sum over every framework atom (small and mid-size batches, or no usable cube grid).  A
+// workgroup of 64 * split * G threads (G = max(1, 4 / split)) takes 64 G trials: wave w holds
+// trial group w / split and part w % split, lane l trial l of its group, so the loop over atoms is
+// uniform across a wave (part p takes atoms p, p + split, ... of each staged chunk, read from LDS
+// by every lane at once).  The parts of a trial meet in LDS in a fixed order.  Atoms are staged
+// through LDS in chunks of 256, with each site's LJ constants for the atom's type beside them (no
+// per-pair type lookup).  Each pair takes the nearest image as a.geo says (exact for pairs within
+// a cutoff <= L_perp/2).  Every part draws its trials' sites itself.  Uncharged trials in a cubic cell work in fractional units: r_f^2 = |f|^2
+// is compared with rc^2 / L^2 and the LJ constants carry the powers of L (A / L^12, B / L^6, the
+// slope times L).  With a.ticket set (uncharged batches), the last workgroup to finish also adds
+// the per-workgroup sums of exp(-dU/T) in workgroup order into *a.sum_out and clears the ticket.
+// Trials per lane of the split kernel: two for one-site molecules (two independent dependency
+// chains per atom read), one otherwise (the sites already give that).
+template <int NS>
+constexpr int split_tpt() { return NS == 1 ? 2 : 1; }
 
-// Direct sum over every framework atom (small batches, or no usable cube grid).  The 256 threads
-// of a workgroup form 256/split groups of `split` threads, each group taking TPT trials; the split
-// threads of a group share the atoms (atom j of a staged chunk to thread j mod split) and their
-// partial sums meet in LDS in a fixed order, so a batch of 1e5 trials still fills the chip.  Atoms
-// are staged through LDS in chunks of 256, with each site's LJ constants for the atom's type
-// beside them (no per-pair type lookup); each atom read from LDS is paired with the thread's TPT
-// trials.  Each pair takes the nearest image as a.geo says (exact for pairs within a cutoff <=
-// L_perp/2).
 template <bool EXACT, int NS, int DEG, int LJK>
-__global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs a) {
+__global__ __launch_bounds__(1024) void widom_split_kernel(const WidomArgs a) {
     constexpr int NR = EXACT ? 2 : 1;
     constexpr bool CH = DEG > 0;
-    constexpr int TPT = split_trials_per_thread(NS);
-    __shared__ double sfx[kWThreads], sfy[kWThreads], sfz[kWThreads], sq[kWThreads];
-    __shared__ double2 sab[NS][kWThreads];                    // A, B of (site, atom)
-    __shared__ double2 scd[LJK > 0 ? NS : 1][kWThreads];      // shift, slope of (site, atom)
+    constexpr int TPT = split_tpt<NS>();
+    constexpr int kStage = 512;  // atoms staged per pass (one pass for up to 512 framework atoms)
+    // one LDS area holds, in turn, the staged atoms and the parts' partial sums
+    constexpr size_t kStaged = sizeof(double) * kStage * (4 + 2 * NS + (LJK > 0 ? 2 * NS : 0));
+    constexpr size_t kParts = sizeof(double) * 1024 * TPT * (CH ? 2 : 1);
+    constexpr size_t kArea = kStaged > kParts ? kStaged : kParts;
+    __shared__ __align__(16) unsigned char s_area[kArea];
+    double* sfx = reinterpret_cast<double*>(s_area);
+    double* sfy = sfx + kStage;
+    double* sfz = sfy + kStage;
+    double* sq = sfz + kStage;
+    auto* sab = reinterpret_cast<double2(*)[kStage]>(sq + kStage);  // [NS]: A, B of (site, atom)
+    auto* scd = sab + NS;                                             // [NS]: shift, slope (LJK > 0)
+    double* part_lj = reinterpret_cast<double*>(s_area);              // [TPT][1024]
+    double* part_re = part_lj + 1024 * TPT;
     __shared__ double4 tab[NS][kMaxTypes];
     __shared__ double s_poly[DEG + 1];
-    __shared__ double part_lj[TPT][kWThreads], part_re[CH ? TPT : 1][kWThreads];
-    __shared__ double red[kWThreads / 64];
+    __shared__ double red[16];
+    __shared__ int s_last;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int geo = a.geo;
+    const bool frac_units = !CH && geo == kGeoCubic;
+    // the first pass's atoms, loaded before anything waits on memory
+    auto load_atom = [&](int at, double& x, double& y, double& z, double& q, int& ty) {
+        const double sc0 = geo == kGeoOrtho ? a.box[0] : 1.0;
+        const double sc1 = geo == kGeoOrtho ? a.box[1] : 1.0;
+        const double sc2 = geo == kGeoOrtho ? a.box[2] : 1.0;
+        x = a.fx[at] * sc0;
+        y = a.fy[at] * sc1;
+        z = a.fz[at] * sc2;
+        q = a.q[at];
+        ty = a.type[at];
+    };
+    double ax0 = 0.0, ay0 = 0.0, az0 = 0.0, aq0 = 0.0;
+    int aty0 = 0;
+    const bool stager = tid < kStage;
+    if (stager && tid < a.natoms) load_atom(tid, ax0, ay0, az0, aq0, aty0);
     stage_constants<NS, DEG>(a, tab, s_poly);
-    __syncthreads();
+    const int split = a.split;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int part = wave % split, group = wave / split;
+    const int groups = blockDim.x / (64 * split);
+    // slot k of a lane: trial (group * TPT + k) * 64 + lane of this workgroup
+    auto local = [&](int k) { return (group * TPT + k) * 64 + lane; };
+    auto trial = [&](int k) { return (uint64_t)blockIdx.x * (groups * 64 * TPT) + local(k); };
+    // trial sites (fractional, or Cartesian for orthorhombic cells), drawn by every part of the
+    // trial's group on its own (no exchange, no barrier)
+    double fs[TPT][NS][3];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        double cart[NS][3];
+        trial_sites<NS>(a, trial(k) < a.n_trials ? trial(k) : 0, cart);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (geo == kGeoOrtho) {
+                fs[k][s][0] = cart[s][0];
+                fs[k][s][1] = cart[s][1];
+                fs[k][s][2] = cart[s][2];
+            } else {
+                to_frac(a, cart[s], fs[k][s]);
+            }
+        }
+    }
+    __syncthreads();  // tab
     double pc[DEG + 1];
 #pragma unroll
     for (int k = 0; k <= DEG; ++k) pc[k] = s_poly[k];
     const double pxs = a.poly_xs;
-    const int split = a.split, tpb = kWThreads / split;
-    const int j = threadIdx.x % tpb, part = threadIdx.x / tpb;
-    // trial of slot k of this group
-    auto trial = [&](int k) { return ((uint64_t)blockIdx.x * TPT + k) * tpb + j; };
-    const int geo = a.geo;
-    // the group's trial sites (fractional, or Cartesian for orthorhombic cells): drawn by its
-    // first thread and shared through LDS
-    __shared__ double s_fs[kWThreads / 2][TPT][NS][3];
-    double fs[TPT][NS][3];
     double qs[NS];
-    auto draw = [&](int k, double (&out)[NS][3]) {
-        double cart[kMaxSites][3];
-        trial_sites(a, trial(k), cart);
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            if (geo == kGeoOrtho) {
-                out[s][0] = cart[s][0];
-                out[s][1] = cart[s][1];
-                out[s][2] = cart[s][2];
-            } else {
-                to_frac(a, cart[s], out[s]);
-            }
-        }
-    };
 #pragma unroll
     for (int s = 0; s < NS; ++s) qs[s] = a.charge[s];
-#pragma unroll
-    for (int k = 0; k < TPT; ++k)
-#pragma unroll
-        for (int s = 0; s < NS; ++s) fs[k][s][0] = fs[k][s][1] = fs[k][s][2] = 0.0;
-    const bool any_valid = trial(0) < a.n_trials;
-    if (split == 1) {
-#pragma unroll
-        for (int k = 0; k < TPT; ++k)
-            if (trial(k) < a.n_trials) draw(k, fs[k]);
-    } else {
-        if (part == 0)
-#pragma unroll
-            for (int k = 0; k < TPT; ++k)
-                if (trial(k) < a.n_trials) draw(k, s_fs[j][k]);
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < TPT; ++k)
-            if (trial(k) < a.n_trials)
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    for (int c = 0; c < 3; ++c) fs[k][s][c] = s_fs[j][k][s][c];
-    }
     double lj[TPT], real[TPT];
 #pragma unroll
     for (int k = 0; k < TPT; ++k) lj[k] = real[k] = 0.0;
-    const double rc2 = a.rc2_lj, rcr2 = a.rc2_real;
-    const bool one = !CH || rc2 == rcr2;
     const double l2 = a.box[0] * a.box[0];  // cubic cells
-    // the pairs of staged atoms part, part + split, ... < cnt with the TPT trials, minimum image
-    // by geometry G
+    const double il2 = 1.0 / l2;
+    const double rc2 = frac_units ? a.rc2_lj * il2 : a.rc2_lj, rcr2 = a.rc2_real;
+    const bool one = !CH || a.rc2_lj == rcr2;
+    // LJ constants in the units of r^2: A L^-12, B L^-6, slope L in fractional units of a cubic cell
+    const double sa = frac_units ? il2 * il2 * il2 * il2 * il2 * il2 : 1.0;
+    const double sb = frac_units ? il2 * il2 * il2 : 1.0;
+    const double sl = frac_units ? a.box[0] : 1.0;
+    // the pairs of staged atoms part, part + split, ... < cnt with this lane's trials, minimum
+    // image by geometry G
     auto pairs = [&](auto geo_tag, auto one_tag, int cnt) {
         constexpr int G = decltype(geo_tag)::value;
         constexpr bool ONE = decltype(one_tag)::value;
@@ -299,7 +321,8 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
                         d1 -= rint(d1);
                         d2 -= rint(d2);
                         if constexpr (G == kGeoCubic) {
-                            r2 = l2 * fma(d2, d2, fma(d1, d1, d0 * d0));
+                            r2 = fma(d2, d2, fma(d1, d1, d0 * d0));
+                            if constexpr (CH) r2 *= l2;
                         } else {
                             const double dx = d0 * a.m[0] + d1 * a.m[3] + d2 * a.m[6];
                             const double dy = d0 * a.m[1] + d1 * a.m[4] + d2 * a.m[7];
@@ -312,28 +335,26 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
             }
         }
     };
-    for (int base = 0; base < a.natoms; base += kWThreads) {
-        __syncthreads();
-        const int at = base + threadIdx.x;
-        if (at < a.natoms) {
-            const double sc0 = geo == kGeoOrtho ? a.box[0] : 1.0;
-            const double sc1 = geo == kGeoOrtho ? a.box[1] : 1.0;
-            const double sc2 = geo == kGeoOrtho ? a.box[2] : 1.0;
-            sfx[threadIdx.x] = a.fx[at] * sc0;
-            sfy[threadIdx.x] = a.fy[at] * sc1;
-            sfz[threadIdx.x] = a.fz[at] * sc2;
-            sq[threadIdx.x] = a.q[at];
-            const int ty = a.type[at];
+    for (int base = 0; base < a.natoms; base += kStage) {
+        const int at = base + tid;
+        if (base > 0) {
+            __syncthreads();  // the previous pass's atoms are no longer read
+            if (stager && at < a.natoms) load_atom(at, ax0, ay0, az0, aq0, aty0);
+        }
+        if (stager && at < a.natoms) {
+            sfx[tid] = ax0;
+            sfy[tid] = ay0;
+            sfz[tid] = az0;
+            sq[tid] = aq0;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                const double4 c = tab[s][ty];
-                sab[s][threadIdx.x] = make_double2(c.x, c.y);
-                if constexpr (LJK > 0) scd[s][threadIdx.x] = make_double2(c.z, c.w);
+                const double4 c = tab[s][aty0];
+                sab[s][tid] = make_double2(c.x * sa, c.y * sb);
+                if constexpr (LJK > 0) scd[s][tid] = make_double2(c.z, c.w * sl);
             }
         }
         __syncthreads();
-        const int cnt = min(kWThreads, a.natoms - base);
-        if (!any_valid) continue;
+        const int cnt = min(kStage, a.natoms - base);
         if (geo == kGeoCubic) {
             if (one) pairs(std::integral_constant<int, kGeoCubic>{}, std::true_type{}, cnt);
             else pairs(std::integral_constant<int, kGeoCubic>{}, std::false_type{}, cnt);
@@ -345,31 +366,52 @@ __global__ __launch_bounds__(kWThreads) void widom_split_kernel(const WidomArgs 
             else pairs(std::integral_constant<int, kGeoTriclinic>{}, std::false_type{}, cnt);
         }
     }
+    __syncthreads();  // the staged atoms are no longer read
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
-        part_lj[k][threadIdx.x] = lj[k];
-        if constexpr (CH) part_re[k][threadIdx.x] = real[k];
+        part_lj[k * 1024 + tid] = lj[k];
+        if constexpr (CH) part_re[k * 1024 + tid] = real[k];
     }
     __syncthreads();
     double boltz = 0.0;
-    if (part == 0) {
+    if (part == 0)
 #pragma unroll
         for (int k = 0; k < TPT; ++k) {
-            const uint64_t t = trial(k);
-            if (t >= a.n_trials) continue;
-            double sl = 0.0, sr = 0.0;
+            if (trial(k) >= a.n_trials) continue;
+            double sl_ = 0.0, sr = 0.0;
             for (int p = 0; p < split; ++p) {
-                sl += part_lj[k][j + p * tpb];
-                if constexpr (CH) sr += part_re[k][j + p * tpb];
+                const int i = k * 1024 + (group * split + p) * 64 + lane;
+                sl_ += part_lj[i];
+                if constexpr (CH) sr += part_re[i];
             }
-            const double du = (CH ? sl + a.kc * sr : sl) + a.tail_delta;
-            a.du[t] = du;
+            const double du = (CH ? sl_ + a.kc * sr : sl_) + a.tail_delta;
+            a.du[trial(k)] = du;
             if (!CH) boltz += exp(-du / a.temperature);
         }
-    }
     if constexpr (!CH) {
-        const double s = block_sum(boltz, red);
-        if (threadIdx.x == 0) a.block_sums[blockIdx.x] = s;
+        const double bs = block_sum(boltz, red);
+        if (tid == 0) {
+            a.block_sums[blockIdx.x] = bs;
+            if (a.ticket) {
+                __threadfence();
+                s_last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
+            }
+        }
+        if (a.ticket) {
+            __syncthreads();
+            if (s_last) {
+                // every other workgroup's sum is visible (written before its ticket)
+                __threadfence();
+                double v = 0.0;
+                for (unsigned b = tid; b < gridDim.x; b += blockDim.x)
+                    v += __hip_atomic_load(a.block_sums + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                v = block_sum(v, red);
+                if (tid == 0) {
+                    *a.sum_out = v;
+                    *a.ticket = 0u;
+                }
+            }
+        }
     }
 }
 
@@ -426,8 +468,8 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
             tr[k] = ok[k] ? c.order[idx] : 0;
             lj[k] = 0.0;
             real[k] = 0.0;
-            double cart[kMaxSites][3];
-            if (ok[k]) trial_sites(a, tr[k], cart);
+            double cart[NS][3];
+            if (ok[k]) trial_sites<NS>(a, tr[k], cart);
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 ox[k][s] = ok[k] ? cart[s][0] - cc[0] : 1e6;
@@ -548,12 +590,18 @@ void launch_cells_deg(const WidomArgs& a, const CellArgs& c, unsigned n_cells, b
     fail(ALEATOR_INVALID_ARGUMENT, "widom: 1 to 5 sites supported");
 }
 
+// threads of a split-kernel workgroup: 64 lanes per group, split waves per group, at least 4 waves
+inline int split_block_threads(int split) { return 64 * split * std::max(1, 4 / split); }
+// trials per lane of the split kernel (split_tpt) for n sites
+inline int split_trials_per_lane(int ns) { return ns == 1 ? 2 : 1; }
+
 template <bool EXACT, int DEG, int LJK>
 void launch_split_deg(const WidomArgs& a, unsigned blocks, hipStream_t s) {
+    const unsigned threads = (unsigned)split_block_threads(a.split);
     switch (a.nsites) {
 #define ALEATOR_WSPLIT(N)                                                                                         \
     case N:                                                                                                       \
-        hipLaunchKernelGGL((widom_split_kernel<EXACT, N, DEG, LJK>), dim3(blocks), dim3(kWThreads), 0, s, a);    \
+        hipLaunchKernelGGL((widom_split_kernel<EXACT, N, DEG, LJK>), dim3(blocks), dim3(threads), 0, s, a);      \
         return;
         ALEATOR_WSPLIT(1) ALEATOR_WSPLIT(2) ALEATOR_WSPLIT(3) ALEATOR_WSPLIT(4) ALEATOR_WSPLIT(5)
 #undef ALEATOR_WSPLIT

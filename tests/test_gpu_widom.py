@@ -172,14 +172,24 @@ def test_widom_all_atom_path_for_large_molecules(ctx):
 
 def test_widom_split_kernel_c0_1e5(ctx):
     """c0's configured batch, 1e5 CH4 trials: the cube grid is rejected (the list sphere would
-    cover more than 70 % of the cell), so the direct kernel runs with 8 threads per trial; its
-    first 300 trials equal the oracle's in FP64 and meet the MIXED bar."""
+    cover more than 70 % of the cell), so the split kernel runs (two trials per lane, each trial's
+    atoms split over 8 waves, fractional units in the cubic cell, the batch sum added by the last
+    workgroup); its first 300 trials equal the oracle's in FP64 and meet the MIXED bar, and K_H
+    equals the oracle's over those 300."""
     print("widom c0 1e5 trials, split kernel", flush=True)
     fw, lj = ol.c0_system()
     dev = device_framework(ctx, fw, lj)
-    _, du_ref = ol.widom(lj, fw, CH4, seed=1, stream=1, n_trials=300, temperature=298.0)
-    _, du = dev.widom(CH4, 100_000, 298.0, seed=1, stream=1, precision=aa.FP64)
+    kh_ref, du_ref = ol.widom(lj, fw, CH4, seed=1, stream=1, n_trials=300, temperature=298.0)
+    s300, _ = dev.widom(CH4, 300, 298.0, seed=1, stream=1, precision=aa.FP64)
+    assert abs(s300 / 300 / 298.0 - kh_ref) <= 1e-10 * abs(kh_ref)
+    s, du = dev.widom(CH4, 100_000, 298.0, seed=1, stream=1, precision=aa.FP64)
     assert rel_err(du[:300], du_ref).max() <= 1e-10
+    # the batch sum that the last workgroup adds up, against the per-trial energies (K_H =
+    # <exp(-dU/T)> / T, monte_carlo_engine.cc:880-916), and again on a second call
+    s_host = np.sum(np.exp(-du / 298.0))
+    assert abs(s - s_host) <= 1e-12 * abs(s_host)
+    s_again, _ = dev.widom(CH4, 100_000, 298.0, seed=1, stream=1, precision=aa.FP64)
+    assert s_again == s
     _, du32 = dev.widom(CH4, 100_000, 298.0, seed=1, stream=1, precision=aa.MIXED)
     mixed_bar("c0 CH4 widom 1e5 (split)", du32[:300], du_ref)
     mixed_bar("c0 CH4 widom 1e5 (split, all vs device FP64)", du32, du)
