@@ -744,8 +744,10 @@ void Mlp::iteration(const double* frac, const double* target, uint64_t n_samples
 }
 
 void Mlp::drop_graph() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    graph_exec = nullptr;
+    for (auto& g : graph_exec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 void Mlp::train(const double* frac, const double* target, uint64_t n_samples, int batch, int n_iters, float lr,
@@ -753,26 +755,34 @@ void Mlp::train(const double* frac, const double* target, uint64_t n_samples, in
     hipStream_t s = ctx->stream;
     ALEATOR_HIP_CHECK(hipMemsetAsync(counters, 0, sizeof(uint64_t), s));  // iteration 0 of this call
     const GraphKey key{frac, target, n_samples, batch, lr, seed, stream};
-    if (use_graph && n_iters > 1 && !(graph_exec && key == graph_key)) {
+    if (use_graph && n_iters > 1 && !(graph_exec[0] && key == graph_key)) {
         drop_graph();
-        hipGraph_t graph = nullptr;
-        ALEATOR_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        iteration(frac, target, n_samples, batch, lr, seed, stream);
-        const hipError_t e = hipStreamEndCapture(s, &graph);
-        ALEATOR_HIP_CHECK(e);
-        const hipError_t ei = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        ALEATOR_HIP_CHECK(ei);
+        for (int g = 0; g < 2; ++g) {
+            hipGraph_t graph = nullptr;
+            ALEATOR_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < (g ? kGraphIters : 1); ++i) iteration(frac, target, n_samples, batch, lr, seed, stream);
+            const hipError_t e = hipStreamEndCapture(s, &graph);
+            ALEATOR_HIP_CHECK(e);
+            const hipError_t ei = hipGraphInstantiate(&graph_exec[g], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            ALEATOR_HIP_CHECK(ei);
+        }
         graph_key = key;
     }
     const bool replay = use_graph && n_iters > 1;
     for (int it0 = 0; it0 < n_iters; it0 += kLossChunk) {
         const int cnt = std::min(kLossChunk, n_iters - it0);
-        for (int i = 0; i < cnt; ++i) {
-            if (replay)
-                ALEATOR_HIP_CHECK(hipGraphLaunch(graph_exec, s));
-            else
-                iteration(frac, target, n_samples, batch, lr, seed, stream);
+        for (int i = 0; i < cnt;) {
+            if (replay && cnt - i >= kGraphIters) {
+                ALEATOR_HIP_CHECK(hipGraphLaunch(graph_exec[1], s));
+                i += kGraphIters;
+            } else {
+                if (replay)
+                    ALEATOR_HIP_CHECK(hipGraphLaunch(graph_exec[0], s));
+                else
+                    iteration(frac, target, n_samples, batch, lr, seed, stream);
+                ++i;
+            }
         }
         ALEATOR_HIP_CHECK(hipGetLastError());
         if (loss_trace)

@@ -66,7 +66,10 @@ struct Mlp {
         bool operator==(const GraphKey&) const = default;
     };
     bool use_graph = true;
-    hipGraphExec_t graph_exec = nullptr;
+    // captured iterations: graph_exec[0] holds one, graph_exec[1] kGraphIters in a row (fewer graph
+    // launches per step)
+    static constexpr int kGraphIters = 8;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     GraphKey graph_key{};
 
     Mlp(Context* c, int max_rows);

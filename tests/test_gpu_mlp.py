@@ -101,7 +101,8 @@ def test_training_fits_a_smooth_periodic_target(ctx):
 
 
 def test_graph_replay_equals_eager_launches(ctx):
-    """Training from a replayed hipGraph gives the eager launches' results bit for bit, over two
+    """Training from replayed hipGraphs (eight iterations per replay, single ones for the rest:
+    25 = 3 x 8 + 1 and 30 = 3 x 8 + 6) gives the eager launches' results bit for bit, over two
     train calls with other device work (a prediction) between them."""
     print("mlp training: graph replay vs eager launches", flush=True)
     frac, target = rows(5000, seed=12)
