@@ -8,7 +8,7 @@
 namespace aleator_gpu {
 
 constexpr int kNumTileShapes = 4;
-constexpr int kFarDrop = 6;  // the shell entries' erf polynomial is this many degrees lower  // 8x8x4, 8x8x8 (2/thread), 8x8x16 (4/thread), 4x8x16 (2/thread)
+constexpr int kFarDrop = 7;  // the shell entries' erf polynomial is this many degrees lower
 
 // Everything one framework's tile kernel reads; stored in device memory, one per framework.
 struct GridArgs {
@@ -41,7 +41,7 @@ struct GridArgs {
     double poly[kErfPolyMax + 1];     // erf(alpha r)/r in x = r^2 poly_xs - 1 (pair.hpp)
     double poly_xs;
     // the same kernel fitted over the distances the shell entries can have (r >= sqrt(inner2) -
-    // tile radius), six degrees lower, in x = r^2 poly_far_xs + poly_far_xo; used when far_ok
+    // tile radius), kFarDrop degrees lower, in x = r^2 poly_far_xs + poly_far_xo; used when far_ok
     double poly_far[kErfPolyMax + 1];
     double poly_far_xs, poly_far_xo;
     int far_ok;

@@ -715,7 +715,7 @@ def main():
         du = torch.empty(10_000_000, dtype=torch.float64, device="cuda")
         sm = torch.empty(1, dtype=torch.float64, device="cuda")
         ch4 = [(0.0, 0.0, 0.0, 4, 0.0)]
-        for n_trials in (100_000, 10_000_000):
+        for n_trials in (100_000, 1_000_000, 10_000_000):
             for p, name in ((aa.MIXED, "mixed"), (aa.FP64, "fp64")):
                 t = timed(lambda: fw0.widom_device(ch4, n_trials, 298.0, du.data_ptr(), sm.data_ptr(), seed=1, stream=1,
                                                    precision=p), 5, 1, sync)
@@ -837,7 +837,7 @@ def main():
         extra["speedup_vs_cpu_baseline_1_thread"] = value / extra["cpu_baseline_1_thread"]["value"]
         extra["speedup_vs_cpu_baseline"] = value / cpu["value"]
         if "c0" in legs:
-            for n_trials in (100_000, 10_000_000):
+            for n_trials in (100_000, 1_000_000, 10_000_000):
                 extra[f"widom_c0_{n_trials}_speedup_vs_cpu_all_threads"] = (
                     extra[f"c0_widom_ch4_{n_trials}_mixed_inserts_per_s"] /
                     extra[f"cpu_oracle_widom_c0_ch4_{th}_thread_inserts_per_s"])
