@@ -191,22 +191,24 @@ __device__ inline void widom_pair(double r2, const double4& cg, double qq, doubl
     }
 }
 
-// Direct sum over every framework atom (small and mid-size batches, or no usable cube grid).  A
-// workgroup of 64 * split * G threads (G = max(1, 4 / split)) takes 64 G trials: wave w holds
-// trial group w / split and part w % split, lane l trial l of its group, so the loop over atoms is
-// uniform across a wave (part p takes atoms p, p + split, ... of each staged chunk, read from LDS
-// by every lane at once).  The parts of a trial meet in LDS in a fixed order.  Atoms are staged
-// through LDS in chunks of 256, with each site's LJ constants for the atom's type beside them (no
-// per-pair type lookup).  Each pair takes the nearest image as a.geo says (exact for pairs within
-// a cutoff <= L_perp/2).  Every part draws its trials' sites itself.  Uncharged trials in a cubic cell work in fractional units: r_f^2 = |f|^2
-// is compared with rc^2 / L^2 and the LJ constants carry the powers of L (A / L^12, B / L^6, the
-// slope times L).  With a.ticket set (uncharged batches), the last workgroup to finish also adds
-// the per-workgroup sums of exp(-dU/T) in workgroup order into *a.sum_out and clears the ticket.
 // Trials per lane of the split kernel: two for one-site molecules (two independent dependency
 // chains per atom read), one otherwise (the sites already give that).
 template <int NS>
 constexpr int split_tpt() { return NS == 1 ? 2 : 1; }
 
+// Direct sum over every framework atom (small and mid-size batches, or no usable cube grid).  A
+// workgroup of 64 * split * G threads (G = max(1, 4 / split)) takes 64 G TPT trials: wave w holds
+// trial group w / split and part w % split, lane l the TPT trials l, l + 64, ... of its group, so
+// the loop over atoms is uniform across a wave (part p takes atoms p, p + split, ... of each staged
+// pass, read from LDS by every lane at once).  The parts of a trial meet in LDS in a fixed order.
+// Atoms are staged through LDS 512 at a time, with each site's LJ constants for the atom's type
+// beside them (no per-pair type lookup).  Each pair takes the nearest image as a.geo says (exact
+// for pairs within a cutoff <= L_perp/2).  Slot k of a trial group is drawn by part k mod split
+// and shared through LDS.  Uncharged trials in a cubic cell work in fractional units: r_f^2 =
+// |f|^2 is compared with rc^2 / L^2 and the LJ constants carry the powers of L (A / L^12, B / L^6,
+// the slope times L).  With a.ticket set (uncharged batches), the last workgroup to finish also
+// adds the per-workgroup sums of exp(-dU/T) in workgroup order into *a.sum_out and clears the
+// ticket.
 template <bool EXACT, int NS, int DEG, int LJK>
 __global__ __launch_bounds__(1024) void widom_split_kernel(const WidomArgs a) {
     constexpr int NR = EXACT ? 2 : 1;
@@ -256,25 +258,37 @@ __global__ __launch_bounds__(1024) void widom_split_kernel(const WidomArgs a) {
     // slot k of a lane: trial (group * TPT + k) * 64 + lane of this workgroup
     auto local = [&](int k) { return (group * TPT + k) * 64 + lane; };
     auto trial = [&](int k) { return (uint64_t)blockIdx.x * (groups * 64 * TPT) + local(k); };
-    // trial sites (fractional, or Cartesian for orthorhombic cells), drawn by every part of the
-    // trial's group on its own (no exchange, no barrier)
+    // trial sites (fractional, or Cartesian for orthorhombic cells): with one part a lane draws its
+    // own; with more, slot k of a group is drawn once, by the group's part k mod split, and shared
+    // through LDS (at most two groups of 64 x TPT slots per workgroup then)
+    __shared__ double s_fs[128 * TPT][NS][3];
     double fs[TPT][NS][3];
-#pragma unroll
-    for (int k = 0; k < TPT; ++k) {
+    auto draw = [&](int k, double (&out)[NS][3]) {
         double cart[NS][3];
         trial_sites<NS>(a, trial(k) < a.n_trials ? trial(k) : 0, cart);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             if (geo == kGeoOrtho) {
-                fs[k][s][0] = cart[s][0];
-                fs[k][s][1] = cart[s][1];
-                fs[k][s][2] = cart[s][2];
+                out[s][0] = cart[s][0];
+                out[s][1] = cart[s][1];
+                out[s][2] = cart[s][2];
             } else {
-                to_frac(a, cart[s], fs[k][s]);
+                to_frac(a, cart[s], out[s]);
             }
         }
+    };
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        if (split == 1) draw(k, fs[k]);
+        else if (k % split == part) draw(k, s_fs[local(k)]);
     }
-    __syncthreads();  // tab
+    __syncthreads();  // tab, s_fs
+    if (split > 1)
+#pragma unroll
+        for (int k = 0; k < TPT; ++k)
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                for (int c = 0; c < 3; ++c) fs[k][s][c] = s_fs[local(k)][s][c];
     double pc[DEG + 1];
 #pragma unroll
     for (int k = 0; k <= DEG; ++k) pc[k] = s_poly[k];
