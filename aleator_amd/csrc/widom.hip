@@ -129,48 +129,65 @@ __global__ void scatter_kernel(const uint32_t* key, uint64_t n, uint32_t* cursor
 }
 
 // Counting sort of the trials by cube with no global atomics (cube counts up to kSortMaxCells):
-// block b takes trials [b per, (b + 1) per) and histograms their cubes in LDS; a column scan
-// over blocks gives each (block, cube) its offset among the cube's trials; the block then places
-// its trials, ranked inside (block, cube) by LDS atomics.
+// block b takes trials [b per, (b + 1) per), keeps each trial's cube in key[] and histograms the
+// cubes in LDS; a column scan over blocks gives each (block, cube) its offset among the cube's
+// trials; the block then places its trials, ranked inside (block, cube) by LDS atomics.
 constexpr int kSortThreads = 1024;
 constexpr uint64_t kSplitWaves = 6144;
 constexpr uint32_t kSortMaxCells = 12288;  // 48 KB of LDS counters
+constexpr int kColScanCells = 16;          // cubes per column-scan workgroup
 
 __global__ __launch_bounds__(kSortThreads) void cube_hist_kernel(const WidomArgs a, const CellArgs c, uint64_t per,
-                                                                 uint32_t n_cells, uint32_t* hist) {
+                                                                 uint32_t n_cells, uint32_t* hist, uint32_t* key) {
     extern __shared__ uint32_t hcount[];
     for (uint32_t i = threadIdx.x; i < n_cells; i += kSortThreads) hcount[i] = 0u;
     __syncthreads();
     const uint64_t lo = (uint64_t)blockIdx.x * per, hi = min(lo + per, a.n_trials);
-    for (uint64_t t = lo + threadIdx.x; t < hi; t += kSortThreads) atomicAdd(&hcount[trial_cell(a, c, t)], 1u);
+    for (uint64_t t = lo + threadIdx.x; t < hi; t += kSortThreads) {
+        const uint32_t k = trial_cell(a, c, t);
+        key[t] = k;
+        atomicAdd(&hcount[k], 1u);
+    }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n_cells; i += kSortThreads) hist[(size_t)blockIdx.x * n_cells + i] = hcount[i];
 }
 
-// hist[b][cube] <- trials of the cube in blocks before b; count[cube] <- its total
-__global__ void cube_colscan_kernel(uint32_t* hist, uint32_t n_blocks, uint32_t n_cells, uint32_t* count) {
-    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= n_cells) return;
-    uint32_t run = 0;
-    for (uint32_t b = 0; b < n_blocks; ++b) {
-        const size_t i = (size_t)b * n_cells + cell;
-        const uint32_t v = hist[i];
-        hist[i] = run;
-        run += v;
+// hist[b][cube] <- trials of the cube in blocks before b; count[cube] <- its total.  One
+// workgroup of 256 threads per kColScanCells cubes (n_blocks <= 256): thread b holds block b's
+// counts of those cubes, and each cube's column is scanned across the four waves.
+__global__ __launch_bounds__(256) void cube_colscan_kernel(uint32_t* hist, uint32_t n_blocks, uint32_t n_cells,
+                                                           uint32_t* count) {
+    __shared__ int wtot[4][kColScanCells];
+    const uint32_t b = threadIdx.x, c0 = blockIdx.x * kColScanCells;
+    const int lane = b & 63, w = b >> 6;
+    int v[kColScanCells], incl[kColScanCells];
+#pragma unroll
+    for (int j = 0; j < kColScanCells; ++j) {
+        v[j] = (b < n_blocks && c0 + j < n_cells) ? (int)hist[(size_t)b * n_cells + c0 + j] : 0;
+        incl[j] = wave_inclusive_scan(v[j]);
+        if (lane == 63) wtot[w][j] = incl[j];
     }
-    count[cell] = run;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kColScanCells; ++j) {
+        if (c0 + j >= n_cells) continue;
+        int off = 0;
+        for (int q = 0; q < w; ++q) off += wtot[q][j];
+        if (b < n_blocks) hist[(size_t)b * n_cells + c0 + j] = (uint32_t)(off + incl[j] - v[j]);
+        if (b == 0) count[c0 + j] = (uint32_t)(wtot[0][j] + wtot[1][j] + wtot[2][j] + wtot[3][j]);
+    }
 }
 
-__global__ __launch_bounds__(kSortThreads) void cube_place_kernel(const WidomArgs a, const CellArgs c, uint64_t per,
-                                                                  uint32_t n_cells, const uint32_t* hist,
-                                                                  const uint32_t* start, uint32_t* order) {
+__global__ __launch_bounds__(kSortThreads) void cube_place_kernel(const WidomArgs a, uint64_t per, uint32_t n_cells,
+                                                                  const uint32_t* hist, const uint32_t* start,
+                                                                  const uint32_t* key, uint32_t* order) {
     extern __shared__ uint32_t hrank[];
     for (uint32_t i = threadIdx.x; i < n_cells; i += kSortThreads) hrank[i] = 0u;
     __syncthreads();
     const uint32_t* base = hist + (size_t)blockIdx.x * n_cells;
     const uint64_t lo = (uint64_t)blockIdx.x * per, hi = min(lo + per, a.n_trials);
     for (uint64_t t = lo + threadIdx.x; t < hi; t += kSortThreads) {
-        const uint32_t k = trial_cell(a, c, t);
+        const uint32_t k = key[t];
         order[start[k] + base[k] + atomicAdd(&hrank[k], 1u)] = (uint32_t)t;
     }
 }
@@ -407,12 +424,12 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
             const uint64_t per = (req.n_trials + sort_blocks - 1) / sort_blocks;
             const size_t lds = (size_t)n_cells * sizeof(uint32_t);
             hipLaunchKernelGGL(cube_hist_kernel, dim3(sort_blocks), dim3(kSortThreads), lds, ctx.stream, a, cg, per,
-                               n_cells, hist);
-            hipLaunchKernelGGL(cube_colscan_kernel, dim3((n_cells + 255) / 256), dim3(256), 0, ctx.stream, hist,
-                               sort_blocks, n_cells, count);
+                               n_cells, hist, key);
+            hipLaunchKernelGGL(cube_colscan_kernel, dim3((n_cells + kColScanCells - 1) / kColScanCells), dim3(256), 0,
+                               ctx.stream, hist, sort_blocks, n_cells, count);
             hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, ctx.stream, count, n_cells, start, cursor);
-            hipLaunchKernelGGL(cube_place_kernel, dim3(sort_blocks), dim3(kSortThreads), lds, ctx.stream, a, cg, per,
-                               n_cells, hist, start, order);
+            hipLaunchKernelGGL(cube_place_kernel, dim3(sort_blocks), dim3(kSortThreads), lds, ctx.stream, a, per,
+                               n_cells, hist, start, key, order);
         } else {
             hipLaunchKernelGGL(fill_u32_kernel, dim3((n_cells + 255) / 256), dim3(256), 0, ctx.stream, count,
                                (uint64_t)n_cells);
