@@ -317,7 +317,7 @@ void release(Framework* f) {
     (void)hipStreamSynchronize(c->stream);
     for (void* p : {(void*)f->d_fx, (void*)f->d_fy, (void*)f->d_fz, (void*)f->d_q, (void*)f->d_type,
                     (void*)f->d_kn, (void*)f->d_kw, (void*)f->d_kseg, (void*)f->d_sre, (void*)f->d_sim,
-                    (void*)f->d_widom_tab, (void*)f->d_nonfinite, (void*)f->d_widom_ticket})
+                    (void*)f->d_widom_tab, (void*)f->d_nonfinite})
         if (p) (void)hipFree(p);
     delete static_cast<aleator_framework*>(f);
     release(c);

@@ -159,7 +159,6 @@ struct Framework {
     bool lj_tail = false;
     // Widom's LJ constants on the device, kept while the inserted site species stay the same
     double* d_widom_tab = nullptr;
-    unsigned* d_widom_ticket = nullptr;  // Widom split kernel: workgroups finished (0 between calls)
     std::vector<uint32_t> widom_tab_species;
 };
 

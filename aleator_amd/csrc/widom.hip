@@ -383,14 +383,6 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
         fw.widom_tab_species = species_key;
     }
     a.ljtab = fw.d_widom_tab;
-    // the split kernel of an uncharged batch adds up its own workgroup sums (last workgroup done)
-    const bool fused_sum = !charged && !n_cells;
-    if (fused_sum && !fw.d_widom_ticket) {
-        ALEATOR_HIP_CHECK(hipMalloc(&fw.d_widom_ticket, sizeof(unsigned)));
-        ALEATOR_HIP_CHECK(hipMemsetAsync(fw.d_widom_ticket, 0, sizeof(unsigned), ctx.stream));
-    }
-    a.ticket = fused_sum ? fw.d_widom_ticket : nullptr;
-    a.sum_out = req.sum_device;
     a.block_sums = reinterpret_cast<double*>(base + tab_bytes);
     a.du = req.du_device ? req.du_device
                          : reinterpret_cast<double*>(base + tab_bytes + parts * sizeof(double));
@@ -448,9 +440,8 @@ void launch_widom(Framework& fw, const WidomRequest& req) {
         if (charged) launch_recip(a, fw, rec_blocks, rec_rt, ctx.stream);
     }
     ctx.time_end();
-    if (!fused_sum)
-        hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, ctx.stream, a.block_sums, (uint64_t)parts,
-                           req.sum_device);
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, ctx.stream, a.block_sums, (uint64_t)parts,
+                       req.sum_device);
     ALEATOR_HIP_CHECK(hipGetLastError());
 }
 

@@ -61,8 +61,6 @@ struct WidomArgs {
     double* du;           // [n_trials] (LJ + real in the direct-sum kernel when charged)
     double* block_sums;   // per workgroup sum of exp(-dU/T)
     int split;            // direct-sum kernel: threads sharing one trial's atoms (1, 2, 4, 8 or 16)
-    unsigned* ticket;     // split kernel, uncharged: workgroups finished (0 between calls), or null
-    double* sum_out;      // split kernel with ticket: the batch's sum of exp(-dU/T)
     double poly[kErfPolyMax + 1];  // erf(alpha r)/r in x = r^2 poly_xs - 1 (pair.hpp)
     double poly_xs;
     double tail_delta;  // analytic LJ tail of one insertion (0 unless the framework asks for it)
@@ -206,9 +204,7 @@ constexpr int split_tpt() { return NS == 1 ? 2 : 1; }
 // for pairs within a cutoff <= L_perp/2).  Slot k of a trial group is drawn by part k mod split
 // and shared through LDS.  Uncharged trials in a cubic cell work in fractional units: r_f^2 =
 // |f|^2 is compared with rc^2 / L^2 and the LJ constants carry the powers of L (A / L^12, B / L^6,
-// the slope times L).  With a.ticket set (uncharged batches), the last workgroup to finish also
-// adds the per-workgroup sums of exp(-dU/T) in workgroup order into *a.sum_out and clears the
-// ticket.
+// the slope times L).  Uncharged batches leave one sum of exp(-dU/T) per workgroup.
 template <bool EXACT, int NS, int DEG, int LJK>
 __global__ __launch_bounds__(1024) void widom_split_kernel(const WidomArgs a) {
     constexpr int NR = EXACT ? 2 : 1;
@@ -231,7 +227,6 @@ __global__ __launch_bounds__(1024) void widom_split_kernel(const WidomArgs a) {
     __shared__ double4 tab[NS][kMaxTypes];
     __shared__ double s_poly[DEG + 1];
     __shared__ double red[16];
-    __shared__ int s_last;
     const int tid = threadIdx.x, lane = tid & 63;
     const int geo = a.geo;
     const bool frac_units = !CH && geo == kGeoCubic;
@@ -404,28 +399,7 @@ __global__ __launch_bounds__(1024) void widom_split_kernel(const WidomArgs a) {
         }
     if constexpr (!CH) {
         const double bs = block_sum(boltz, red);
-        if (tid == 0) {
-            a.block_sums[blockIdx.x] = bs;
-            if (a.ticket) {
-                __threadfence();
-                s_last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
-            }
-        }
-        if (a.ticket) {
-            __syncthreads();
-            if (s_last) {
-                // every other workgroup's sum is visible (written before its ticket)
-                __threadfence();
-                double v = 0.0;
-                for (unsigned b = tid; b < gridDim.x; b += blockDim.x)
-                    v += __hip_atomic_load(a.block_sums + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                v = block_sum(v, red);
-                if (tid == 0) {
-                    *a.sum_out = v;
-                    *a.ticket = 0u;
-                }
-            }
-        }
+        if (tid == 0) a.block_sums[blockIdx.x] = bs;
     }
 }
 

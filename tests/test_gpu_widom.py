@@ -173,9 +173,8 @@ def test_widom_all_atom_path_for_large_molecules(ctx):
 def test_widom_split_kernel_c0_1e5(ctx):
     """c0's configured batch, 1e5 CH4 trials: the cube grid is rejected (the list sphere would
     cover more than 70 % of the cell), so the split kernel runs (two trials per lane, each trial's
-    atoms split over 8 waves, fractional units in the cubic cell, the batch sum added by the last
-    workgroup); its first 300 trials equal the oracle's in FP64 and meet the MIXED bar, and K_H
-    equals the oracle's over those 300."""
+    atoms split over 8 waves, fractional units in the cubic cell); its first 300 trials equal the
+    oracle's in FP64 and meet the MIXED bar, and K_H equals the oracle's over those 300."""
     print("widom c0 1e5 trials, split kernel", flush=True)
     fw, lj = ol.c0_system()
     dev = device_framework(ctx, fw, lj)
@@ -184,7 +183,7 @@ def test_widom_split_kernel_c0_1e5(ctx):
     assert abs(s300 / 300 / 298.0 - kh_ref) <= 1e-10 * abs(kh_ref)
     s, du = dev.widom(CH4, 100_000, 298.0, seed=1, stream=1, precision=aa.FP64)
     assert rel_err(du[:300], du_ref).max() <= 1e-10
-    # the batch sum that the last workgroup adds up, against the per-trial energies (K_H =
+    # the batch sum, against the per-trial energies (K_H =
     # <exp(-dU/T)> / T, monte_carlo_engine.cc:880-916), and again on a second call
     s_host = np.sum(np.exp(-du / 298.0))
     assert abs(s - s_host) <= 1e-12 * abs(s_host)
