@@ -293,6 +293,22 @@ void gpu_checks() {
             orc_gcmc_free(&og);
         }
         expect(same, "GcmcChains with a FrameworkEnergyGrid follows the oracle's grid-mode trajectory");
+        // a chain saved and loaded into a new chain set takes the same steps; uncharged chains never resync
+        const auto rec = chains.saveChain(1);
+        ag::GcmcChains copy(fw, ch4, {cfg[1]}, 1.5, 0.5, 256, &grid);
+        copy.loadChain(0, rec);
+        chains.run(500);
+        copy.run(500);
+        const auto a1 = chains.sitePositions(1), b1 = copy.sitePositions(0);
+        bool resumed = a1.size() == b1.size() && !a1.empty();
+        for (size_t m = 0; m < a1.size() && resumed; ++m) resumed &= a1[m] == b1[m];
+        expect(resumed, "GcmcChains::saveChain / loadChain: the loaded chain continues step for step");
+        const auto drift = chains.maxResyncDrift();
+        expect(drift.size() == cfg.size() && drift[0] == 0.0 && drift[1] == 0.0,
+               "GcmcChains::maxResyncDrift: uncharged chains report no drift");
+        expect(throws<std::invalid_argument>([&] { copy.loadChain(0, std::vector<unsigned char>(rec.begin(), rec.end() - 8)); }),
+               "GcmcChains::loadChain: a record of the wrong size -> invalid_argument");
+        expect(fw.nonfiniteCount() == 0, "Framework::nonfiniteCount: a clean framework's grid writes no NaN / Inf");
         ag::FrameworkEnergyGrid other(dev, s.lj, s.p, s.lat, {5}, 0.5);
         expect(throws<std::out_of_range>([&] { ag::GcmcChains(fw, ch4, cfg, 1.5, 0.5, 256, &other); }),
                "GcmcChains: grid without the site's species -> out_of_range");
