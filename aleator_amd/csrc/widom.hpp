@@ -408,9 +408,10 @@ __global__ __launch_bounds__(1024) void widom_split_kernel(const WidomArgs a) {
 // workgroup's trials per occupied cube).  One workgroup per cube builds, as the grid tile kernel
 // does, an LDS list of the framework images within R = max cutoff + cube half-diagonal +
 // molecule radius of the cube centre (atom order, block scan), and each thread sums TPT trials'
-// sites against it.  Every image within the cutoff of a site lies in the list, and at most one
-// image of an atom does (cutoff <= L_perp/2), so the sum is the reference's minimum-image sum,
-// taken in atom order per trial.
+// sites against it (with one trial per thread, a cube of at most 128 or 64 trials splits each
+// trial's list between 2 or 4 threads).  Every image within the cutoff of a site lies in the list,
+// and at most one image of an atom does (cutoff <= L_perp/2), so the sum is the reference's
+// minimum-image sum, taken in atom order per trial (per part, with the parts added in order).
 template <bool EXACT, int NS, int DEG, int LJK, int TPT>
 __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a, const CellArgs c) {
     constexpr int NR = EXACT ? 2 : 1;
@@ -421,6 +422,7 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
     __shared__ double4 tab[NS][kMaxTypes];
     __shared__ double s_poly[DEG + 1];
     __shared__ int scan_tmp[kWThreads / 64 + 1];
+    __shared__ double part_lj[TPT == 1 ? kWThreads : 1], part_re[TPT == 1 && CH ? kWThreads : 1];
     const int tid = threadIdx.x;
     const int cell = blockIdx.x;
     const uint32_t lo = c.start[cell], hi = c.start[cell + 1];
@@ -444,14 +446,21 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
     for (int s = 0; s < NS; ++s) qs[s] = a.charge[s];
     const double rc2 = a.rc2_lj, rcr2 = a.rc2_real;
     const bool one = !CH || rc2 == rcr2;
-    for (uint32_t pass = lo; pass < hi; pass += kWThreads * TPT) {
+    // a sparsely filled cube (one trial per thread) shares each trial's list among sp threads:
+    // slot tid % slots, part tid / slots takes entries part, part + sp, ... (a part spans whole
+    // waves), and the parts meet in LDS in a fixed order
+    const uint32_t n_cube = hi - lo;
+    const int sp = TPT == 1 ? (n_cube <= 64 ? 4 : (n_cube <= 128 ? 2 : 1)) : 1;
+    const int slots = kWThreads / sp;
+    const int slot = tid % slots, part = tid / slots;
+    for (uint32_t pass = lo; pass < hi; pass += slots * TPT) {
         uint64_t tr[TPT];
         bool ok[TPT];
         double ox[TPT][NS], oy[TPT][NS], oz[TPT][NS];
         double lj[TPT], real[TPT];
 #pragma unroll
         for (int k = 0; k < TPT; ++k) {
-            const uint32_t idx = pass + k * kWThreads + tid;
+            const uint32_t idx = pass + k * slots + slot;
             ok[k] = idx < hi;
             tr[k] = ok[k] ? c.order[idx] : 0;
             lj[k] = 0.0;
@@ -474,7 +483,7 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
             if (any) {
                 auto pairs = [&](auto one_tag) {
                     constexpr bool ONE = decltype(one_tag)::value;
-                    for (int e = 0; e < count; ++e) {
+                    for (int e = part; e < count; e += sp) {
                         const double2 xy = e_xy[e], zq = e_zq[e];
                         const int ty = e_t[e];
 #pragma unroll
@@ -555,9 +564,20 @@ __global__ __launch_bounds__(kWThreads) void widom_cell_kernel(const WidomArgs a
             pending += total;
         }
         compute(pending);
+        if (sp > 1) {  // uniform over the workgroup
+            part_lj[tid] = lj[0];
+            if constexpr (CH) part_re[tid] = real[0];
+            __syncthreads();
+            if (part == 0)
+                for (int p = 1; p < sp; ++p) {
+                    lj[0] += part_lj[p * slots + slot];
+                    if constexpr (CH) real[0] += part_re[p * slots + slot];
+                }
+            __syncthreads();
+        }
 #pragma unroll
         for (int k = 0; k < TPT; ++k)
-            if (ok[k]) a.du[tr[k]] = (CH ? lj[k] + a.kc * real[k] : lj[k]) + a.tail_delta;
+            if (ok[k] && part == 0) a.du[tr[k]] = (CH ? lj[k] + a.kc * real[k] : lj[k]) + a.tail_delta;
     }
 }
 
