@@ -25,7 +25,6 @@ namespace aleator_gpu {
 
 namespace {
 
-constexpr int kChunkSmall = 128;  // atoms per list pass when an atom may add 8 images
 
 // T1[n0][n1][iz] = sum_n2 C[n0][n1][n2] * E2[n2][iz]
 __global__ void recip_stage_a(const double2* c, const double2* e2, int nk, int nz, double2* t1) {

@@ -8,6 +8,7 @@
 namespace aleator_gpu {
 
 constexpr int kNumTileShapes = 4;
+constexpr int kChunkSmall = 128;  // atoms per list pass when an atom may add 8 images
 constexpr int kFarDrop = 7;  // the shell entries' erf polynomial is this many degrees lower
 
 // Everything one framework's tile kernel reads; stored in device memory, one per framework.

@@ -42,7 +42,8 @@ void plan_bind_shape(aleator_grid_plan* p, int shape, int chunk) {
     std::vector<GridArgs> descs;
     std::vector<int> prefix(1, 0);
     for (auto& j : p->jobs) {
-        j.set_shape(shape, chunk);
+        // frameworks that may list several images of an atom keep 128 atoms per pass
+        j.set_shape(shape, j.chunk_fits(shape, chunk) ? chunk : kChunkSmall);
         descs.push_back(j.args);
         prefix.push_back(prefix.back() + j.blocks);
     }
@@ -138,14 +139,14 @@ aleator_status aleator_gpu_grid_plan_create(aleator_ctx* ctx_in, int32_t n_frame
             }
             require(!shapes.empty(), ALEATOR_INVALID_ARGUMENT,
                     "grid: cutoff plus tile radius exceeds the perpendicular width");
-            // atoms per list-build pass allowed with a shape: 128 always, 256, 512 and 1024 when
-            // every framework keeps one image per atom at that shape
+            // atoms per list-build pass with a shape: 128, and 256, 512 and 1024 when some framework
+            // keeps one image per atom at that shape (the frameworks that do not keep 128)
             auto chunks_for = [&](int s) {
-                std::vector<int> c{128};
+                std::vector<int> c{kChunkSmall};
                 for (int k : {256, 512, 1024}) {
-                    bool ok = true;
-                    for (const auto& j : p->jobs) ok = ok && j.chunk_fits(s, k);
-                    if (ok) c.push_back(k);
+                    bool any = false;
+                    for (const auto& j : p->jobs) any = any || j.chunk_fits(s, k);
+                    if (any) c.push_back(k);
                 }
                 return c;
             };

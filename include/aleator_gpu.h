@@ -210,7 +210,8 @@ aleator_status aleator_gpu_grid_plan_execute(aleator_grid_plan* plan);
  * 4 floats. */
 aleator_status aleator_gpu_grid_plan_tile(const aleator_grid_plan* plan, int32_t* tile, float* tuning_ms);
 /* Framework atoms staged per pass of the tile kernel's LDS list build, tuned with the shape
- * (128; 256, 512 or 1024 when at most one image of an atom can be listed). */
+ * (128; 256, 512 or 1024 for the frameworks of which at most one image of an atom can be
+ * listed, the others keeping 128). */
 aleator_status aleator_gpu_grid_plan_atoms_per_pass(const aleator_grid_plan* plan, int32_t* atoms);
 aleator_status aleator_gpu_grid_plan_destroy(aleator_grid_plan* plan);
 
