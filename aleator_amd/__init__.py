@@ -616,6 +616,12 @@ class GridPlan:
         _check(_load().aleator_gpu_grid_plan_atoms_per_pass(self._h, C.byref(n)))
         return n.value
 
+    def erf_degree(self) -> tuple:
+        """Degrees of the Coulomb erf polynomials: (inner entries, shell entries); 0 without one."""
+        d, f = C.c_int32(), C.c_int32()
+        _check(_load().aleator_gpu_grid_plan_erf_degree(self._h, C.byref(d), C.byref(f)))
+        return d.value, f.value
+
     def close(self):
         if self._h:
             _load().aleator_gpu_grid_plan_destroy(self._h)

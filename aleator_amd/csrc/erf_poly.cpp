@@ -34,19 +34,32 @@ long double g_kernel(long double u) {
 
 namespace {
 
+constexpr int kN = 64;
+
+// cos(pi j (k + 1/2) / 64) for j, k < 64, computed once (row 1 holds the Chebyshev points)
+struct CosTable {
+    long double c[kN][kN];
+    CosTable() {
+        constexpr long double kPiL = 3.141592653589793238462643383279502884L;
+        for (int j = 0; j < kN; ++j)
+            for (int k = 0; k < kN; ++k) c[j][k] = cosl(kPiL * j * (k + 0.5L) / kN);
+    }
+};
+
+const CosTable& cos_table() {
+    static const CosTable table;  // thread-safe one-time initialisation
+    return table;
+}
+
 // Chebyshev coefficients of G(alpha^2 s) for s in [smin, smax] (64 points, long double)
 void cheb_fit(double alpha, long double smin, long double smax, long double* cheb) {
-    constexpr int kN = 64;
-    constexpr long double kPiL = 3.141592653589793238462643383279502884L;
+    const CosTable& t = cos_table();
     const long double a2 = (long double)alpha * alpha;
     long double f[kN];
-    for (int k = 0; k < kN; ++k) {
-        const long double x = cosl(kPiL * (k + 0.5L) / kN);
-        f[k] = g_kernel(a2 * (smin + (x + 1.0L) * 0.5L * (smax - smin)));
-    }
+    for (int k = 0; k < kN; ++k) f[k] = g_kernel(a2 * (smin + (t.c[1][k] + 1.0L) * 0.5L * (smax - smin)));
     for (int j = 0; j < kN; ++j) {
         long double s = 0.0L;
-        for (int k = 0; k < kN; ++k) s += f[k] * cosl(kPiL * j * (k + 0.5L) / kN);
+        for (int k = 0; k < kN; ++k) s += f[k] * t.c[j][k];
         cheb[j] = 2.0L * s / kN;
     }
     cheb[0] *= 0.5L;
@@ -76,7 +89,7 @@ void to_monomials(double alpha, const long double* cheb, int deg, double* coef) 
 
 long double tail_after(const long double* cheb, int deg) {
     long double tail = 0.0L;
-    for (int j = deg + 1; j < 64; ++j) tail += fabsl(cheb[j]);
+    for (int j = deg + 1; j < kN; ++j) tail += fabsl(cheb[j]);
     return tail;
 }
 
@@ -85,7 +98,7 @@ long double tail_after(const long double* cheb, int deg) {
 bool erf_kernel_poly_range(double alpha, double rmin, double rmax, double tol, int deg, double* coef, double* xs,
                            double* xo) {
     const long double smin = (long double)rmin * rmin, smax = (long double)rmax * rmax;
-    long double cheb[64];
+    long double cheb[kN];
     cheb_fit(alpha, smin, smax, cheb);
     to_monomials(alpha, cheb, deg, coef);
     *xs = (double)(2.0L / (smax - smin));
@@ -95,7 +108,7 @@ bool erf_kernel_poly_range(double alpha, double rmin, double rmax, double tol, i
 
 int erf_kernel_poly(double alpha, double rmax, double tol, int min_deg, double* coef, double* xs) {
     const long double smax = (long double)rmax * rmax;
-    long double cheb[64];
+    long double cheb[kN];
     cheb_fit(alpha, 0.0L, smax, cheb);
     // smallest allowed degree (at least min_deg) whose dropped coefficients sum below tol
     static const int kDegrees[] = {18, 24, kErfPolyMax};

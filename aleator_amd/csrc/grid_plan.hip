@@ -250,6 +250,17 @@ aleator_status aleator_gpu_grid_plan_atoms_per_pass(const aleator_grid_plan* p, 
     });
 }
 
+aleator_status aleator_gpu_grid_plan_erf_degree(const aleator_grid_plan* p, int32_t* degree, int32_t* shell_degree) {
+    return guarded([&] {
+        require(p && degree && shell_degree, ALEATOR_INVALID_ARGUMENT, "null argument");
+        const int deg = p->jobs.empty() ? 0 : p->jobs[0].deg;
+        bool split = false;
+        for (const auto& j : p->jobs) split = split || j.args.far_ok;
+        *degree = deg;
+        *shell_degree = split ? deg - kFarDrop : deg;
+    });
+}
+
 aleator_status aleator_gpu_grid_plan_destroy(aleator_grid_plan* p) {
     return guarded([&] {
         if (!p) return;

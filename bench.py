@@ -667,12 +667,13 @@ def main():
     value = 3 * nodes / t_step
 
     tables_main = out.cpu().numpy().reshape(3, nodes)
-    # the erf kernel polynomial degree the build used (same fit and rule as the library's)
-    erf_deg = len(aa.erf_kernel_poly(w["alpha"], w["rc"], 1e-12 if prec == aa.MIXED else 1e-17)[0]) - 1
+    # the erf kernel polynomial degree the build used (the tile kernel's template argument)
+    erf_deg = plan.erf_degree()[0]
 
     # ---------------- extras
     extra = {"nodes_per_s": nodes / t_step, "tile_kernel_ms": k_avg * 1e3, "pairs_per_node": nbar,
-             "tile_shape": tile, "tile_atoms_per_pass": plan.atoms_per_pass(), "autotune_ms_per_shape": tuning}
+             "tile_shape": tile, "tile_atoms_per_pass": plan.atoms_per_pass(), "autotune_ms_per_shape": tuning,
+             "erf_degree_inner_shell": list(plan.erf_degree())}
     # the same grid in the other precision (FP64 beside the MIXED headline), same plan machinery;
     # profiling runs (--headline-only) leave it out so that their counters cover the headline alone
     tables_mixed = tables_fp64 = None

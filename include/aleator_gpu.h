@@ -213,6 +213,10 @@ aleator_status aleator_gpu_grid_plan_tile(const aleator_grid_plan* plan, int32_t
  * (128; 256, 512 or 1024 for the frameworks of which at most one image of an atom can be
  * listed, the others keeping 128). */
 aleator_status aleator_gpu_grid_plan_atoms_per_pass(const aleator_grid_plan* plan, int32_t* atoms);
+/* No reference counterpart (profiling): degrees of the erf(alpha r)/r polynomials of the plan's
+ * Coulomb tables (0 without one): `degree` for the list entries inside both cutoffs for every
+ * node of a tile, `shell_degree` for the others (equal to `degree` when the list is not split). */
+aleator_status aleator_gpu_grid_plan_erf_degree(const aleator_grid_plan* plan, int32_t* degree, int32_t* shell_degree);
 aleator_status aleator_gpu_grid_plan_destroy(aleator_grid_plan* plan);
 
 /* ---- energy-grid cache file, format v1 -------------------------------------------------

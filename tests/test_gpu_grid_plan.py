@@ -101,3 +101,29 @@ def test_autotune_picks_a_fitting_shape(ctx):
     got = buf.to_host()
     ref = dev.grid([4], counts, precision=aa.MIXED)[0].ravel()
     assert rel_err(got, ref).max() <= 1e-6
+
+
+def test_erf_degrees_of_the_shell_split_on_c1(ctx):
+    """c1 (alpha 0.265, rc 12 A): in MIXED (tolerance 1e-12 on the dropped Chebyshev terms) the
+    entries inside both cutoffs take the degree-18 erf polynomial and the shell entries one of
+    degree 11 fitted over their own distances; FP64 (1e-17) takes 24 and 17.  Both plans agree with each other to the MIXED bar
+    (1e-5 relative where |E| > 1 K) on every node, the Coulomb table included."""
+    fw, lj = ol.c1_system()
+    sc = ol.replicate(fw, ol.min_supercell(fw.m, 12.0))
+    counts = ol.grid_counts(fw.m, 0.15)
+    dev = device_framework(ctx, sc, lj, ewald=(0.265, 12.0, 8))
+    n = int(np.prod(counts))
+    tabs = {}
+    for prec, want in ((aa.MIXED, (18, 11)), (aa.FP64, (24, 17))):
+        buf = aa.DeviceBuffer(3 * n)
+        plan = aa.GridPlan(ctx, [dev], [4, 5], [counts], [int(buf)], coulomb=True, precision=prec,
+                           grid_lattices=[fw.m], tile=1)
+        assert plan.erf_degree() == want
+        plan.execute()
+        tabs[prec] = buf.to_host().reshape(3, n)
+        plan.close()
+    for t in range(3):
+        ref = tabs[aa.FP64][t]
+        sel = (np.abs(ref) > 1.0) & (ref < 1e5)
+        err = np.abs(tabs[aa.MIXED][t][sel] - ref[sel]) / np.abs(ref[sel])
+        assert err.max() <= 1e-5, (t, err.max())
